@@ -1,0 +1,209 @@
+"""Benchmark of the MI355X pseudo-spectral stepper (one GPU).
+
+Workload: the 256^3 FP64 plain stepper on the perturbed-interface initial
+state (interface height Lz/2 + 0.5 cos(2 pi x/Lx) cos(2 pi y/Ly), density
+1.0/0.8, 1e-3 seeded noise, dt = 0.01), with the diagnostics every 50 steps
+as the reference's loop does.  The FP32-mixed stepper on the same state is
+timed beside it.  A "step" is one iteration of the reference's loop body
+(smaiso-wcomp.cpp:1081-2198).
+
+Prints one JSON line (rank 0, N=1).  The timed region is bracketed by a
+device-wide synchronisation of the HIP runtime libsmaiso.so uses (torch's
+ROCm wheel carries a separate copy of the runtime that cannot see this
+library's stream, so torch.cuda.synchronize() would not be a barrier here).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "smaiso-wcomp_amd"))
+
+METRIC = "steps/s at 256^3: FP64 plain vs FP32-mixed @ ≤1e-6 rel L2; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic field passes per step on the canonical dataflow (30 transforms
+# counted as 1 read + 1 write each, fused single-operand multipliers free):
+# 133 passes; FP64 moves 8 B per pass, FP32-mixed 126 passes at 4 B + 7 at 8 B.
+BYTES_PER_POINT = {"f64": 133 * 8, "mixed": 126 * 4 + 7 * 8}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def make_params(S, n, precision):
+    return S.Params(nx=n, ny=n, nz=n, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0,
+                    precision=precision)
+
+
+def time_stepper(S, params, psi, rho, steps, warmup, cadence=50):
+    st = S.Stepper(params)
+    st.set_state(psi, rho)
+    st.step(warmup)
+    S.device_synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    diag = None
+    while done < steps:
+        chunk = min(cadence, steps - done)
+        st.step(chunk)
+        done += chunk
+        if chunk == cadence:
+            diag = st.diagnostics(allow_nonfinite=True)
+    S.device_synchronize()
+    t1 = time.perf_counter()
+    return st, (t1 - t0), diag
+
+
+def kernel_timing(st, nsteps):
+    """Per-kernel-class HIP-event timing over nsteps (events on the stepper's stream)."""
+    st.timing(True)
+    st.step(nsteps)
+    res = {}
+    for kc in ("r2r_z", "r2r_y", "r2r_x", "spectral", "pointwise"):
+        ms, n, by = st.timing_get(kc)
+        res[kc] = dict(total_ms=ms, launches=n, bytes=by)
+    st.timing(False)
+    return res
+
+
+def cpu_baseline(n, psi, rho, seconds_budget=20.0):
+    """The FP64 oracle (numpy + scipy.fft) on the host on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["SMAISO_ORACLE_THREADS"] = str(threads)
+    import smaiso_oracle as O
+    O._WORKERS = threads
+    op = O.Params(nx=n, ny=n, nz=n, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0)
+    o = O.Oracle(op)
+    o.set_state(psi, rho)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.step(1)
+        k += 1
+        if time.perf_counter() - t0 > seconds_budget * 0.6 or k >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=k / dt, unit="steps/s", cores=threads, kind="port",
+                sample=f"{k} FP64 steps of the numpy/scipy.fft oracle at {n}^3 from the same initial state "
+                       f"(scipy.fft workers={threads}, numpy elementwise single-threaded; "
+                       f"host {cpu_model()}, nproc={os.cpu_count()})")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import smaiso_amd as S
+
+    n = args.n
+    log(f"[bench] building IC {n}^3")
+    p64 = make_params(S, n, S.FP64)
+    ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=0.8, rho_sm=1.0, undulation=0.5,
+                       noise=1e-3, seed=2)
+    psi, rho = S.make_interface_ic(p64, ic)
+
+    log(f"[bench] FP64: warmup {args.warmup}, timed {args.steps}")
+    st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup)
+    sps64 = args.steps / sec64
+    log(f"[bench] FP64 {sps64:.2f} steps/s, diag {diag64}")
+    kt64 = kernel_timing(st64, args.profile_steps)
+    del st64
+
+    mixed = None
+    if not args.no_mixed:
+        pm = make_params(S, n, S.FP32_MIXED)
+        log(f"[bench] FP32-mixed: warmup {args.warmup}, timed {args.steps}")
+        stm, secm, diagm = time_stepper(S, pm, psi, rho, args.steps, args.warmup)
+        spsm = args.steps / secm
+        ktm = kernel_timing(stm, args.profile_steps)
+        del stm
+        log(f"[bench] mixed {spsm:.2f} steps/s, diag {diagm}")
+        mixed = dict(value=spsm, ms_per_step=1e3 / spsm, speedup_vs_fp64=spsm / sps64,
+                     step_roofline_frac=BYTES_PER_POINT["mixed"] * n ** 3 * spsm / (HBM_PEAK_GBS * 1e9),
+                     kernels={k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
+                                      launches_per_step=v["launches"] / args.profile_steps,
+                                      gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
+                              for k, v in ktm.items()})
+
+    # dominant kernel class of the FP64 stepper
+    dom = max(kt64, key=lambda k: kt64[k]["total_ms"])
+    d = kt64[dom]
+    avg_ms = d["total_ms"] / max(d["launches"], 1)
+    bytes_per_launch = d["bytes"] / max(d["launches"], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None, kernel=dom,
+                    avg_launch_ms=avg_ms, algorithmic_bytes_per_launch=bytes_per_launch)
+    step_bytes = BYTES_PER_POINT["f64"] * n ** 3
+    step_roof = dict(algorithmic_bytes_per_step=step_bytes,
+                     achieved=step_bytes * sps64 / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=step_bytes * sps64 / (HBM_PEAK_GBS * 1e9))
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(n, psi, rho)
+        log(f"[bench] CPU {cpu['value']:.4f} steps/s")
+
+    out = {
+        "metric": METRIC,
+        "value": sps64,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / sps64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded perturbed-interface initial state built on the host",
+        "config": {"workload": f"plain FP64 stepper, {n}^3 grid, interface h=Lz/2+0.5cos(2pi x/Lx)cos(2pi y/Ly), "
+                               f"rho 1.0/0.8, dt=0.01, rho_m=1, diagnostics every 50 steps",
+                   "grid": [n, n, n], "precision": "fp64", "dt": 0.01, "rho_m": 1.0},
+        "roofline": roofline,
+        "step_roofline": step_roof,
+        "kernels_fp64": {k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
+                                 launches_per_step=v["launches"] / args.profile_steps,
+                                 gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
+                         for k, v in kt64.items()},
+        "fp32_mixed": mixed,
+        "cpu_baseline": cpu,
+        "diagnostics_fp64": diag64,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

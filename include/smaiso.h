@@ -1,0 +1,222 @@
+/*
+ * smaiso.h - C ABI of the MI355X pseudo-spectral stepper for the weakly
+ * compressible smectic-A / isotropic phase-field model (order parameter psi,
+ * density rho, quasi-static velocity v).
+ *
+ * The reference (smaiso-wcomp.cpp, cited as S:) has no library API: its hot
+ * path is the body of main()'s loop `while (L1 > limL1)` (S:1081-2198), and
+ * its only operator interface is FFTW's r2r plan/execute pair (S:499-545,
+ * S:2483-2492).  Each entry point below names the reference code it stands
+ * in for.
+ *
+ * Conventions
+ *  - Plain C types only.  Status: 0 = ok, < 0 = error (SMAISO_E_*).  The text
+ *    of the last error is returned by smaiso_last_error().
+ *  - Host arrays use the reference layout: index (i*Ny + j)*Nz + k with k
+ *    fastest (S:943), FP64.
+ *  - The caller owns host buffers; the library owns device memory.  One handle
+ *    owns one HIP stream; a handle is not thread-safe, distinct handles are
+ *    independent.
+ */
+#ifndef SMAISO_H
+#define SMAISO_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMAISO_ABI_VERSION 1
+
+/* status codes */
+#define SMAISO_OK 0
+#define SMAISO_E_INVALID (-1)   /* bad argument or parameter */
+#define SMAISO_E_HIP (-2)       /* HIP runtime error */
+#define SMAISO_E_NOMEM (-3)     /* device allocation failed */
+#define SMAISO_E_STATE (-4)     /* call out of order (e.g. step before set_state) */
+#define SMAISO_E_NONFINITE (-5) /* diagnostics found NaN/Inf in the state */
+
+/* precision of the stepper */
+#define SMAISO_FP64 0        /* every field and transform in FP64 */
+#define SMAISO_FP32_MIXED 1  /* FP32 transforms/pointwise; FP64 rho, f_rho, reductions, denominators */
+
+/* FFTW r2r kind numbering used by smaiso_r2r3d (S:499-545) */
+#define SMAISO_REDFT10 0  /* DCT-II  */
+#define SMAISO_REDFT01 1  /* DCT-III */
+#define SMAISO_RODFT10 2  /* DST-II  */
+#define SMAISO_RODFT01 3  /* DST-III */
+
+/* fields readable with smaiso_get_field (real space unless noted) */
+#define SMAISO_FIELD_PSI 0
+#define SMAISO_FIELD_RHO 1
+#define SMAISO_FIELD_AMP 2        /* Gaussian-smoothed amplitude, S:1254-1289 */
+#define SMAISO_FIELD_VX 3         /* total velocity vsol + virr, S:2125-2131 */
+#define SMAISO_FIELD_VY 4
+#define SMAISO_FIELD_VZ 5
+#define SMAISO_FIELD_VSOLX 6      /* solenoidal part, S:2094-2104 */
+#define SMAISO_FIELD_VSOLY 7
+#define SMAISO_FIELD_VSOLZ 8
+#define SMAISO_FIELD_VIRRX 9      /* irrotational part, S:2106-2116 */
+#define SMAISO_FIELD_VIRRY 10
+#define SMAISO_FIELD_VIRRZ 11
+#define SMAISO_FIELD_DIVV 12      /* div v, S:2085-2088 */
+#define SMAISO_FIELD_LPSI 13      /* alpha (q0^2 + lap) psi at the step start, S:1293-1308 */
+#define SMAISO_FIELD_MU 14        /* chemical potential, S:1411-1413 */
+#define SMAISO_FIELD_ENERGY 15    /* free-energy density of the diagnostics, S:2227-2233 */
+#define SMAISO_FIELD_PSIQ 16      /* normalised DCT coefficients of psi (spectral) */
+#define SMAISO_FIELD_PSIQ_OLD 17  /* previous psiq (spectral) */
+#define SMAISO_FIELD_NLQ 18       /* DCT of the nonlinear term (spectral, unnormalised) */
+#define SMAISO_FIELD_F_RHO 19     /* AB2 history of the density, S:2136 */
+#define SMAISO_FIELD_DIVF 20      /* div f, S:2090-2092 */
+#define SMAISO_FIELD_GPSIX 21     /* grad psi, S:1226-1236 */
+#define SMAISO_FIELD_GPSIY 22
+#define SMAISO_FIELD_GPSIZ 23
+#define SMAISO_FIELD_GRHOX 24     /* grad rho, S:1240-1250 */
+#define SMAISO_FIELD_GRHOY 25
+#define SMAISO_FIELD_GRHOZ 26
+#define SMAISO_FIELD_LAPRHOL 27   /* lap(rho L psi), S:1312-1380 */
+#define SMAISO_FIELD_PSI_OLD 28   /* psi at the start of the last step */
+#define SMAISO_FIELD_COUNT 29
+
+/*
+ * Parameters (S:206-283).  smaiso_params_default() fills the reference's
+ * constants for a 512^3 grid; the caller then edits what it needs.
+ * Derived quantities follow the reference: Amp = sqrt((3 beta +
+ * sqrt(9 beta^2 + 40 epsilon gamma))/5) (S:250), kp = (1 - rho0)/Amp (S:252),
+ * rho_s = kp Amp + rho0 (S:253), scale = 1/(8 Nx Ny Nz) (S:278).
+ */
+typedef struct smaiso_params {
+  int32_t nx, ny, nz;        /* grid (S:218) */
+  double dx, dy, dz;         /* spacing, 2 pi/(q0 Nw) in the reference (S:267-272) */
+  double dt;                 /* time step (S:262) */
+  double alpha, beta, gamma; /* phase-field coefficients (S:239-241) */
+  double epsilon;            /* -0.001*ep_arg (S:242-243) */
+  double q0;                 /* layer wavenumber (S:244) */
+  double nu;                 /* viscosity, argv[1] (S:249) */
+  double lambda;             /* second viscosity (S:256) */
+  double zeta;               /* density coupling (S:257) */
+  double rho0;               /* isotropic density rho_0 (S:251) */
+  double rho_m;              /* stabilisation density in C1, C2 (S:255: rho_s/2); <= 0 means rho_s/2 */
+  int32_t divv_switch;       /* relaxation steps with rho slaved to amp (S:214) */
+  int32_t load_mode;         /* 1 = restart semantics: skip relaxation (S:1312) */
+  int32_t precision;         /* SMAISO_FP64 or SMAISO_FP32_MIXED */
+  int32_t slab_n0;           /* reference slab height for quirk Q3 (S:1932); 0 = Nx */
+} smaiso_params;
+
+/* Interface initial condition (builder-defined, see smaiso_make_interface_ic). */
+typedef struct smaiso_ic {
+  double amplitude;   /* A in psi = A cos(q0 z) * profile */
+  double h0;          /* mean interface height; < 0 means Lz/2 */
+  double width;       /* tanh width w */
+  double rho_iso;     /* density above the interface */
+  double rho_sm;      /* density below the interface */
+  double undulation;  /* u in h = h0 + u cos(2 pi x/Lx) cos(2 pi y/Ly) */
+  double noise;       /* uniform noise amplitude on psi */
+  uint64_t seed;      /* splitmix64 seed */
+} smaiso_ic;
+
+typedef struct smaiso_handle smaiso_handle;
+
+/* Library ABI version (SMAISO_ABI_VERSION). No reference counterpart. */
+int smaiso_abi_version(void);
+
+/* Fill the reference's constants (S:206-283, argv defaults nu=1, ep_arg=5,
+ * 512^3 grid). */
+void smaiso_params_default(smaiso_params* p);
+
+/* Replaces the reference's setup S:293-545 and the operator tables S:936-963:
+ * validates p, allocates device fields on the current HIP device and builds
+ * the transform tables.  Returns SMAISO_E_INVALID for N < 2, dt <= 0, nu <= 0,
+ * lambda + 2 nu <= 0, non-finite parameters or an unknown precision. */
+int smaiso_create(const smaiso_params* p, smaiso_handle** out);
+
+/* Replaces fftw_destroy_plan/MPI::Finalize (S:2483-2496). NULL is a no-op. */
+int smaiso_destroy(smaiso_handle* h);
+
+/* Text of the last error on h (or of the last failed smaiso_create when h is
+ * NULL).  Never NULL. */
+const char* smaiso_last_error(const smaiso_handle* h);
+
+/* Replaces the IC / load block and the pre-loop (S:556-796, S:965-1010):
+ * psiq = scale*DCT(psi), psi = iDCT(psiq), psiq_old = psiq, f_rho = 0 and the
+ * step counter reset (quirk Q5).  psi, rho: host FP64 arrays of Nx*Ny*Nz. */
+int smaiso_set_state(smaiso_handle* h, const double* psi, const double* rho);
+
+/* As smaiso_set_state with device FP64 arrays (already in HBM). */
+int smaiso_set_state_device(smaiso_handle* h, const void* psi_dev, const void* rho_dev);
+
+/* Replaces nsteps iterations of the loop body S:1081-2198.  Synchronous. */
+int smaiso_step(smaiso_handle* h, int32_t nsteps);
+
+/* Enqueue nsteps on the handle's stream without waiting. */
+int smaiso_step_async(smaiso_handle* h, int32_t nsteps);
+
+/* Wait for the handle's stream. */
+int smaiso_synchronize(smaiso_handle* h);
+
+/* Steps completed since the last set_state (the reference's nLoop). */
+int64_t smaiso_step_count(const smaiso_handle* h);
+
+/* Copy a field (SMAISO_FIELD_*) of the last completed step to host FP64 out
+ * (Nx*Ny*Nz values, reference layout).  Replaces reading the reference's
+ * arrays (psi_local, rho_local, ...) and its slice writers (S:2323-2446). */
+int smaiso_get_field(smaiso_handle* h, int32_t field, double* out);
+
+/* As smaiso_get_field into a device FP64 buffer. */
+int smaiso_get_field_device(smaiso_handle* h, int32_t field, void* out_dev);
+
+/* Diagnostics of the last completed step, S:2202-2263: L1 = sum|psi-psi_old|/
+ * (dt sum|psi|), mass = sum(rho) dx dy dz, free energy (S:2227-2233).  FP64
+ * compensated sums on the device.  Returns SMAISO_E_NONFINITE (values still
+ * written) when any of them is NaN or Inf (the reference's loop silently
+ * exits in that case, S:1081). */
+int smaiso_diagnostics(smaiso_handle* h, double* l1, double* mass, double* energy);
+
+/* Standalone 3-D r2r transform with FFTW semantics (unnormalised), the
+ * equivalent of fftw_mpi_plan_r2r_3d + fftw_execute (S:499-545): kinds[a]
+ * is the kind on axis a (0 = x, slowest).  in/out: host FP64 arrays of
+ * nx*ny*nz; dtype SMAISO_FP64 computes in FP64, SMAISO_FP32_MIXED in FP32.
+ * Uses the current HIP device and the null stream; synchronous. */
+int smaiso_r2r3d(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
+                 const double* in, double* out, int32_t dtype);
+
+/* As smaiso_r2r3d on device buffers of the given dtype (double or float),
+ * enqueued on `stream` (a hipStream_t, NULL = null stream), asynchronous. */
+int smaiso_r2r3d_device(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
+                        const void* in_dev, void* out_dev, int32_t dtype, void* stream);
+
+/* Builder-defined initial condition (the reference's own IC is S:571-590):
+ *   profile = 1/2 (1 - tanh((z - h(x,y))/width)),  z = k dz, x = i dx, y = j dy
+ *   h(x,y)  = h0 + undulation cos(2 pi x/Lx) cos(2 pi y/Ly)  (or height[i*ny+j])
+ *   psi = amplitude cos(q0 z) profile + noise (2u - 1)
+ *   rho = rho_iso + (rho_sm - rho_iso) profile
+ * u = U[0,1) from splitmix64 (state seed + (n+1)*0x9E3779B97F4A7C15, top 53
+ * bits) in row-major order.  height may be NULL.  Host only. */
+int smaiso_make_interface_ic(const smaiso_params* p, const smaiso_ic* ic, const double* height,
+                             double* psi, double* rho);
+
+/* Per-kernel-class timing with HIP events on the handle's stream.
+ * enable != 0 starts collecting (and clears totals). */
+int smaiso_timing_enable(smaiso_handle* h, int32_t enable);
+
+/* Kernel classes for smaiso_timing_get. */
+#define SMAISO_KC_R2R_Z 0       /* axis pass along z */
+#define SMAISO_KC_R2R_Y 1       /* axis pass along y */
+#define SMAISO_KC_R2R_X 2       /* axis pass along x */
+#define SMAISO_KC_SPECTRAL 3    /* per-mode kernels */
+#define SMAISO_KC_POINTWISE 4   /* real-space kernels */
+#define SMAISO_KC_REDUCE 5      /* diagnostics */
+#define SMAISO_KC_COUNT 6
+
+/* Total milliseconds, launch count and algorithmic bytes moved by kernel
+ * class kc since timing was enabled. */
+int smaiso_timing_get(smaiso_handle* h, int32_t kc, double* total_ms, int64_t* launches,
+                      double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMAISO_H */

@@ -1,0 +1,549 @@
+// Per-mode and pointwise kernels of the time step (smaiso-wcomp.cpp:1081-2263).
+//
+// Every kernel is a flat grid over the Nx*Ny*Nz points with the reference's
+// layout (i*Ny + j)*Nz + k.  The reference's halo exchanges between x-slabs
+// (S:1108-1164, S:1492-1579, S:1713-1753, S:1979-2035) are plain +-1 index
+// shifts here; a shifted read outside the grid is 0, as on the first/last rank.
+// The operator tables of S:936-963 (mq2c, aLin, C1, C2, CM1a, CM2a) are
+// evaluated in registers from the per-axis wavenumber vectors.
+#include "stepper.h"
+
+#include <cmath>
+
+namespace smaiso {
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+struct Idx {
+  int i, j, k;
+};
+
+__device__ __forceinline__ Idx decompose(unsigned idx, int ny, int nz) {
+  Idx r;
+  unsigned t = idx / (unsigned)nz;
+  r.k = (int)(idx - t * (unsigned)nz);
+  unsigned i = t / (unsigned)ny;
+  r.j = (int)(t - i * (unsigned)ny);
+  r.i = (int)i;
+  return r;
+}
+
+// -1.57*1.57*|q|^2*2 evaluated left to right as in S:1281
+__device__ __forceinline__ double gauss(double mq2) { return exp(-1.57 * 1.57 * mq2 * 2); }
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// ---------------------------------------------------------------------------
+// a6: spectral gradients (S:1166-1221)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grad(StepConsts c, const double* __restrict__ psiq,
+                                                 const T* __restrict__ rhoq, T* gpx, T* gpy,
+                                                 T* gpz, T* rdx, T* rdy, T* rdz) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const Idx p = decompose(idx, c.ny, c.nz);
+  const unsigned sx = (unsigned)c.ny * c.nz, sy = (unsigned)c.nz;
+  double gx = 0, gy = 0, gz = 0, rx = 0, ry = 0, rz = 0;
+  if (p.i < c.nx - 1) {
+    const double v = c.vs[0][p.i];
+    gx = -v * psiq[idx + sx];
+    rx = -c.scale * v * (double)rhoq[idx + sx];
+  }
+  if (p.j < c.ny - 1) {
+    const double v = c.vs[1][p.j];
+    gy = -v * psiq[idx + sy];
+    ry = -c.scale * v * (double)rhoq[idx + sy];
+  }
+  if (p.k < c.nz - 1) {
+    const double v = c.vs[2][p.k];
+    gz = -v * psiq[idx + 1];
+    rz = -c.scale * v * (double)rhoq[idx + 1];
+  }
+  gpx[idx] = (T)gx; gpy[idx] = (T)gy; gpz[idx] = (T)gz;
+  rdx[idx] = (T)rx; rdy[idx] = (T)ry; rdz[idx] = (T)rz;
+}
+
+// ---------------------------------------------------------------------------
+// a7: amplitude (S:1254-1266)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amp(StepConsts c, long long n3, const T* __restrict__ psi,
+                                                const T* __restrict__ gx, const T* __restrict__ gy,
+                                                const T* __restrict__ gz, T* __restrict__ amp) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const T ps = psi[idx], a = gx[idx], b = gy[idx], d = gz[idx];
+  amp[idx] = sqrt((T)c.q02 * (ps * ps) + a * a + b * b + d * d);
+}
+
+// ---------------------------------------------------------------------------
+// diagonal spectral multipliers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mq2c_at(const StepConsts& c, const Idx& p) {
+  return sq(c.vq[0][p.i]) + sq(c.vq[1][p.j]) + sq(c.vq[2][p.k]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spec(StepConsts c, int op, const T* __restrict__ in,
+                                                 T* out) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const double x = (double)in[idx];
+  double r;
+  if (op == SPEC_SCALE) {
+    r = c.scale * x;
+  } else {
+    const Idx p = decompose(idx, c.ny, c.nz);
+    const double m = mq2c_at(c, p);
+    if (op == SPEC_AMP_SMOOTH) r = c.scale * x * gauss(m);
+    else r = -c.scale * m * x;
+  }
+  out[idx] = (T)r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_psiq_op(StepConsts c, int op,
+                                                    const double* __restrict__ psiq, T* out) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const Idx p = decompose(idx, c.ny, c.nz);
+  const double m = mq2c_at(c, p);
+  double r;
+  if (op == PSIQ_SH) r = c.alpha * (c.q02 - m) * psiq[idx];
+  else r = -m * c.alpha * (c.q02 - m) * psiq[idx];
+  out[idx] = (T)r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prod(long long n3, const T* __restrict__ a,
+                                                 const T* __restrict__ b, T* __restrict__ out) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  out[idx] = a[idx] * b[idx];
+}
+
+// a9 relaxation branch (S:1364-1378): quirk Q4, rho is overwritten
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, long long n3,
+                                                  const T* __restrict__ amp, T* laprl,
+                                                  double* rho, T* rho_t) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const double r = c.rho0 + ((1 - c.rho0) / c.amp_eq) * (double)amp[idx];
+  rho[idx] = r;
+  if (rho_t) rho_t[idx] = (T)r;
+  laprl[idx] = (T)(r * (double)laprl[idx]);
+}
+
+// ---------------------------------------------------------------------------
+// a10: stresses, forces and chemical potential (S:1385-1425)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_forces(
+    StepConsts c, long long n3, const T* __restrict__ lpsi, const T* __restrict__ laprl,
+    const T* __restrict__ rho, const T* __restrict__ amp, const T* __restrict__ psi,
+    const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
+    const T* __restrict__ rdx, const T* __restrict__ rdy, const T* __restrict__ rdz,
+    T* __restrict__ fgrad, T* __restrict__ erho, T* __restrict__ mu, T* __restrict__ sx,
+    T* __restrict__ sy, T* __restrict__ sz, T* __restrict__ fx, T* __restrict__ fy,
+    T* __restrict__ fz) {
+  using A = T;  // arithmetic type of the pointwise terms
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const A L = lpsi[idx], lr = laprl[idx], r = rho[idx], am = amp[idx], ps = psi[idx];
+  const A kp = (A)c.kp, zeta = (A)c.zeta, rho0 = (A)c.rho0;
+  const A e = A(0.5) * (L * L);
+  const A ra = rho0 + kp * am;
+  fgrad[idx] = A(0.5) * zeta * (r * r - ra * ra);
+  const A mu1 = lr + r * L;
+  const A gx = gpx[idx], gy = gpy[idx], gz = gpz[idx];
+  sx[idx] = mu1 * gx + e * rdx[idx];
+  sy[idx] = mu1 * gy + e * rdy[idx];
+  sz[idx] = mu1 * gz + e * rdz[idx];
+  const A p2 = ps * ps, p3 = p2 * ps, p5 = p3 * p2;
+  mu[idx] = lr + r * (L - (A)c.eps * ps - (A)c.beta * p3 + (A)c.gamma * p5);
+  erho[idx] = r * e;
+  const A fc = kp * zeta * (r - rho0 - kp * am) / (am + A(0.001));
+  fx[idx] = fc * gx;
+  fy[idx] = fc * gy;
+  fz[idx] = fc * gz;
+}
+
+// ---------------------------------------------------------------------------
+// a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_velocity(
+    StepConsts c, const T* __restrict__ sxq, const T* __restrict__ syq,
+    const T* __restrict__ szq, const T* __restrict__ fgq, const T* __restrict__ erq,
+    T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
+    T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const Idx p = decompose(idx, c.ny, c.nz);
+  const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
+  const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
+  const double wx = c.vs[0][p.i], wy = c.vs[1][p.j], wz = c.vs[2][p.k];
+  const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
+  const double inu = c.nu, ilam = c.lam + 2 * c.nu;
+  const double gfac = -1.57 * 1.57 * 2 * c.scale;
+  const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
+
+  // fgrad^ + erho^ G at a (shifted) mode index, S:1479-1488
+  auto fg_at = [&](unsigned q, int ii, int jj, int kk) -> double {
+    const double m = sq(c.vq[0][ii]) + sq(c.vq[1][jj]) + sq(c.vq[2][kk]);
+    return (double)fgq[q] + (double)erq[q] * exp(-1.57 * 1.57 * m * 2);
+  };
+
+  // x component, i <= Nx-2 (quirk Q2), S:1583-1709
+  double ox = 0, rx = 0;
+  if (p.i < c.nx - 1) {
+    const double Syp = (p.j > 0) ? (double)syq[idx + sI - sJ] : 0.0;
+    const double Szp = (p.k > 0) ? (double)szq[idx + sI - 1] : 0.0;
+    const double dot = wx * Sx + qy * Syp + qz * Szp;
+    const double mq2 = wx * wx + qy2 + qz2;
+    const double cm1 = c.scale / mq2, cm2 = wx / mq2;
+    ox = (cm1 / inu) * (Sx - cm2 * dot);
+    rx = (cm1 / ilam) * (wx * fg_at(idx + sI, p.i + 1, p.j, p.k) + cm2 * dot * exp(gfac / cm1));
+  }
+  double oy = 0, ry = 0, oz = 0, rz = 0;
+  if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
+    // y component
+    {
+      const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + sJ] : 0.0;
+      const double Szp = (p.k > 0) ? (double)szq[idx + sJ - 1] : 0.0;
+      const double dot = qx * Sxp + wy * Sy + qz * Szp;
+      const double mq2 = qx2 + wy * wy + qz2;
+      const double cm1 = c.scale / mq2, cm2 = wy / mq2;
+      oy = (cm1 / inu) * (Sy - cm2 * dot);
+      ry = (cm1 / ilam) * (wy * fg_at(idx + sJ, p.i, p.j + 1, p.k) + cm2 * dot * exp(gfac / cm1));
+    }
+    // z component
+    {
+      const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + 1] : 0.0;
+      const double Syp = (p.j > 0) ? (double)syq[idx - sJ + 1] : 0.0;
+      const double dot = qx * Sxp + qy * Syp + wz * Sz;
+      const double mq2 = qx2 + qy2 + wz * wz;
+      const double cm1 = c.scale / mq2, cm2 = wz / mq2;
+      oz = (cm1 / inu) * (Sz - cm2 * dot);
+      // quirk Q3 (S:1932): CM1y in the exponent at k=0, j>=1, except the
+      // first plane of each reference slab
+      double cme = cm1;
+      const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
+      if (p.k == 0 && p.j > 0 && (p.i % slab) != 0) cme = c.scale / (qx2 + wy * wy + qz2);
+      rz = (cm1 / ilam) * (wz * fg_at(idx + 1, p.i, p.j, p.k + 1) + cm2 * dot * exp(gfac / cme));
+    }
+  }
+  vsolx[idx] = (T)ox; virrx[idx] = (T)rx;
+  vsoly[idx] = (T)oy; virry[idx] = (T)ry;
+  vsolz[idx] = (T)oz; virrz[idx] = (T)rz;
+}
+
+// ---------------------------------------------------------------------------
+// a13: divergences (S:2042-2080)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_divergence(
+    StepConsts c, const T* __restrict__ virrx, const T* __restrict__ virry,
+    const T* __restrict__ virrz, const T* __restrict__ fxq, const T* __restrict__ fyq,
+    const T* __restrict__ fzq, T* __restrict__ divv, T* __restrict__ divf) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const Idx p = decompose(idx, c.ny, c.nz);
+  const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
+  double vxm = 0, fxm = 0, vym = 0, fym = 0, vzm = 0, fzm = 0;
+  if (p.i > 0) { vxm = (double)virrx[idx - sI]; fxm = (double)fxq[idx - sI]; }
+  if (p.j > 0) { vym = (double)virry[idx - sJ]; fym = (double)fyq[idx - sJ]; }
+  if (p.k > 0) { vzm = (double)virrz[idx - 1]; fzm = (double)fzq[idx - 1]; }
+  const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
+  divv[idx] = (T)(qx * vxm + qy * vym + qz * vzm);
+  divf[idx] = (T)(c.scale * (qx * fxm + qy * fym + qz * fzm));
+}
+
+// ---------------------------------------------------------------------------
+// a15 + a16: density AB2 (S:2121-2139) and the nonlinear term (S:2144-2162)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_density_nl(
+    StepConsts c, long long n3, const T* __restrict__ vx, const T* __restrict__ vy,
+    const T* __restrict__ vz, const T* __restrict__ rdx, const T* __restrict__ rdy,
+    const T* __restrict__ rdz, const T* __restrict__ divv, double* __restrict__ rho,
+    double* __restrict__ f_rho, T* __restrict__ rho_t, const T* __restrict__ mu,
+    const T* __restrict__ amp, const T* __restrict__ psi, const T* __restrict__ divf,
+    const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
+    T* __restrict__ nl) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  // density in FP64 (both precisions)
+  const double ux = vx[idx], uy = vy[idx], uz = vz[idx];
+  const double r0 = rho[idx];
+  const double src = -ux * (double)rdx[idx] - uy * (double)rdy[idx] - uz * (double)rdz[idx] -
+                     r0 * (double)divv[idx];
+  const double r1 = r0 + c.dt * (1.5 * src - 0.5 * f_rho[idx]);
+  rho[idx] = r1;
+  f_rho[idx] = src;
+  if (rho_t) rho_t[idx] = (T)r1;
+  using A = T;
+  const A am = amp[idx];
+  const A kp = (A)c.kp;
+  const A v = -(A)mu[idx] + ((A)(c.kp * c.zeta) / (am + A(0.001))) * ((A)r1 - (A)c.rho0 - kp * am) * psi[idx] -
+              divf[idx] - (A)ux * gpx[idx] - (A)uy * gpy[idx] - (A)uz * gpz[idx];
+  nl[idx] = v;
+}
+
+// ---------------------------------------------------------------------------
+// a17: stabilised BDF2 / AB2 update (S:2177-2190), FP64 denominators
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_psi_update(StepConsts c, const double* __restrict__ psiq,
+                                                       double* psiq_old, const T* __restrict__ nlq,
+                                                       const T* __restrict__ nlq_old, int first,
+                                                       T* __restrict__ psiq_t) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const Idx p = decompose(idx, c.ny, c.nz);
+  const double m = mq2c_at(c, p);
+  const double alin = c.eps - c.alpha * sq(c.q02 - m);
+  const double c1 = 2.0 - c.dt * c.rho_m * alin;
+  const double c2 = 1.5 - c.dt * c.rho_m * alin;
+  const double n1 = (double)nlq[idx];
+  const double n0 = first ? n1 : (double)nlq_old[idx];
+  const double r = (c1 * psiq[idx] - 0.5 * psiq_old[idx] + (c.dt / 2) * c.scale * (3.0 * n1 - n0)) / c2;
+  psiq_old[idx] = r;
+  if (psiq_t) psiq_t[idx] = (T)r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_energy_density(StepConsts c, long long n3,
+                                                           const T* __restrict__ psi,
+                                                           const T* __restrict__ lpsi,
+                                                           const T* __restrict__ rho,
+                                                           const T* __restrict__ amp,
+                                                           T* __restrict__ out) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= n3) return;
+  const double ps = psi[idx], L = lpsi[idx], r = rho[idx], am = amp[idx];
+  const double p2 = ps * ps;
+  const double e = -0.5 * c.eps * p2 + 0.5 * (L * L) - 0.25 * c.beta * (p2 * p2) +
+                   (c.gamma / 6) * (p2 * p2 * p2);
+  const double d = r - c.rho0 - c.kp * am;
+  out[idx] = (T)(r * e + 0.5 * c.zeta * (d * d));
+}
+
+// ---------------------------------------------------------------------------
+// a18: diagnostics with compensated (two-sum) accumulation
+// ---------------------------------------------------------------------------
+struct CSum {
+  double s, c;
+};
+
+__device__ __forceinline__ void csum_add(CSum& a, double x) {
+  const double t = a.s + x;
+  // Neumaier: carry the low part of whichever operand is larger
+  if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+  else a.c += (x - t) + a.s;
+  a.s = t;
+}
+
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+  CSum r = a;
+  csum_add(r, b.s);
+  r.c += b.c;
+  return r;
+}
+
+constexpr int kDiagBlocks = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag(StepConsts c, long long n3,
+                                                 const T* __restrict__ psi,
+                                                 const T* __restrict__ psi_old,
+                                                 const double* __restrict__ rho,
+                                                 const T* __restrict__ lpsi,
+                                                 const T* __restrict__ amp,
+                                                 double* __restrict__ partials) {
+  CSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (long long idx = (long long)blockIdx.x * kBlock + threadIdx.x; idx < n3;
+       idx += (long long)gridDim.x * kBlock) {
+    const double ps = psi[idx], po = psi_old[idx], r = rho[idx], L = lpsi[idx], am = amp[idx];
+    csum_add(acc[0], fabs(ps - po));
+    csum_add(acc[1], fabs(ps));
+    csum_add(acc[2], r);
+    const double p2 = ps * ps;
+    const double e = -0.5 * c.eps * p2 + 0.5 * (L * L) - 0.25 * c.beta * (p2 * p2) +
+                     (c.gamma / 6) * (p2 * p2 * p2);
+    const double d = r - c.rho0 - c.kp * am;
+    csum_add(acc[3], r * e + 0.5 * c.zeta * (d * d));
+  }
+  __shared__ CSum sh[4][kBlock];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = csum_merge(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) {
+    const CSum r = sh[threadIdx.x][0];
+    partials[2 * (4 * blockIdx.x + threadIdx.x)] = r.s;
+    partials[2 * (4 * blockIdx.x + threadIdx.x) + 1] = r.c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_final(const double* __restrict__ partials, int nb,
+                                                       double* __restrict__ out) {
+  CSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      acc[q] = csum_merge(acc[q], CSum{partials[2 * (4 * b + q)], partials[2 * (4 * b + q) + 1]});
+  }
+  __shared__ CSum sh[4][kBlock];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = csum_merge(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0].s + sh[threadIdx.x][0].c;
+}
+
+__global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, long long n) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx < n) out[idx] = (float)in[idx];
+}
+__global__ void k_f2d(const float* __restrict__ in, double* __restrict__ out, long long n) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx < n) out[idx] = (double)in[idx];
+}
+__global__ void k_zero_d(double* __restrict__ out, long long n) {
+  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx < n) out[idx] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline long long n3_of(const StepConsts& c) { return (long long)c.nx * c.ny * c.nz; }
+
+template <typename T>
+void StepKernels<T>::grad(const StepConsts& c, const double* psiq, const T* rhoq, T* gpx, T* gpy,
+                          T* gpz, T* rdx, T* rdy, T* rdz, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, psiq, rhoq, gpx,
+                     gpy, gpz, rdx, rdy, rdz);
+}
+template <typename T>
+void StepKernels<T>::amp(const StepConsts& c, const T* psi, const T* gx, const T* gy, const T* gz,
+                         T* amp, hipStream_t s) {
+  hipLaunchKernelGGL(k_amp<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), psi, gx,
+                     gy, gz, amp);
+}
+template <typename T>
+void StepKernels<T>::spec(const StepConsts& c, int op, const T* in, T* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_spec<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, op, in, out);
+}
+template <typename T>
+void StepKernels<T>::psiq_op(const StepConsts& c, int op, const double* psiq, T* out,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_psiq_op<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, op, psiq, out);
+}
+template <typename T>
+void StepKernels<T>::prod(const StepConsts& c, const T* a, const T* b, T* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_prod<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, n3_of(c), a, b, out);
+}
+template <typename T>
+void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho, T* rho_t,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_relax<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), amp,
+                     laprl, rho, rho_t);
+}
+template <typename T>
+void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const T* rho,
+                            const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
+                            const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu,
+                            T* sx, T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s) {
+  hipLaunchKernelGGL(k_forces<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), lpsi,
+                     laprl, rho, amp, psi, gpx, gpy, gpz, rdx, rdy, rdz, fgrad, erho, mu, sx, sy,
+                     sz, fx, fy, fz);
+}
+template <typename T>
+void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
+                              const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
+                              T* virry, T* virrz, hipStream_t s) {
+  hipLaunchKernelGGL(k_velocity<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, sxq, syq, szq,
+                     fgq, erq, vsolx, vsoly, vsolz, virrx, virry, virrz);
+}
+template <typename T>
+void StepKernels<T>::divergence(const StepConsts& c, const T* virrx, const T* virry,
+                                const T* virrz, const T* fxq, const T* fyq, const T* fzq, T* divv,
+                                T* divf, hipStream_t s) {
+  hipLaunchKernelGGL(k_divergence<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, virrx, virry,
+                     virrz, fxq, fyq, fzq, divv, divf);
+}
+template <typename T>
+void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
+                                const T* rdx, const T* rdy, const T* rdz, const T* divv,
+                                double* rho, double* f_rho, T* rho_t, const T* mu, const T* amp,
+                                const T* psi, const T* divf, const T* gpx, const T* gpy,
+                                const T* gpz, T* nl, hipStream_t s) {
+  hipLaunchKernelGGL(k_density_nl<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), vx,
+                     vy, vz, rdx, rdy, rdz, divv, rho, f_rho, rho_t, mu, amp, psi, divf, gpx, gpy,
+                     gpz, nl);
+}
+template <typename T>
+void StepKernels<T>::psi_update(const StepConsts& c, const double* psiq, double* psiq_old_to_new,
+                                const T* nlq, const T* nlq_old, int first, T* psiq_t,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_psi_update<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, psiq,
+                     psiq_old_to_new, nlq, nlq_old, first, psiq_t);
+}
+template <typename T>
+void StepKernels<T>::energy_density(const StepConsts& c, const T* psi, const T* lpsi, const T* rho,
+                                    const T* amp, T* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_density<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c),
+                     psi, lpsi, rho, amp, out);
+}
+template <typename T>
+int StepKernels<T>::diag_partial(const StepConsts& c, const T* psi, const T* psi_old,
+                                 const double* rho, const T* lpsi, const T* amp, double* partials,
+                                 hipStream_t s) {
+  long long nb = grid_for(n3_of(c));
+  if (nb > kDiagBlocks) nb = kDiagBlocks;
+  hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, c, n3_of(c), psi, psi_old,
+                     rho, lpsi, amp, partials);
+  return (int)nb;
+}
+
+void diag_final(const double* partials, int nb, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, partials, nb, out);
+}
+void convert_d2f(const double* in, float* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+}
+void convert_f2d(const float* in, double* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);
+}
+void fill_zero_d(double* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, out, n);
+}
+
+template struct StepKernels<double>;
+template struct StepKernels<float>;
+
+}  // namespace smaiso

@@ -1,0 +1,175 @@
+"""GPU stepper vs the FP64 oracle (per field, per step) and error paths."""
+import numpy as np
+import pytest
+
+import smaiso_amd as S
+import smaiso_oracle as O
+from helpers import config1, group_scales, params_pair, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10  # per-field max-abs and L2 error, relative to the field's scale
+
+
+def oracle_fields(o: O.Oracle):
+    f, x = o.f, o.f.extra
+    d = dict(psi=f.psi, rho=f.rho, psiq=f.psiq, psiq_old=f.psiq_old, nlq=f.nlq, f_rho=f.f_rho)
+    for k in ("amp", "vx", "vy", "vz", "vsolx", "vsoly", "vsolz", "virrx", "virry", "virrz",
+              "divv", "divf", "lpsi", "mu", "gpx", "gpy", "gpz", "rdx", "rdy", "rdz", "lap_rl",
+              "psi_old"):
+        d[k] = x[k]
+    d["energy"] = O.energy_density(o.p, f.psi, x["lpsi"], f.rho, x["amp"])
+    return d
+
+
+def run_pair(sp, op, psi, rho, steps):
+    o = O.Oracle(op)
+    o.set_state(psi, rho)
+    st = S.Stepper(sp)
+    st.set_state(psi, rho)
+    o.step(steps)
+    st.step(steps)
+    return st, o
+
+
+def compare_all(st, o, tol, names=None):
+    ref = oracle_fields(o)
+    sc = group_scales(ref)
+    bad = {}
+    for name in names or S.FIELDS:
+        got = st.field(name)
+        mx, l2 = rel_err(got, ref[name], *sc[name])
+        if not (mx < tol and l2 < tol):
+            bad[name] = (mx, l2)
+    assert not bad, bad
+
+
+def test_first_step_every_field_odd_grid():
+    """One step on a non-cubic grid mixing the FFT (32, 48) and direct (20) paths."""
+    sp, op = params_pair(nx=32, ny=20, nz=48, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(undulation=0.5, seed=5))
+    st, o = run_pair(sp, op, psi, rho, 1)
+    compare_all(st, o, TOL64)
+
+
+def test_config1_20_steps_every_field():
+    sp, op = config1(64)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=1))
+    st, o = run_pair(sp, op, psi, rho, 20)
+    compare_all(st, o, TOL64)
+    l1, mass, en = st.diagnostics()
+    rl1, rmass, ren = o.diagnostics()
+    assert abs(l1 - rl1) <= 1e-10 * abs(rl1)
+    assert abs(mass - rmass) <= 1e-12 * abs(rmass)
+    assert abs(en - ren) <= 1e-10 * abs(ren)
+
+
+def test_config1_200_steps_trajectory():
+    """Config 1 as specified (64^3, dt=0.01, 200 steps, FP64), with the mass
+    and free-energy trajectories every 50 steps."""
+    sp, op = config1(64)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=1))
+    o = O.Oracle(op); o.set_state(psi, rho)
+    st = S.Stepper(sp); st.set_state(psi, rho)
+    for _ in range(4):
+        o.step(50); st.step(50)
+        l1, mass, en = st.diagnostics()
+        rl1, rmass, ren = o.diagnostics()
+        assert abs(mass - rmass) <= 1e-12 * abs(rmass), (mass, rmass)
+        assert abs(en - ren) <= 1e-10 * abs(ren), (en, ren)
+        assert abs(l1 - rl1) <= 1e-9 * abs(rl1), (l1, rl1)
+    compare_all(st, o, TOL64, ["psi", "rho", "vx", "vy", "vz", "amp", "psiq"])
+
+
+def test_relaxation_branch_reference_ic():
+    """divvSwitch > 0 with the reference's own IC (S:571-590) and rho = 0:
+    rho is slaved to the amplitude for the first steps (quirk Q4)."""
+    sp, op = params_pair(nx=32, ny=16, nz=64, dt=0.001, divv_switch=3)
+    psi, rho = O.reference_layers_ic(op)
+    st, o = run_pair(sp, op, psi, rho, 6)
+    compare_all(st, o, TOL64)
+
+
+@pytest.mark.parametrize("slab", [1, 8, 5])
+def test_slab_quirk_q3(slab):
+    """Quirk Q3 follows the emulated x-slab height."""
+    sp, op = params_pair(nx=16, ny=16, nz=16, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0, slab_n0=slab)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(rho_iso=0.8, undulation=0.5, seed=9))
+    st, o = run_pair(sp, op, psi, rho, 2)
+    compare_all(st, o, TOL64, ["virrz", "vz", "psi", "rho"])
+
+
+def test_load_mode_skips_relaxation():
+    sp, op = params_pair(nx=16, ny=16, nz=32, dt=0.005, rho0=0.9, rho_m=1.0, divv_switch=50, load_mode=1)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=2))
+    st, o = run_pair(sp, op, psi, rho, 3)
+    compare_all(st, o, TOL64, ["psi", "rho", "lap_rl"])
+
+
+def test_uneven_parameters():
+    """Non-default nu, lambda, zeta, alpha, beta, gamma, q0, anisotropic dx."""
+    kw = dict(nx=24, ny=32, nz=40, dx=0.7, dy=0.8, dz=0.75, dt=0.004, alpha=1.3, beta=1.7,
+              gamma=0.9, epsilon=-0.02, q0=1.1, nu=0.3, lam=0.5, zeta=1.7, rho0=0.85, rho_m=1.1,
+              divv_switch=1)
+    sp, op = params_pair(**kw)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(rho_iso=0.85, undulation=0.7, seed=4))
+    st, o = run_pair(sp, op, psi, rho, 3)
+    compare_all(st, o, TOL64)
+
+
+def test_mixed_precision_close_to_fp64():
+    sp, op = config1(64, precision=S.FP32_MIXED)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=1))
+    st, o = run_pair(sp, op, psi, rho, 20)
+    ref = oracle_fields(o)
+    sc = group_scales(ref)
+    for name, tol in (("psi", 1e-5), ("rho", 1e-5), ("vx", 1e-3), ("vz", 1e-3)):
+        mx, l2 = rel_err(st.field(name), ref[name], *sc[name])
+        assert l2 < tol, (name, mx, l2)
+
+
+def test_uniform_state_is_stationary():
+    sp, op = params_pair(nx=16, ny=16, nz=16, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
+    psi = np.zeros(sp.shape)
+    rho = np.full(sp.shape, 0.9)
+    st = S.Stepper(sp)
+    st.set_state(psi, rho)
+    st.step(5)
+    assert np.abs(st.field("psi")).max() == 0.0
+    assert np.abs(st.field("rho") - 0.9).max() < 1e-15
+
+
+def test_error_paths():
+    sp, _ = config1(16)
+    st = S.Stepper(sp)
+    with pytest.raises(S.SmaisoError) as e:
+        st.step(1)
+    assert e.value.code == S.E_STATE
+    psi = np.zeros(sp.shape); rho = np.full(sp.shape, 0.9)
+    st.set_state(psi, rho)
+    with pytest.raises(S.SmaisoError) as e:
+        st.diagnostics()
+    assert e.value.code == S.E_STATE
+    with pytest.raises(KeyError):
+        st.field("nope")
+    psi[3, 4, 5] = np.nan
+    st.set_state(psi, rho)
+    st.step(1)
+    with pytest.raises(S.SmaisoError) as e:
+        st.diagnostics()
+    assert e.value.code == S.E_NONFINITE
+
+
+def test_nloop_and_async():
+    sp, op = config1(32)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=8))
+    st = S.Stepper(sp)
+    st.set_state(psi, rho)
+    st.step_async(3)
+    st.synchronize()
+    assert st.nloop == 3
+    a = st.field("psi")
+    st.set_state(psi, rho)
+    assert st.nloop == 0
+    st.step(3)
+    assert np.array_equal(a, st.field("psi"))
