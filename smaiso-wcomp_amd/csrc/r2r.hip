@@ -125,41 +125,68 @@ __device__ __forceinline__ void dft(cplx<T>* a) {
 }
 
 // ---------------------------------------------------------------------------
-// compile-time FFT plans
+// compile-time FFT plans: radix of stage s for length n when no radix may
+// exceed maxr (16, 8 or 4); 0 ends the plan
 // ---------------------------------------------------------------------------
-constexpr int fft_radix(int n, int s) {
-  switch (n) {
-    case 16: return s == 0 ? 16 : 0;
-    case 32: return s == 0 ? 8 : s == 1 ? 4 : 0;
-    case 48: return s == 0 ? 16 : s == 1 ? 3 : 0;
-    case 64: return s == 0 ? 8 : s == 1 ? 8 : 0;
-    case 96: return s == 0 ? 8 : s == 1 ? 4 : s == 2 ? 3 : 0;
-    case 128: return s == 0 ? 16 : s == 1 ? 8 : 0;
-    case 192: return s == 0 ? 16 : s == 1 ? 4 : s == 2 ? 3 : 0;
-    case 256: return s == 0 ? 16 : s == 1 ? 16 : 0;
-    case 384: return s == 0 ? 16 : s == 1 ? 8 : s == 2 ? 3 : 0;
-    case 512: return s == 0 ? 8 : s == 1 ? 8 : s == 2 ? 8 : 0;
-    case 1024: return s == 0 ? 16 : s == 1 ? 16 : s == 2 ? 4 : 0;
-    default: return 0;
+constexpr int fft_radix(int n, int maxr, int s) {
+  if (maxr >= 16) {
+    switch (n) {
+      case 16: { const int r[] = {16, 0}; return r[s < 1 ? s : 1]; }
+      case 32: { const int r[] = {16, 2, 0}; return r[s < 2 ? s : 2]; }
+      case 48: { const int r[] = {16, 3, 0}; return r[s < 2 ? s : 2]; }
+      case 64: { const int r[] = {16, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 96: { const int r[] = {16, 3, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 128: { const int r[] = {16, 8, 0}; return r[s < 2 ? s : 2]; }
+      case 192: { const int r[] = {16, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 256: { const int r[] = {16, 16, 0}; return r[s < 2 ? s : 2]; }
+      case 384: { const int r[] = {16, 8, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 512: { const int r[] = {16, 8, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 1024: { const int r[] = {16, 16, 4, 0}; return r[s < 3 ? s : 3]; }
+      default: return 0;
+    }
+  } else if (maxr >= 8) {
+    switch (n) {
+      case 16: { const int r[] = {8, 2, 0}; return r[s < 2 ? s : 2]; }
+      case 32: { const int r[] = {8, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 48: { const int r[] = {8, 3, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 64: { const int r[] = {8, 8, 0}; return r[s < 2 ? s : 2]; }
+      case 96: { const int r[] = {8, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 128: { const int r[] = {8, 4, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 192: { const int r[] = {8, 8, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 256: { const int r[] = {8, 8, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 384: { const int r[] = {8, 8, 3, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 512: { const int r[] = {8, 8, 8, 0}; return r[s < 3 ? s : 3]; }
+      case 1024: { const int r[] = {8, 8, 4, 4, 0}; return r[s < 4 ? s : 4]; }
+      default: return 0;
+    }
+  } else {
+    switch (n) {
+      case 16: { const int r[] = {4, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 32: { const int r[] = {4, 4, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 48: { const int r[] = {4, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 64: { const int r[] = {4, 4, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 96: { const int r[] = {4, 4, 3, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 128: { const int r[] = {4, 4, 4, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 192: { const int r[] = {4, 4, 4, 3, 0}; return r[s < 4 ? s : 4]; }
+      case 256: { const int r[] = {4, 4, 4, 4, 0}; return r[s < 4 ? s : 4]; }
+      case 384: { const int r[] = {4, 4, 4, 3, 2, 0}; return r[s < 5 ? s : 5]; }
+      case 512: { const int r[] = {4, 4, 4, 4, 2, 0}; return r[s < 5 ? s : 5]; }
+      case 1024: { const int r[] = {4, 4, 4, 4, 4, 0}; return r[s < 5 ? s : 5]; }
+      default: return 0;
+    }
   }
 }
 
-constexpr int ns_before(int n, int s) {
+constexpr int ns_before(int n, int maxr, int s) {
   int p = 1;
-  for (int i = 0; i < s; ++i) p *= fft_radix(n, i);
+  for (int i = 0; i < s; ++i) p *= fft_radix(n, maxr, i);
   return p;
 }
 
-// Line pairs per workgroup: about 4096 complex values of LDS per workgroup.
-template <int N>
-constexpr int pairs_per_block() {
-  int p = 4096 / N;
-  int q = 1;
-  while (q * 2 <= p) q *= 2;
-  return q < 2 ? 2 : q;
+// Largest radix that still gives every thread of the block a butterfly.
+constexpr int pick_maxr(int n, int p, int nt) {
+  return (p * n / 16 >= nt) ? 16 : (p * n / 8 >= nt) ? 8 : 4;
 }
-
-constexpr int kThreads = 256;
 
 template <typename T>
 struct PassArgs {
@@ -171,6 +198,7 @@ struct PassArgs {
   int nx, ny, nz;
   int axis;
   int kind;
+  int vec;  // 16-byte pair accesses allowed (nz even, pointers 16-byte aligned)
 };
 
 struct TileGeo {
@@ -207,21 +235,21 @@ __device__ __forceinline__ TileGeo tile_geometry(int axis, int nx, int ny, int n
   return t;
 }
 
-template <typename T, int N, int P, int SIGN, int S>
+template <typename T, int N, int P, int NT, int MAXR, int SIGN, int S>
 __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
-  constexpr int R = fft_radix(N, S);
+  constexpr int R = fft_radix(N, MAXR, S);
   if constexpr (R != 0) {
-    constexpr int NS = ns_before(N, S);
+    constexpr int NS = ns_before(N, MAXR, S);
     constexpr int J = N / R;
     constexpr int TOT = P * J;
-    constexpr int IT = (TOT + kThreads - 1) / kThreads;
+    constexpr int IT = (TOT + NT - 1) / NT;
     constexpr int STRIDE = N / (NS * R);
     cplx<T> v[IT][R];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int w = tid + it * kThreads;
-      if (TOT % kThreads == 0 || w < TOT) {
+      const int w = tid + it * NT;
+      if (TOT % NT == 0 || w < TOT) {
         const int p = w / J;
         const int j = w - p * J;
 #pragma unroll
@@ -241,8 +269,8 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      const int w = tid + it * kThreads;
-      if (TOT % kThreads == 0 || w < TOT) {
+      const int w = tid + it * NT;
+      if (TOT % NT == 0 || w < TOT) {
         const int p = w / J;
         const int j = w - p * J;
         const int k = j % NS;
@@ -252,14 +280,16 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
       }
     }
     __syncthreads();
-    fft_stage<T, N, P, SIGN, S + 1>(buf, tw);
+    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1>(buf, tw);
   }
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs<T> a) {
-  constexpr int P = pairs_per_block<N>();
+// One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
+// form complex sequence c.
+template <typename T, int N, int P, int NT>
+__global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   constexpr int TL = 2 * P;
+  constexpr int MAXR = pick_maxr(N, P, NT);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   T* rbuf = reinterpret_cast<T*>(smem_raw);
@@ -268,30 +298,78 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs<T> a) {
   const bool fwd = (a.kind == REDFT10 || a.kind == RODFT10);
   const bool sine = (a.kind == RODFT10 || a.kind == RODFT01);
 
-  // ---- load ---------------------------------------------------------------
-  constexpr int E = TL * N;
-  constexpr int IT = (E + kThreads - 1) / kThreads;
-#pragma unroll 4
-  for (int it = 0; it < IT; ++it) {
-    const int e = tid + it * kThreads;
-    if (E % kThreads == 0 || e < E) {
-      int l, m;
-      if (a.axis == 2) { l = e / N; m = e - l * N; }
-      else { m = e / TL; l = e - m * TL; }
-      T v = T(0);
-      if (l < g.nvalid) {
-        const long long addr = g.base + l * g.line_step + m * g.stride_m;
-        v = a.in[addr];
-        if (a.in2) v += a.in2[addr];
+  // ---- load: every global load of the tile is issued before the first LDS
+  // write, so each thread keeps all of its loads in flight -----------------
+  if (a.axis == 2) {
+    constexpr int E = TL * N;
+    constexpr int IT = (E + NT - 1) / NT;
+    T v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int l = e / N, m = e - (e / N) * N;
+      v[it] = T(0);
+      if ((E % NT == 0 || e < E) && l < g.nvalid) {
+        const long long addr = g.base + (long long)l * N + m;
+        v[it] = a.in[addr];
+        if (a.in2) v[it] += a.in2[addr];
       }
-      if (fwd) {
-        if (sine && (m & 1)) v = -v;
-        const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-        const int p = (l < P) ? l : l - P;
-        rbuf[2 * (p * N + pos) + (l < P ? 0 : 1)] = v;
-      } else {
-        const int pos = sine ? (N - 1 - m) : m;
-        rbuf[l * N + pos] = v;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (E % NT == 0 || e < E) {
+        const int l = e / N, m = e - (e / N) * N;
+        int pos;
+        T x = v[it];
+        if (fwd) {
+          if (sine && (m & 1)) x = -x;
+          pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        } else {
+          pos = sine ? (N - 1 - m) : m;
+        }
+        rbuf[2 * ((l >> 1) * N + pos) + (l & 1)] = x;
+      }
+    }
+  } else {
+    constexpr int E = P * N;  // pairs of adjacent columns
+    constexpr int IT = (E + NT - 1) / NT;
+    cplx<T> v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int c = e % P, m = e / P;
+      v[it] = cplx<T>{T(0), T(0)};
+      if (E % NT == 0 || e < E) {
+        const int l0 = 2 * c;
+        const long long addr = g.base + l0 + (long long)m * g.stride_m;
+        if (a.vec && l0 + 1 < g.nvalid) {
+          v[it] = *reinterpret_cast<const cplx<T>*>(a.in + addr);
+          if (a.in2) {
+            const cplx<T> w = *reinterpret_cast<const cplx<T>*>(a.in2 + addr);
+            v[it].x += w.x;
+            v[it].y += w.y;
+          }
+        } else {
+          if (l0 < g.nvalid) { v[it].x = a.in[addr]; if (a.in2) v[it].x += a.in2[addr]; }
+          if (l0 + 1 < g.nvalid) { v[it].y = a.in[addr + 1]; if (a.in2) v[it].y += a.in2[addr + 1]; }
+        }
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (E % NT == 0 || e < E) {
+        const int c = e % P, m = e / P;
+        int pos;
+        cplx<T> x = v[it];
+        if (fwd) {
+          if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
+          pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        } else {
+          pos = sine ? (N - 1 - m) : m;
+        }
+        buf[c * N + pos] = x;
       }
     }
   }
@@ -302,96 +380,103 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs<T> a) {
     // packed as z_k = U0_k + i U1_k = c_k (Y0_k + Y1_{N-k}, Y1_k - Y0_{N-k}).
     constexpr int H = N / 2 + 1;
     constexpr int W = P * H;
-    constexpr int ITW = (W + kThreads - 1) / kThreads;
-    T r0k[ITW], r0n[ITW], r1k[ITW], r1n[ITW];
+    constexpr int ITW = (W + NT - 1) / NT;
+    cplx<T> rk[ITW], rn[ITW];
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
-      const int w = tid + it * kThreads;
+      const int w = tid + it * NT;
       if (w < W) {
-        const int p = w / H;
-        const int k = w - p * H;
-        const T* y0 = rbuf + p * N;
-        const T* y1 = rbuf + (p + P) * N;
-        r0k[it] = (k < N) ? y0[k] : T(0);
-        r1k[it] = (k < N) ? y1[k] : T(0);
-        r0n[it] = (k > 0) ? y0[N - k] : T(0);
-        r1n[it] = (k > 0) ? y1[N - k] : T(0);
+        const int c = w / H;
+        const int k = w - c * H;
+        rk[it] = buf[c * N + (k < N ? k : 0)];
+        rn[it] = (k > 0) ? buf[c * N + N - k] : cplx<T>{T(0), T(0)};
       }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
-      const int w = tid + it * kThreads;
+      const int w = tid + it * NT;
       if (w < W) {
-        const int p = w / H;
-        const int k = w - p * H;
+        const int c = w / H;
+        const int k = w - c * H;
         if (k < N) {
-          cplx<T> c = a.mk[k];
-          c.y = -c.y;
-          buf[p * N + k] = cmul(c, cplx<T>{r0k[it] + r1n[it], r1k[it] - r0n[it]});
+          cplx<T> cc = a.mk[k];
+          cc.y = -cc.y;
+          buf[c * N + k] = cmul(cc, cplx<T>{rk[it].x + rn[it].y, rk[it].y - rn[it].x});
         }
         if (k > 0 && k < N - k) {
-          cplx<T> c = a.mk[N - k];
-          c.y = -c.y;
-          buf[p * N + N - k] = cmul(c, cplx<T>{r0n[it] + r1k[it], r1n[it] - r0k[it]});
+          cplx<T> cc = a.mk[N - k];
+          cc.y = -cc.y;
+          buf[c * N + N - k] = cmul(cc, cplx<T>{rn[it].x + rk[it].y, rn[it].y - rk[it].x});
         }
       }
     }
     __syncthreads();
-    fft_stage<T, N, P, +1, 0>(buf, a.tw);
-    // ---- store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n} ---------------------
+    fft_stage<T, N, P, NT, MAXR, +1, 0>(buf, a.tw);
+    // ---- store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n}, line 2c from Re, 2c+1 from Im
     constexpr int HN = N / 2;
     constexpr int WO = P * HN;
-    constexpr int ITO = (WO + kThreads - 1) / kThreads;
-#pragma unroll 4
+    constexpr int ITO = (WO + NT - 1) / NT;
+    const T s = sine ? T(-1) : T(1);
+#pragma unroll
     for (int it = 0; it < ITO; ++it) {
-      const int w = tid + it * kThreads;
-      if (WO % kThreads == 0 || w < WO) {
-        int p, n;
-        if (a.axis == 2) { p = w / HN; n = w - p * HN; }
-        else { n = w / P; p = w - n * P; }
-        const cplx<T> ua = buf[p * N + n];
-        const cplx<T> ub = buf[p * N + N - 1 - n];
-        const T s = sine ? T(-1) : T(1);
-        if (p < g.nvalid) {
-          T* o = a.out + g.base + p * g.line_step;
-          o[(2 * n) * g.stride_m] = ua.x;
-          o[(2 * n + 1) * g.stride_m] = s * ub.x;
-        }
-        if (p + P < g.nvalid) {
-          T* o = a.out + g.base + (p + P) * g.line_step;
-          o[(2 * n) * g.stride_m] = ua.y;
-          o[(2 * n + 1) * g.stride_m] = s * ub.y;
+      const int w = tid + it * NT;
+      if (WO % NT == 0 || w < WO) {
+        int c, n;
+        if (a.axis == 2) { c = w / HN; n = w - c * HN; }
+        else { n = w / P; c = w - n * P; }
+        const cplx<T> ua = buf[c * N + n];
+        const cplx<T> ub = buf[c * N + N - 1 - n];
+        const int l0 = 2 * c;
+        if (a.axis == 2) {
+          if (l0 < g.nvalid) {
+            T* o = a.out + g.base + (long long)l0 * N + 2 * n;
+            if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.x, s * ub.x};
+            else { o[0] = ua.x; o[1] = s * ub.x; }
+          }
+          if (l0 + 1 < g.nvalid) {
+            T* o = a.out + g.base + (long long)(l0 + 1) * N + 2 * n;
+            if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.y, s * ub.y};
+            else { o[0] = ua.y; o[1] = s * ub.y; }
+          }
+        } else {
+          T* o0 = a.out + g.base + l0 + (long long)(2 * n) * g.stride_m;
+          T* o1 = a.out + g.base + l0 + (long long)(2 * n + 1) * g.stride_m;
+          if (a.vec && l0 + 1 < g.nvalid) {
+            *reinterpret_cast<cplx<T>*>(o0) = ua;
+            *reinterpret_cast<cplx<T>*>(o1) = cplx<T>{s * ub.x, s * ub.y};
+          } else {
+            if (l0 < g.nvalid) { o0[0] = ua.x; o1[0] = s * ub.x; }
+            if (l0 + 1 < g.nvalid) { o0[1] = ua.y; o1[1] = s * ub.y; }
+          }
         }
       }
     }
   } else {
-    fft_stage<T, N, P, -1, 0>(buf, a.tw);
+    fft_stage<T, N, P, NT, MAXR, -1, 0>(buf, a.tw);
     // ---- post: Y0_k = Re(mk_k (Z_k + conj Z_{N-k})), Y1_k = Im(mk_k (Z_k - conj Z_{N-k}))
     constexpr int H = N / 2 + 1;
     constexpr int W = P * H;
-    constexpr int ITW = (W + kThreads - 1) / kThreads;
-#pragma unroll 4
+    constexpr int ITW = (W + NT - 1) / NT;
+#pragma unroll
     for (int it = 0; it < ITW; ++it) {
-      const int w = tid + it * kThreads;
+      const int w = tid + it * NT;
       if (w < W) {
-        int p, k;
-        if (a.axis == 2) { p = w / H; k = w - p * H; }
-        else { k = w / P; p = w - k * P; }
+        int c, k;
+        if (a.axis == 2) { c = w / H; k = w - c * H; }
+        else { k = w / P; c = w - k * P; }
         const int kr = (k == 0) ? 0 : N - k;
-        const cplx<T> zk = buf[p * N + (k < N ? k : 0)];
-        const cplx<T> zr = buf[p * N + kr];
+        const cplx<T> zk = buf[c * N + k];
+        const cplx<T> zr = buf[c * N + kr];
         T y0k, y1k, y0r, y1r;
         {
-          // output k
-          const cplx<T> m = a.mk[k < N ? k : 0];
+          const cplx<T> m = a.mk[k];
           const cplx<T> S = {zk.x + zr.x, zk.y - zr.y};
           const cplx<T> D = {zk.x - zr.x, zk.y + zr.y};
           y0k = m.x * S.x - m.y * S.y;
           y1k = m.x * D.y + m.y * D.x;
         }
         {
-          // output N-k: Z_{N-k} and conj Z_k
           const cplx<T> m = a.mk[kr];
           const cplx<T> S = {zr.x + zk.x, zr.y - zk.y};
           const cplx<T> D = {zr.x - zk.x, zr.y + zk.y};
@@ -401,15 +486,28 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs<T> a) {
         const int ok = sine ? (N - 1 - k) : k;
         const int orr = sine ? (N - 1 - kr) : kr;
         const bool has_r = (k > 0 && k < N - k);
-        if (p < g.nvalid) {
-          T* o = a.out + g.base + p * g.line_step;
-          if (k < N) o[ok * g.stride_m] = y0k;
-          if (has_r) o[orr * g.stride_m] = y0r;
-        }
-        if (p + P < g.nvalid) {
-          T* o = a.out + g.base + (p + P) * g.line_step;
-          if (k < N) o[ok * g.stride_m] = y1k;
-          if (has_r) o[orr * g.stride_m] = y1r;
+        const int l0 = 2 * c;
+        if (a.axis == 2) {
+          if (l0 < g.nvalid) {
+            T* o = a.out + g.base + (long long)l0 * N;
+            o[ok] = y0k;
+            if (has_r) o[orr] = y0r;
+          }
+          if (l0 + 1 < g.nvalid) {
+            T* o = a.out + g.base + (long long)(l0 + 1) * N;
+            o[ok] = y1k;
+            if (has_r) o[orr] = y1r;
+          }
+        } else {
+          T* ok_p = a.out + g.base + l0 + (long long)ok * g.stride_m;
+          T* or_p = a.out + g.base + l0 + (long long)orr * g.stride_m;
+          if (a.vec && l0 + 1 < g.nvalid) {
+            *reinterpret_cast<cplx<T>*>(ok_p) = cplx<T>{y0k, y1k};
+            if (has_r) *reinterpret_cast<cplx<T>*>(or_p) = cplx<T>{y0r, y1r};
+          } else {
+            if (l0 < g.nvalid) { ok_p[0] = y0k; if (has_r) or_p[0] = y0r; }
+            if (l0 + 1 < g.nvalid) { ok_p[1] = y1k; if (has_r) or_p[1] = y1r; }
+          }
         }
       }
     }
@@ -419,6 +517,8 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs<T> a) {
 // ---------------------------------------------------------------------------
 // direct O(N^2) fallback for any length
 // ---------------------------------------------------------------------------
+constexpr int kThreads = 256;
+
 template <typename T>
 struct DirectArgs {
   const T* in;
@@ -502,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> a) 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-bool R2R::fft_supported(int n) { return fft_radix(n, 0) != 0; }
+bool R2R::fft_supported(int n) { return fft_radix(n, 16, 0) != 0; }
 
 R2R::~R2R() {
   for (auto& kv : cache_) {
@@ -546,10 +646,11 @@ hipError_t R2R::tables(DType dt, int n, Tables** out) {
   return hipSuccess;
 }
 
-template <typename T, int N>
-static hipError_t launch_fft(const PassIO& io, const R2R* /*self*/, void* tw, void* mk, int kind,
-                             int axis, const Geom& g, hipStream_t s) {
-  constexpr int P = pairs_per_block<N>();
+static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <typename T, int N, int P, int NT>
+hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
+                         hipStream_t s) {
   constexpr int TL = 2 * P;
   PassArgs<T> a;
   a.in = static_cast<const T*>(io.in);
@@ -560,12 +661,38 @@ static hipError_t launch_fft(const PassIO& io, const R2R* /*self*/, void* tw, vo
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
   a.axis = axis;
   a.kind = kind;
+  a.vec = (g.nz % 2 == 0) && aligned16(io.in) && aligned16(io.out) &&
+          (io.in2 == nullptr || aligned16(io.in2));
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
   const size_t lds = (size_t)P * N * sizeof(cplx<T>);
-  hipLaunchKernelGGL((fft_pass_kernel<T, N>), dim3((unsigned)blocks), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT>), dim3((unsigned)blocks), dim3(NT), lds, s, a);
   return hipGetLastError();
+}
+
+// Tile and block size per (type, length, axis).  The strided y/x passes keep
+// TL*sizeof(T) >= 128 bytes so that every row segment is a whole cache line.
+template <typename T, int N>
+struct TileChoice {
+  static constexpr int cap(int p) { return p < 1 ? 1 : p; }
+  static constexpr int budget(int b) {
+    int p = b / N, q = 1;
+    while (q * 2 <= p) q *= 2;
+    return q;
+  }
+  static constexpr int PZ = cap(budget(2048) < 8 ? budget(2048) : 8);
+  static constexpr int PS = cap(budget(4096) < (int)(64 / sizeof(T)) ? budget(4096) : (int)(64 / sizeof(T)));
+  static constexpr int NTZ = 256;
+  static constexpr int NTS = 256;
+};
+
+template <typename T, int N>
+static hipError_t launch_fft(const PassIO& io, void* tw, void* mk, int kind, int axis,
+                             const Geom& g, hipStream_t s) {
+  using C = TileChoice<T, N>;
+  if (axis == 2) return launch_fft_pn<T, N, C::PZ, C::NTZ>(io, tw, mk, kind, axis, g, s);
+  return launch_fft_pn<T, N, C::PS, C::NTS>(io, tw, mk, kind, axis, g, s);
 }
 
 template <typename T>
@@ -592,20 +719,20 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
 }
 
 template <typename T>
-static hipError_t dispatch(int n, const PassIO& io, const R2R* self, void* tw, void* mk,
-                           void* ct, int kind, int axis, const Geom& g, hipStream_t s) {
+static hipError_t dispatch(int n, const PassIO& io, void* tw, void* mk, void* ct, int kind,
+                           int axis, const Geom& g, hipStream_t s) {
   switch (n) {
-    case 16: return launch_fft<T, 16>(io, self, tw, mk, kind, axis, g, s);
-    case 32: return launch_fft<T, 32>(io, self, tw, mk, kind, axis, g, s);
-    case 48: return launch_fft<T, 48>(io, self, tw, mk, kind, axis, g, s);
-    case 64: return launch_fft<T, 64>(io, self, tw, mk, kind, axis, g, s);
-    case 96: return launch_fft<T, 96>(io, self, tw, mk, kind, axis, g, s);
-    case 128: return launch_fft<T, 128>(io, self, tw, mk, kind, axis, g, s);
-    case 192: return launch_fft<T, 192>(io, self, tw, mk, kind, axis, g, s);
-    case 256: return launch_fft<T, 256>(io, self, tw, mk, kind, axis, g, s);
-    case 384: return launch_fft<T, 384>(io, self, tw, mk, kind, axis, g, s);
-    case 512: return launch_fft<T, 512>(io, self, tw, mk, kind, axis, g, s);
-    case 1024: return launch_fft<T, 1024>(io, self, tw, mk, kind, axis, g, s);
+    case 16: return launch_fft<T, 16>(io, tw, mk, kind, axis, g, s);
+    case 32: return launch_fft<T, 32>(io, tw, mk, kind, axis, g, s);
+    case 48: return launch_fft<T, 48>(io, tw, mk, kind, axis, g, s);
+    case 64: return launch_fft<T, 64>(io, tw, mk, kind, axis, g, s);
+    case 96: return launch_fft<T, 96>(io, tw, mk, kind, axis, g, s);
+    case 128: return launch_fft<T, 128>(io, tw, mk, kind, axis, g, s);
+    case 192: return launch_fft<T, 192>(io, tw, mk, kind, axis, g, s);
+    case 256: return launch_fft<T, 256>(io, tw, mk, kind, axis, g, s);
+    case 384: return launch_fft<T, 384>(io, tw, mk, kind, axis, g, s);
+    case 512: return launch_fft<T, 512>(io, tw, mk, kind, axis, g, s);
+    case 1024: return launch_fft<T, 1024>(io, tw, mk, kind, axis, g, s);
     default: return launch_direct<T>(io, ct, kind, axis, n, g, s);
   }
 }
@@ -616,8 +743,8 @@ hipError_t R2R::pass(DType dt, int kind, int axis, const PassIO& io, const Geom&
   Tables* tb = nullptr;
   hipError_t e = tables(dt, n, &tb);
   if (e != hipSuccess) return e;
-  if (dt == F64) return dispatch<double>(n, io, this, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
-  return dispatch<float>(n, io, this, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
+  if (dt == F64) return dispatch<double>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
+  return dispatch<float>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
 }
 
 hipError_t R2R::transform3d(DType dt, const int kinds[3], const PassIO& io, const Geom& g,

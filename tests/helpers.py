@@ -21,8 +21,10 @@ def rel_err(a, b, scale=None, l2=None):
     return np.abs(a - b).max() / scale, np.linalg.norm(a - b) / max(l2, 1e-300)
 
 
-VECTOR_GROUPS = [("gpx", "gpy", "gpz"), ("rdx", "rdy", "rdz"), ("vx", "vy", "vz"),
-                 ("vsolx", "vsoly", "vsolz"), ("virrx", "virry", "virrz")]
+# components of a vector share its scale; the solenoidal and irrotational
+# parts are measured against the whole velocity (either can vanish by symmetry)
+VECTOR_GROUPS = [("gpx", "gpy", "gpz"), ("rdx", "rdy", "rdz"),
+                 ("vx", "vy", "vz", "vsolx", "vsoly", "vsolz", "virrx", "virry", "virrz")]
 
 
 def group_scales(ref: dict):
