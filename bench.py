@@ -84,6 +84,35 @@ def kernel_timing(st, nsteps):
     return res
 
 
+def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000)):
+    """FP32-mixed vs FP64 on the same state: relative L2 errors of psi, rho and
+    v (all three components against the FP64 velocity's norm) at checkpoints.
+    The FP64 GPU stepper is the reference here; its own parity with the FP64
+    oracle is checked by the test suite."""
+    import numpy as np
+    a = S.Stepper(make_params(S, n, S.FP64))
+    b = S.Stepper(make_params(S, n, S.FP32_MIXED))
+    a.set_state(psi, rho)
+    b.set_state(psi, rho)
+    out, done = {}, 0
+    for c in checkpoints:
+        a.step(c - done)
+        b.step(c - done)
+        done = c
+        res = {}
+        for name in ("psi", "rho"):
+            fa, fb = a.field(name), b.field(name)
+            res[name] = float(np.linalg.norm(fb - fa) / np.linalg.norm(fa))
+        va = [a.field(k) for k in ("vx", "vy", "vz")]
+        vb = [b.field(k) for k in ("vx", "vy", "vz")]
+        num = math.sqrt(sum(float(np.linalg.norm(y - x)) ** 2 for x, y in zip(va, vb)))
+        den = math.sqrt(sum(float(np.linalg.norm(x)) ** 2 for x in va))
+        res["v"] = num / den
+        out[str(c)] = res
+        log(f"[bench] mixed vs fp64 at step {c}: {res}")
+    return out
+
+
 def cpu_baseline(n, psi, rho, seconds_budget=20.0):
     """The FP64 oracle (numpy + scipy.fft) on the host on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -117,6 +146,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--accuracy-steps", type=str, default="100,500,2000",
+                    help="checkpoints of the FP32-mixed vs FP64 comparison ('' to skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -155,6 +186,10 @@ def main():
                                       launches_per_step=v["launches"] / args.profile_steps,
                                       gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
                               for k, v in ktm.items()})
+
+    if mixed is not None and args.accuracy_steps:
+        cps = tuple(int(x) for x in args.accuracy_steps.split(","))
+        mixed["rel_l2_vs_fp64"] = mixed_accuracy(S, n, psi, rho, cps)
 
     # dominant kernel class of the FP64 stepper
     dom = max(kt64, key=lambda k: kt64[k]["total_ms"])

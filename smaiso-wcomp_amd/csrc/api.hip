@@ -71,11 +71,12 @@ struct smaiso_handle {
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
   double *partials = nullptr, *diag_out = nullptr, *out64 = nullptr;
   // work fields (T = double or float)
-  void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr, *rhoq = nullptr;
+  void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
+  double* rhoq = nullptr;  // DCT of rho, FP64 in both precisions (rho is ~constant: its
+                           // gradient is a small difference of large coefficients)
   void *gp[3] = {}, *rd[3] = {}, *S[3] = {}, *f[3] = {}, *vsolq[3] = {}, *virrq[3] = {}, *v[3] = {};
   void *amp = nullptr, *lpsi = nullptr, *laprl = nullptr, *mu = nullptr, *fgrad = nullptr,
        *erho = nullptr, *divv = nullptr, *divf = nullptr, *nl = nullptr, *tmp = nullptr;
-  void *rho_t = nullptr, *psiq_t = nullptr;  // FP32 copies of the FP64 state (mixed only)
 
   // timing
   bool timing = false;
@@ -146,25 +147,70 @@ struct smaiso_handle {
   }
 
   // --- transforms ----------------------------------------------------------
-  hipError_t xform(DType dt, const int kinds[3], const void* in, const void* in2, void* out) {
-    const size_t es = dt == F64 ? 8 : 4;
+  Fuse fuse_base() const {
+    Fuse f;
+    for (int a = 0; a < 3; ++a) { f.vq[a] = vq[a]; f.vs[a] = vs[a]; }
+    f.q02 = p.q0 * p.q0;
+    f.alpha = p.alpha;
+    f.scale = scale;
+    f.dt = p.dt;
+    f.rho_m = rho_m;
+    f.eps = p.epsilon;
+    return f;
+  }
+  // Algorithmic bytes of one pass: its inputs and outputs, N^3 each.
+  double pass_bytes(DType dt, const PassIO& io, int axis) const {
+    const double es = dt == F64 ? 8 : 4, n = (double)n3();
+    const int op = io.f.op;
+    double in = es, out = es;
+    if (op == OP_PLAIN) in = es * (io.in2 ? 2 : 1);
+    else if (op == OP_LD_GRAD_D || op == OP_LD_SH || op == OP_LD_RELAXLAP) in = 8;
+    else if (op == OP_LD_AMP) in = 4 * es;
+    else if (op == OP_LD_PROD) in = 8 + es;
+    else if (op == OP_LD_DIV) in = 3 * es;
+    else if (op == OP_LD_PSI) { in = 16 + 2 * es; out = es + 8; }
+    (void)axis;
+    return n * (in + out);
+  }
+  hipError_t run_pass(DType dt, int kind, int axis, const PassIO& io) {
+    Scope sc(this, axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X,
+             pass_bytes(dt, io, axis));
+    return r2r.pass(dt, kind, axis, io, g, stream);
+  }
+  // 3-D transform z, y, x; f's load op acts in the z pass
+  hipError_t xform(DType dt, const int kinds[3], const void* in, const void* in2, void* out,
+                   const Fuse* f = nullptr) {
     PassIO io;
     io.in = in; io.in2 = in2; io.out = out;
-    for (int ax = 2; ax >= 0; --ax) {
-      const double bytes = (double)n3() * es * ((io.in2 ? 2 : 1) + 1);
-      hipError_t e;
-      {
-        Scope sc(this, ax == 2 ? SMAISO_KC_R2R_Z : ax == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X, bytes);
-        e = r2r.pass(dt, kinds[ax], ax, io, g, stream);
-      }
-      if (e != hipSuccess) return e;
-      io.in = out;
-      io.in2 = nullptr;
-    }
-    return hipSuccess;
+    if (f) io.f = *f;
+    hipError_t e = run_pass(dt, kinds[2], 2, io);
+    if (e != hipSuccess) return e;
+    io.in = out; io.in2 = nullptr; io.f.op = OP_PLAIN;
+    if ((e = run_pass(dt, kinds[1], 1, io)) != hipSuccess) return e;
+    return run_pass(dt, kinds[0], 0, io);
   }
   hipError_t xw(const int kinds[3], const void* in, void* out, const void* in2 = nullptr) {
     return xform(mixed ? F32 : F64, kinds, in, in2, out);
+  }
+  hipError_t xwf(const int kinds[3], const Fuse& f, void* out) {
+    return xform(mixed ? F32 : F64, kinds, nullptr, nullptr, out, &f);
+  }
+  // DCT-II (z pass with f's load op), DCT-II along y, then one x pass doing
+  // DCT-II, the diagonal operator mid_op and DCT-III, then DCT-III along y, z.
+  hipError_t xmid(const Fuse& f, int mid_op, void* out) {
+    const DType dt = mixed ? F32 : F64;
+    PassIO io;
+    io.out = out;
+    io.f = f;
+    hipError_t e = run_pass(dt, REDFT10, 2, io);
+    if (e != hipSuccess) return e;
+    io.in = out; io.f.op = OP_PLAIN;
+    if ((e = run_pass(dt, REDFT10, 1, io)) != hipSuccess) return e;
+    io.f.op = mid_op;
+    if ((e = run_pass(dt, REDFT10, 0, io)) != hipSuccess) return e;
+    io.f.op = OP_PLAIN;
+    if ((e = run_pass(dt, REDFT01, 1, io)) != hipSuccess) return e;
+    return run_pass(dt, REDFT01, 2, io);
   }
   double fb(int nfields) const { return (double)n3() * tsz * nfields; }
 };
@@ -236,68 +282,55 @@ int run_step(smaiso_handle* h) {
   T* const* vsolq = reinterpret_cast<T* const*>(h->vsolq);
   T* const* virrq = reinterpret_cast<T* const*>(h->virrq);
   T* const* v = reinterpret_cast<T* const*>(h->v);
-  T* rho_t = static_cast<T*>(h->rho_t);
   const double fb1 = h->fb(1);
   const double db1 = (double)h->n3() * 8;
+  const Fuse fb = h->fuse_base();
 
   h->nloop += 1;
   std::swap(h->nlq, h->nlq_old);  // Nl_old_local = Nl_local, S:1100
 
-  // a6 (S:1104-1250)
-  HIPTRY(h, h->xw(kCT, rho_t, h->rhoq));
-  {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, db1 + fb1 * 7);
-    K::grad(c, h->psiq, static_cast<T*>(h->rhoq), gp[0], gp[1], gp[2], rd[0], rd[1], rd[2], s);
-  }
+  // a6 (S:1104-1250): rho^ = DCT(rho) in FP64; the DCT->DST shift and -Vs
+  // (times scale for rho) are applied on load by the z pass of each inverse
+  // transform
+  HIPTRY(h, h->xform(F64, kCT, h->rho, nullptr, h->rhoq));
   for (int a = 0; a < 3; ++a) {
-    HIPTRY(h, h->xw(kIST[a], gp[a], gp[a]));
-    HIPTRY(h, h->xw(kIST[a], rd[a], rd[a]));
+    Fuse g = fb;
+    g.axis = a;
+    g.op = OP_LD_GRAD_D; g.src[0] = h->psiq; g.c0 = 1.0;
+    HIPTRY(h, h->xwf(kIST[a], g, gp[a]));
+    g.op = OP_LD_GRAD_D; g.src[0] = h->rhoq; g.c0 = h->scale;
+    HIPTRY(h, h->xwf(kIST[a], g, rd[a]));
   }
-  // a7 (S:1254-1289)
+  // a7 (S:1254-1289): |amp| computed on load, Gaussian applied between the
+  // forward and inverse x passes
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 5);
-    K::amp(c, static_cast<T*>(h->psi), gp[0], gp[1], gp[2], static_cast<T*>(h->amp), s);
+    Fuse g = fb;
+    g.op = OP_LD_AMP;
+    g.src[0] = h->psi; g.src[1] = gp[0]; g.src[2] = gp[1]; g.src[3] = gp[2];
+    HIPTRY(h, h->xmid(g, OP_MID_AMP, h->amp));
   }
-  HIPTRY(h, h->xw(kCT, h->amp, h->amp));
-  {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 2);
-    K::spec(c, SPEC_AMP_SMOOTH, static_cast<T*>(h->amp), static_cast<T*>(h->amp), s);
-  }
-  HIPTRY(h, h->xw(kICT, h->amp, h->amp));
   // a8 (S:1293-1308)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, db1 + fb1);
-    K::psiq_op(c, PSIQ_SH, h->psiq, static_cast<T*>(h->lpsi), s);
+    Fuse g = fb;
+    g.op = OP_LD_SH; g.src[0] = h->psiq;
+    HIPTRY(h, h->xwf(kICT, g, h->lpsi));
   }
-  HIPTRY(h, h->xw(kICT, h->lpsi, h->lpsi));
   // a9 (S:1312-1380)
   if (h->nloop > h->p.divv_switch || h->p.load_mode == 1) {
-    {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 3);
-      K::prod(c, rho_t, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), s);
-    }
-    HIPTRY(h, h->xw(kCT, h->laprl, h->laprl));
-    {
-      smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 2);
-      K::spec(c, SPEC_NEGLAP, static_cast<T*>(h->laprl), static_cast<T*>(h->laprl), s);
-    }
-    HIPTRY(h, h->xw(kICT, h->laprl, h->laprl));
+    Fuse g = fb;
+    g.op = OP_LD_PROD; g.src[0] = h->rho; g.src[1] = h->lpsi;
+    HIPTRY(h, h->xmid(g, OP_MID_NEGLAP, h->laprl));
   } else {
-    {
-      smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, db1 + fb1);
-      K::psiq_op(c, PSIQ_RELAXLAP, h->psiq, static_cast<T*>(h->laprl), s);
-    }
-    HIPTRY(h, h->xw(kICT, h->laprl, h->laprl));
-    {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 3 + db1 + (h->mixed ? fb1 : 0));
-      K::relax(c, static_cast<T*>(h->amp), static_cast<T*>(h->laprl), h->rho,
-               h->mixed ? rho_t : nullptr, s);
-    }
+    Fuse g = fb;
+    g.op = OP_LD_RELAXLAP; g.src[0] = h->psiq;
+    HIPTRY(h, h->xwf(kICT, g, h->laprl));
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 3 + db1);
+    K::relax(c, static_cast<T*>(h->amp), static_cast<T*>(h->laprl), h->rho, s);
   }
   // a10 (S:1385-1425)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 20);
-    K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), rho_t,
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 19 + db1);
+    K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), h->rho,
               static_cast<T*>(h->amp), static_cast<T*>(h->psi), gp[0], gp[1], gp[2], rd[0],
               rd[1], rd[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
               static_cast<T*>(h->mu), S[0], S[1], S[2], f[0], f[1], f[2], s);
@@ -315,35 +348,42 @@ int run_step(smaiso_handle* h) {
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
                 vsolq[0], vsolq[1], vsolq[2], virrq[0], virrq[1], virrq[2], s);
   }
-  // a13 (S:1979-2080)
+  // a13 + a14 (S:1979-2117): the divergences are formed on load by the z
+  // pass of their inverse transforms; v = iST(vsol^ + virr^) by linearity
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 8);
-    K::divergence(c, virrq[0], virrq[1], virrq[2], f[0], f[1], f[2], static_cast<T*>(h->divv),
-                  static_cast<T*>(h->divf), s);
+    Fuse g = fb;
+    g.op = OP_LD_DIV; g.c0 = 1.0;
+    g.src[0] = virrq[0]; g.src[1] = virrq[1]; g.src[2] = virrq[2];
+    HIPTRY(h, h->xwf(kICT, g, h->divv));
+    g.c0 = h->scale;
+    g.src[0] = f[0]; g.src[1] = f[1]; g.src[2] = f[2];
+    HIPTRY(h, h->xwf(kICT, g, h->divf));
   }
-  // a14 (S:2085-2117): v = iST(vsol^ + virr^) by linearity of the transform
-  HIPTRY(h, h->xw(kICT, h->divv, h->divv));
-  HIPTRY(h, h->xw(kICT, h->divf, h->divf));
   for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vsolq[a], v[a], virrq[a]));
   // a15 + a16 (S:2121-2162)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 18 + db1 * 4 + (h->mixed ? fb1 : 0));
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 18 + db1 * 4);
     K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
-                  h->f_rho, h->mixed ? rho_t : nullptr, static_cast<T*>(h->mu),
+                  h->f_rho, static_cast<T*>(h->mu),
                   static_cast<T*>(h->amp), static_cast<T*>(h->psi), static_cast<T*>(h->divf),
                   gp[0], gp[1], gp[2], static_cast<T*>(h->nl), s);
   }
   // a16 tail (S:2166-2172)
   HIPTRY(h, h->xw(kCT, h->nl, h->nlq));
-  // a17 (S:2177-2198)
+  // a17 (S:2177-2198): the z pass of psi = iDCT(psiq_new) computes the
+  // update on load and stores psiq_new over psiq_old
   std::swap(h->psi, h->psi_prev);  // psi_old_local = psi_local, S:2192
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, db1 * 3 + fb1 * 2 + (h->mixed ? fb1 : 0));
-    K::psi_update(c, h->psiq, h->psiq_old, static_cast<T*>(h->nlq), static_cast<T*>(h->nlq_old),
-                  h->nloop == 1 ? 1 : 0, h->mixed ? static_cast<T*>(h->psiq_t) : nullptr, s);
+    Fuse g = fb;
+    g.op = OP_LD_PSI;
+    g.src[0] = h->nlq;
+    g.psiq = h->psiq;
+    g.psiq_new = h->psiq_old;
+    g.nlq_old = h->nlq_old;
+    g.first = h->nloop == 1 ? 1 : 0;  // quirk Q5
+    HIPTRY(h, h->xwf(kICT, g, h->psi));
   }
   std::swap(h->psiq, h->psiq_old);
-  HIPTRY(h, h->xw(kICT, h->mixed ? h->psiq_t : static_cast<void*>(h->psiq), h->psi));
   return SMAISO_OK;
 }
 
@@ -362,9 +402,7 @@ int do_set_state(smaiso_handle* h, const double* psi, const double* rho, hipMemc
   if (h->mixed) convert_d2f(h->f_rho, static_cast<float*>(h->psi), n3, s);
   else HIPTRY(h, hipMemcpyAsync(h->psi, h->f_rho, db, hipMemcpyDeviceToDevice, s));
   HIPTRY(h, hipMemcpyAsync(h->psiq_old, h->psiq, db, hipMemcpyDeviceToDevice, s));
-  if (h->mixed) convert_d2f(h->psiq, static_cast<float*>(h->psiq_t), n3, s);
   HIPTRY(h, hipMemcpyAsync(h->rho, rho, db, kind, s));
-  if (h->mixed) convert_d2f(h->rho, static_cast<float*>(h->rho_t), n3, s);
   fill_zero_d(h->f_rho, n3, s);
   HIPTRY(h, hipGetLastError());
   HIPTRY(h, hipStreamSynchronize(s));
@@ -407,7 +445,7 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_ENERGY:
       if (h->mixed)
         StepKernels<float>::energy_density(c, static_cast<float*>(h->psi), static_cast<float*>(h->lpsi),
-                                           static_cast<float*>(h->rho_t), static_cast<float*>(h->amp),
+                                           h->rho, static_cast<float*>(h->amp),
                                            static_cast<float*>(h->tmp), s);
       else
         StepKernels<double>::energy_density(c, static_cast<double*>(h->psi), static_cast<double*>(h->lpsi),
@@ -509,7 +547,7 @@ int smaiso_create(const smaiso_params* p, smaiso_handle** out) {
     if ((e = hipMemcpy(h->vs[a], w.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
       return hipfail(e, "hipMemcpy");
   }
-  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho};
+  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rhoq};
   for (double** b : dbufs) {
     if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc (state)");
     *b = static_cast<double*>(tmpp);
@@ -518,7 +556,7 @@ int smaiso_create(const smaiso_params* p, smaiso_handle** out) {
   h->partials = static_cast<double*>(tmpp);
   if ((e = alloc(8 * 4, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
   h->diag_out = static_cast<double*>(tmpp);
-  std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->rhoq, &h->amp,
+  std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
                                &h->divf, &h->nl, &h->tmp};
   for (int a = 0; a < 3; ++a) {
@@ -526,17 +564,11 @@ int smaiso_create(const smaiso_params* p, smaiso_handle** out) {
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vsolq[a]); tbufs.push_back(&h->virrq[a]);
     tbufs.push_back(&h->v[a]);
   }
-  if (h->mixed) {
-    tbufs.push_back(&h->rho_t);
-    tbufs.push_back(&h->psiq_t);
-  }
   for (void** b : tbufs)
     if ((e = alloc(tb, b)) != hipSuccess) return hipfail(e, "hipMalloc (work fields)");
   if (h->mixed) {
     if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
     h->out64 = static_cast<double*>(tmpp);
-  } else {
-    h->rho_t = h->rho;  // FP64: the work copy of rho is rho itself
   }
   *out = h.release();
   return SMAISO_OK;

@@ -16,12 +16,46 @@ namespace smaiso {
 
 enum DType : int { F64 = 0, F32 = 1 };
 
-// Optional fused operations on the load side of an axis pass:
-//   x = in[a] (+ in2[a])
+// Operations fused into an axis pass.  Load ops replace the element loads of
+// the first (z) pass; the mid ops make an x pass compute forward REDFT10,
+// multiply every mode by a diagonal operator and transform back with REDFT01;
+// OP_LD_PSI makes the z pass of the inverse transform of psi compute the psi
+// update on load and store the new coefficients on the way.
+enum FuseOp : int {
+  OP_PLAIN = 0,        // x = in (+ in2)
+  OP_LD_GRAD_D = 1,    // x = -c0 Vs_a[n_a] src[n + e_a]   (src double, 0 at n_a = N_a-1)
+  OP_LD_GRAD_T = 2,    // same with src of the work type
+  OP_LD_AMP = 3,       // x = sqrt(q0^2 s0^2 + s1^2 + s2^2 + s3^2)
+  OP_LD_SH = 4,        // x = alpha (q0^2 - |q|^2) psiq              (psiq double)
+  OP_LD_RELAXLAP = 5,  // x = -|q|^2 alpha (q0^2 - |q|^2) psiq
+  OP_LD_PROD = 6,      // x = s0 s1                                  (s0 double)
+  OP_LD_DIV = 7,       // x = c0 sum_a Vq_a[n_a] s_a[n - e_a]        (0 at n_a = 0)
+  OP_MID_AMP = 8,      // mode *= scale exp(-1.57*1.57*|q|^2*2)
+  OP_MID_NEGLAP = 9,   // mode *= -scale |q|^2
+  OP_LD_PSI = 10,      // x = new psiq from psiq, psiq_new (= old), Nl^ (src[0]), Nl^_old; stored to psiq_new
+  OP_COUNT = 11,
+};
+
+struct Fuse {
+  int op = OP_PLAIN;
+  const void* src[4] = {nullptr, nullptr, nullptr, nullptr};
+  int axis = 0;          // shift axis of OP_LD_GRAD_*
+  double c0 = 1.0;       // multiplier of OP_LD_GRAD_* and OP_LD_DIV
+  const double* vq[3] = {nullptr, nullptr, nullptr};  // DCT-lattice wavenumbers
+  const double* vs[3] = {nullptr, nullptr, nullptr};  // DST-lattice wavenumbers
+  double q02 = 0, alpha = 0, scale = 0, dt = 0, rho_m = 0, eps = 0;
+  // OP_LD_PSI
+  const double* psiq = nullptr;
+  double* psiq_new = nullptr;     // holds psiq_old on entry, overwritten in place
+  const void* nlq_old = nullptr;
+  int first = 0;                  // first step: Nl_old = Nl
+};
+
 struct PassIO {
   const void* in = nullptr;
-  const void* in2 = nullptr;   // added element-wise to `in` on load when set
+  const void* in2 = nullptr;   // added element-wise to `in` on load when set (OP_PLAIN)
   void* out = nullptr;
+  Fuse f;
 };
 
 class R2R {
@@ -31,7 +65,8 @@ class R2R {
   // One axis pass: out = kind_axis(in (+ in2)).  in may alias out.
   hipError_t pass(DType dt, int kind, int axis, const PassIO& io, const Geom& g,
                   hipStream_t s);
-  // Full 3-D transform with kinds[a] on axis a; passes run z, y, x.
+  // Full 3-D transform with kinds[a] on axis a; passes run z, y, x.  A load
+  // op in io.f applies to the z pass.
   hipError_t transform3d(DType dt, const int kinds[3], const PassIO& io, const Geom& g,
                          hipStream_t s);
   // True when length n runs on the LDS FFT path (else the direct O(n^2) path).

@@ -1,7 +1,9 @@
-// Per-mode (spectral) and pointwise (real-space) kernels of one time step.
+// Per-mode (spectral) and pointwise (real-space) kernels of one time step
+// that are not fused into a transform pass (the fused ones live in r2r.hip).
 // Each launcher enqueues one kernel on stream s.  T is the storage type of the
 // work fields (double for SMAISO_FP64, float for SMAISO_FP32_MIXED); the
-// time-integrated state psiq, psiq_old, rho and f_rho is always double.
+// time-integrated state psiq, psiq_old, rho and f_rho is always double, and the
+// pointwise kernels evaluate their expressions in FP64 registers.
 #pragma once
 
 #include "common.h"
@@ -20,32 +22,15 @@ struct StepConsts {
 
 enum SpecOp : int {
   SPEC_SCALE = 0,        // out = scale * in                         (pre-loop, S:982)
-  SPEC_AMP_SMOOTH = 1,   // out = scale * in * G(|q|^2)              (S:1279-1282)
-  SPEC_NEGLAP = 2,       // out = -scale |q|^2 in                    (S:1334-1337)
-};
-
-enum PsiqOp : int {
-  PSIQ_SH = 0,           // out = alpha (q0^2 - |q|^2) psiq          (S:1300-1301)
-  PSIQ_RELAXLAP = 1,     // out = -|q|^2 alpha (q0^2 - |q|^2) psiq   (S:1356-1357)
 };
 
 template <typename T>
 struct StepKernels {
-  // a6: spectral gradients of psiq (normalised) and rhoq (unnormalised DCT of rho)
-  static void grad(const StepConsts& c, const double* psiq, const T* rhoq, T* gpx, T* gpy, T* gpz,
-                   T* rdx, T* rdy, T* rdz, hipStream_t s);
-  // a7: |amp| = sqrt(q0^2 psi^2 + |grad psi|^2)
-  static void amp(const StepConsts& c, const T* psi, const T* gx, const T* gy, const T* gz, T* amp,
-                  hipStream_t s);
   static void spec(const StepConsts& c, int op, const T* in, T* out, hipStream_t s);
-  static void psiq_op(const StepConsts& c, int op, const double* psiq, T* out, hipStream_t s);
-  // w = rho L psi
-  static void prod(const StepConsts& c, const T* a, const T* b, T* out, hipStream_t s);
   // a9 relaxation branch: rho = rho0 + (1-rho0)/Amp amp; laprl *= rho
-  static void relax(const StepConsts& c, const T* amp, T* laprl, double* rho, T* rho_t,
-                    hipStream_t s);
-  // a10
-  static void forces(const StepConsts& c, const T* lpsi, const T* laprl, const T* rho,
+  static void relax(const StepConsts& c, const T* amp, T* laprl, double* rho, hipStream_t s);
+  // a10 (FP64 arithmetic)
+  static void forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                      const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
                      const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu, T* sx,
                      T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s);
@@ -53,21 +38,14 @@ struct StepKernels {
   static void velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                        const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
                        T* virry, T* virrz, hipStream_t s);
-  // a13
-  static void divergence(const StepConsts& c, const T* virrx, const T* virry, const T* virrz,
-                         const T* fxq, const T* fyq, const T* fzq, T* divv, T* divf,
-                         hipStream_t s);
-  // a15 + a16
+  // a15 + a16 (FP64 arithmetic)
   static void density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
                          const T* rdx, const T* rdy, const T* rdz, const T* divv, double* rho,
-                         double* f_rho, T* rho_t, const T* mu, const T* amp, const T* psi,
+                         double* f_rho, const T* mu, const T* amp, const T* psi,
                          const T* divf, const T* gpx, const T* gpy, const T* gpz, T* nl,
                          hipStream_t s);
-  // a17: psiq_new written over psiq_old; first != 0 uses nlq for nlq_old (quirk Q5)
-  static void psi_update(const StepConsts& c, const double* psiq, double* psiq_old_to_new,
-                         const T* nlq, const T* nlq_old, int first, T* psiq_t, hipStream_t s);
   // free-energy density (S:2227-2233) for field readout
-  static void energy_density(const StepConsts& c, const T* psi, const T* lpsi, const T* rho,
+  static void energy_density(const StepConsts& c, const T* psi, const T* lpsi, const double* rho,
                              const T* amp, T* out, hipStream_t s);
   // a18: 4 block-partial compensated sums -> partials[4*nblocks]; returns nblocks
   static int diag_partial(const StepConsts& c, const T* psi, const T* psi_old, const double* rho,

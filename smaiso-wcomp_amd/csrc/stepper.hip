@@ -30,114 +30,30 @@ __device__ __forceinline__ Idx decompose(unsigned idx, int ny, int nz) {
   return r;
 }
 
-// -1.57*1.57*|q|^2*2 evaluated left to right as in S:1281
-__device__ __forceinline__ double gauss(double mq2) { return exp(-1.57 * 1.57 * mq2 * 2); }
-
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
 // ---------------------------------------------------------------------------
-// a6: spectral gradients (S:1166-1221)
+// pre-loop scaling (S:982)
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_grad(StepConsts c, const double* __restrict__ psiq,
-                                                 const T* __restrict__ rhoq, T* gpx, T* gpy,
-                                                 T* gpz, T* rdx, T* rdy, T* rdz) {
-  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const Idx p = decompose(idx, c.ny, c.nz);
-  const unsigned sx = (unsigned)c.ny * c.nz, sy = (unsigned)c.nz;
-  double gx = 0, gy = 0, gz = 0, rx = 0, ry = 0, rz = 0;
-  if (p.i < c.nx - 1) {
-    const double v = c.vs[0][p.i];
-    gx = -v * psiq[idx + sx];
-    rx = -c.scale * v * (double)rhoq[idx + sx];
-  }
-  if (p.j < c.ny - 1) {
-    const double v = c.vs[1][p.j];
-    gy = -v * psiq[idx + sy];
-    ry = -c.scale * v * (double)rhoq[idx + sy];
-  }
-  if (p.k < c.nz - 1) {
-    const double v = c.vs[2][p.k];
-    gz = -v * psiq[idx + 1];
-    rz = -c.scale * v * (double)rhoq[idx + 1];
-  }
-  gpx[idx] = (T)gx; gpy[idx] = (T)gy; gpz[idx] = (T)gz;
-  rdx[idx] = (T)rx; rdy[idx] = (T)ry; rdz[idx] = (T)rz;
-}
-
-// ---------------------------------------------------------------------------
-// a7: amplitude (S:1254-1266)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_amp(StepConsts c, long long n3, const T* __restrict__ psi,
-                                                const T* __restrict__ gx, const T* __restrict__ gy,
-                                                const T* __restrict__ gz, T* __restrict__ amp) {
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const T ps = psi[idx], a = gx[idx], b = gy[idx], d = gz[idx];
-  amp[idx] = sqrt((T)c.q02 * (ps * ps) + a * a + b * b + d * d);
-}
-
-// ---------------------------------------------------------------------------
-// diagonal spectral multipliers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double mq2c_at(const StepConsts& c, const Idx& p) {
-  return sq(c.vq[0][p.i]) + sq(c.vq[1][p.j]) + sq(c.vq[2][p.k]);
-}
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spec(StepConsts c, int op, const T* __restrict__ in,
                                                  T* out) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
   const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n3) return;
-  const double x = (double)in[idx];
-  double r;
-  if (op == SPEC_SCALE) {
-    r = c.scale * x;
-  } else {
-    const Idx p = decompose(idx, c.ny, c.nz);
-    const double m = mq2c_at(c, p);
-    if (op == SPEC_AMP_SMOOTH) r = c.scale * x * gauss(m);
-    else r = -c.scale * m * x;
-  }
-  out[idx] = (T)r;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_psiq_op(StepConsts c, int op,
-                                                    const double* __restrict__ psiq, T* out) {
-  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const Idx p = decompose(idx, c.ny, c.nz);
-  const double m = mq2c_at(c, p);
-  double r;
-  if (op == PSIQ_SH) r = c.alpha * (c.q02 - m) * psiq[idx];
-  else r = -m * c.alpha * (c.q02 - m) * psiq[idx];
-  out[idx] = (T)r;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_prod(long long n3, const T* __restrict__ a,
-                                                 const T* __restrict__ b, T* __restrict__ out) {
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  out[idx] = a[idx] * b[idx];
+  (void)op;  // SPEC_SCALE
+  out[idx] = (T)(c.scale * (double)in[idx]);
 }
 
 // a9 relaxation branch (S:1364-1378): quirk Q4, rho is overwritten
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, long long n3,
                                                   const T* __restrict__ amp, T* laprl,
-                                                  double* rho, T* rho_t) {
+                                                  double* rho) {
   const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n3) return;
   const double r = c.rho0 + ((1 - c.rho0) / c.amp_eq) * (double)amp[idx];
   rho[idx] = r;
-  if (rho_t) rho_t[idx] = (T)r;
   laprl[idx] = (T)(r * (double)laprl[idx]);
 }
 
@@ -147,32 +63,34 @@ __global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, long long n3,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_forces(
     StepConsts c, long long n3, const T* __restrict__ lpsi, const T* __restrict__ laprl,
-    const T* __restrict__ rho, const T* __restrict__ amp, const T* __restrict__ psi,
+    const double* __restrict__ rho, const T* __restrict__ amp, const T* __restrict__ psi,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
     const T* __restrict__ rdx, const T* __restrict__ rdy, const T* __restrict__ rdz,
     T* __restrict__ fgrad, T* __restrict__ erho, T* __restrict__ mu, T* __restrict__ sx,
     T* __restrict__ sy, T* __restrict__ sz, T* __restrict__ fx, T* __restrict__ fy,
     T* __restrict__ fz) {
-  using A = T;  // arithmetic type of the pointwise terms
+  // FP64 arithmetic for both storage types: rho^2 - (rho0 + kp amp)^2 and
+  // rho - rho0 - kp amp cancel to a few 1e-3 inside the smectic
+  using A = double;
   const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n3) return;
   const A L = lpsi[idx], lr = laprl[idx], r = rho[idx], am = amp[idx], ps = psi[idx];
   const A kp = (A)c.kp, zeta = (A)c.zeta, rho0 = (A)c.rho0;
   const A e = A(0.5) * (L * L);
   const A ra = rho0 + kp * am;
-  fgrad[idx] = A(0.5) * zeta * (r * r - ra * ra);
+  fgrad[idx] = (T)(A(0.5) * zeta * (r * r - ra * ra));
   const A mu1 = lr + r * L;
   const A gx = gpx[idx], gy = gpy[idx], gz = gpz[idx];
-  sx[idx] = mu1 * gx + e * rdx[idx];
-  sy[idx] = mu1 * gy + e * rdy[idx];
-  sz[idx] = mu1 * gz + e * rdz[idx];
+  sx[idx] = (T)(mu1 * gx + e * (A)rdx[idx]);
+  sy[idx] = (T)(mu1 * gy + e * (A)rdy[idx]);
+  sz[idx] = (T)(mu1 * gz + e * (A)rdz[idx]);
   const A p2 = ps * ps, p3 = p2 * ps, p5 = p3 * p2;
-  mu[idx] = lr + r * (L - (A)c.eps * ps - (A)c.beta * p3 + (A)c.gamma * p5);
-  erho[idx] = r * e;
+  mu[idx] = (T)(lr + r * (L - (A)c.eps * ps - (A)c.beta * p3 + (A)c.gamma * p5));
+  erho[idx] = (T)(r * e);
   const A fc = kp * zeta * (r - rho0 - kp * am) / (am + A(0.001));
-  fx[idx] = fc * gx;
-  fy[idx] = fc * gy;
-  fz[idx] = fc * gz;
+  fx[idx] = (T)(fc * gx);
+  fy[idx] = (T)(fc * gy);
+  fz[idx] = (T)(fc * gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -247,28 +165,6 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
 }
 
 // ---------------------------------------------------------------------------
-// a13: divergences (S:2042-2080)
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_divergence(
-    StepConsts c, const T* __restrict__ virrx, const T* __restrict__ virry,
-    const T* __restrict__ virrz, const T* __restrict__ fxq, const T* __restrict__ fyq,
-    const T* __restrict__ fzq, T* __restrict__ divv, T* __restrict__ divf) {
-  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const Idx p = decompose(idx, c.ny, c.nz);
-  const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
-  double vxm = 0, fxm = 0, vym = 0, fym = 0, vzm = 0, fzm = 0;
-  if (p.i > 0) { vxm = (double)virrx[idx - sI]; fxm = (double)fxq[idx - sI]; }
-  if (p.j > 0) { vym = (double)virry[idx - sJ]; fym = (double)fyq[idx - sJ]; }
-  if (p.k > 0) { vzm = (double)virrz[idx - 1]; fzm = (double)fzq[idx - 1]; }
-  const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
-  divv[idx] = (T)(qx * vxm + qy * vym + qz * vzm);
-  divf[idx] = (T)(c.scale * (qx * fxm + qy * fym + qz * fzm));
-}
-
-// ---------------------------------------------------------------------------
 // a15 + a16: density AB2 (S:2121-2139) and the nonlinear term (S:2144-2162)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -276,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_density_nl(
     StepConsts c, long long n3, const T* __restrict__ vx, const T* __restrict__ vy,
     const T* __restrict__ vz, const T* __restrict__ rdx, const T* __restrict__ rdy,
     const T* __restrict__ rdz, const T* __restrict__ divv, double* __restrict__ rho,
-    double* __restrict__ f_rho, T* __restrict__ rho_t, const T* __restrict__ mu,
+    double* __restrict__ f_rho, const T* __restrict__ mu,
     const T* __restrict__ amp, const T* __restrict__ psi, const T* __restrict__ divf,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
     T* __restrict__ nl) {
@@ -290,43 +186,18 @@ __global__ __launch_bounds__(kBlock) void k_density_nl(
   const double r1 = r0 + c.dt * (1.5 * src - 0.5 * f_rho[idx]);
   rho[idx] = r1;
   f_rho[idx] = src;
-  if (rho_t) rho_t[idx] = (T)r1;
-  using A = T;
-  const A am = amp[idx];
-  const A kp = (A)c.kp;
-  const A v = -(A)mu[idx] + ((A)(c.kp * c.zeta) / (am + A(0.001))) * ((A)r1 - (A)c.rho0 - kp * am) * psi[idx] -
-              divf[idx] - (A)ux * gpx[idx] - (A)uy * gpy[idx] - (A)uz * gpz[idx];
-  nl[idx] = v;
-}
-
-// ---------------------------------------------------------------------------
-// a17: stabilised BDF2 / AB2 update (S:2177-2190), FP64 denominators
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_psi_update(StepConsts c, const double* __restrict__ psiq,
-                                                       double* psiq_old, const T* __restrict__ nlq,
-                                                       const T* __restrict__ nlq_old, int first,
-                                                       T* __restrict__ psiq_t) {
-  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const Idx p = decompose(idx, c.ny, c.nz);
-  const double m = mq2c_at(c, p);
-  const double alin = c.eps - c.alpha * sq(c.q02 - m);
-  const double c1 = 2.0 - c.dt * c.rho_m * alin;
-  const double c2 = 1.5 - c.dt * c.rho_m * alin;
-  const double n1 = (double)nlq[idx];
-  const double n0 = first ? n1 : (double)nlq_old[idx];
-  const double r = (c1 * psiq[idx] - 0.5 * psiq_old[idx] + (c.dt / 2) * c.scale * (3.0 * n1 - n0)) / c2;
-  psiq_old[idx] = r;
-  if (psiq_t) psiq_t[idx] = (T)r;
+  // FP64 arithmetic (rho_new - rho0 - kp amp cancels inside the smectic)
+  const double am = amp[idx];
+  const double v = -(double)mu[idx] + ((c.kp * c.zeta) / (am + 0.001)) * (r1 - c.rho0 - c.kp * am) * (double)psi[idx] -
+                   (double)divf[idx] - ux * (double)gpx[idx] - uy * (double)gpy[idx] - uz * (double)gpz[idx];
+  nl[idx] = (T)v;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_energy_density(StepConsts c, long long n3,
                                                            const T* __restrict__ psi,
                                                            const T* __restrict__ lpsi,
-                                                           const T* __restrict__ rho,
+                                                           const double* __restrict__ rho,
                                                            const T* __restrict__ amp,
                                                            T* __restrict__ out) {
   const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -443,38 +314,17 @@ __global__ void k_zero_d(double* __restrict__ out, long long n) {
 static inline long long n3_of(const StepConsts& c) { return (long long)c.nx * c.ny * c.nz; }
 
 template <typename T>
-void StepKernels<T>::grad(const StepConsts& c, const double* psiq, const T* rhoq, T* gpx, T* gpy,
-                          T* gpz, T* rdx, T* rdy, T* rdz, hipStream_t s) {
-  hipLaunchKernelGGL(k_grad<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, psiq, rhoq, gpx,
-                     gpy, gpz, rdx, rdy, rdz);
-}
-template <typename T>
-void StepKernels<T>::amp(const StepConsts& c, const T* psi, const T* gx, const T* gy, const T* gz,
-                         T* amp, hipStream_t s) {
-  hipLaunchKernelGGL(k_amp<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), psi, gx,
-                     gy, gz, amp);
-}
-template <typename T>
 void StepKernels<T>::spec(const StepConsts& c, int op, const T* in, T* out, hipStream_t s) {
   hipLaunchKernelGGL(k_spec<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, op, in, out);
 }
 template <typename T>
-void StepKernels<T>::psiq_op(const StepConsts& c, int op, const double* psiq, T* out,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_psiq_op<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, op, psiq, out);
-}
-template <typename T>
-void StepKernels<T>::prod(const StepConsts& c, const T* a, const T* b, T* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_prod<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, n3_of(c), a, b, out);
-}
-template <typename T>
-void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho, T* rho_t,
+void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_relax<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), amp,
-                     laprl, rho, rho_t);
+                     laprl, rho);
 }
 template <typename T>
-void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const T* rho,
+void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                             const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
                             const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu,
                             T* sx, T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s) {
@@ -490,31 +340,17 @@ void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, c
                      fgq, erq, vsolx, vsoly, vsolz, virrx, virry, virrz);
 }
 template <typename T>
-void StepKernels<T>::divergence(const StepConsts& c, const T* virrx, const T* virry,
-                                const T* virrz, const T* fxq, const T* fyq, const T* fzq, T* divv,
-                                T* divf, hipStream_t s) {
-  hipLaunchKernelGGL(k_divergence<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, virrx, virry,
-                     virrz, fxq, fyq, fzq, divv, divf);
-}
-template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
                                 const T* rdx, const T* rdy, const T* rdz, const T* divv,
-                                double* rho, double* f_rho, T* rho_t, const T* mu, const T* amp,
+                                double* rho, double* f_rho, const T* mu, const T* amp,
                                 const T* psi, const T* divf, const T* gpx, const T* gpy,
                                 const T* gpz, T* nl, hipStream_t s) {
   hipLaunchKernelGGL(k_density_nl<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), vx,
-                     vy, vz, rdx, rdy, rdz, divv, rho, f_rho, rho_t, mu, amp, psi, divf, gpx, gpy,
+                     vy, vz, rdx, rdy, rdz, divv, rho, f_rho, mu, amp, psi, divf, gpx, gpy,
                      gpz, nl);
 }
 template <typename T>
-void StepKernels<T>::psi_update(const StepConsts& c, const double* psiq, double* psiq_old_to_new,
-                                const T* nlq, const T* nlq_old, int first, T* psiq_t,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_psi_update<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, psiq,
-                     psiq_old_to_new, nlq, nlq_old, first, psiq_t);
-}
-template <typename T>
-void StepKernels<T>::energy_density(const StepConsts& c, const T* psi, const T* lpsi, const T* rho,
+void StepKernels<T>::energy_density(const StepConsts& c, const T* psi, const T* lpsi, const double* rho,
                                     const T* amp, T* out, hipStream_t s) {
   hipLaunchKernelGGL(k_energy_density<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c),
                      psi, lpsi, rho, amp, out);

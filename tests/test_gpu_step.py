@@ -173,3 +173,17 @@ def test_nloop_and_async():
     assert st.nloop == 0
     st.step(3)
     assert np.array_equal(a, st.field("psi"))
+
+
+def test_config2_full_size_two_steps():
+    """Config 2 at its full size (256^3, perturbed interface, rho 1.0/0.8,
+    dt=0.01): every field after 2 steps within 1e-10 of the FP64 oracle."""
+    kw = dict(nx=256, ny=256, nz=256, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0)
+    sp, op = params_pair(**kw)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(rho_iso=0.8, undulation=0.5, seed=2))
+    st, o = run_pair(sp, op, psi, rho, 2)
+    compare_all(st, o, TOL64, ["psi", "rho", "vx", "vy", "vz", "amp", "mu", "divv", "psiq", "lap_rl"])
+    l1, mass, en = st.diagnostics()
+    rl1, rmass, ren = o.diagnostics()
+    assert abs(mass - rmass) <= 1e-12 * abs(rmass)
+    assert abs(en - ren) <= 1e-10 * abs(ren)
