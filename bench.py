@@ -197,9 +197,21 @@ def main():
     avg_ms = d["total_ms"] / max(d["launches"], 1)
     bytes_per_launch = d["bytes"] / max(d["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_256cubed_fp64.json")
+    if n == 256 and os.path.exists(tpath):
+        # HBM bytes per launch of this kernel class from the committed
+        # rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this bench (copy-calibrated)
+        with open(tpath) as f:
+            tr = json.load(f)
+        cls = tr.get("r2r_z" if dom == "r2r_z" else "r2r_yx")
+        if cls:
+            traffic = cls["read_bytes"] + cls["write_bytes"]
+            traffic_src = tr.get("source")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None, kernel=dom,
-                    avg_launch_ms=avg_ms, algorithmic_bytes_per_launch=bytes_per_launch)
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=dom,
+                    avg_launch_ms=avg_ms, algorithmic_bytes_per_launch=bytes_per_launch,
+                    traffic_source=traffic_src)
     step_bytes = BYTES_PER_POINT["f64"] * n ** 3
     step_roof = dict(algorithmic_bytes_per_step=step_bytes,
                      achieved=step_bytes * sps64 / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",

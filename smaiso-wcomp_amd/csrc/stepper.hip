@@ -103,7 +103,13 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
     T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
+  // XCD-aware order: workgroups are dealt to the 8 XCDs round robin; map them
+  // so that each XCD walks a contiguous range of lines and finds the j+-1 and
+  // i+-1 neighbour lines in its own L2
+  const unsigned nb = gridDim.x;
+  unsigned b = blockIdx.x;
+  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  const unsigned idx = b * kBlock + threadIdx.x;
   if (idx >= n3) return;
   const Idx p = decompose(idx, c.ny, c.nz);
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
