@@ -159,6 +159,13 @@ int smaiso_synchronize(smaiso_handle* h);
 /* Steps completed since the last set_state (the reference's nLoop). */
 int64_t smaiso_step_count(const smaiso_handle* h);
 
+/* enable != 0: smaiso_step/smaiso_step_async replay pairs of steady steps
+ * (past the relaxation branch, S:1312-1380) from one captured hipGraph of two
+ * steps instead of launching them one by one (off by default; ignored while
+ * timing is enabled).  Results are identical to eager steps.  No reference
+ * counterpart: it replaces S:1081-2198 run twice. */
+int smaiso_set_graph(smaiso_handle* h, int32_t enable);
+
 /* Copy a field (SMAISO_FIELD_*) of the last completed step to host FP64 out
  * (Nx*Ny*Nz values, reference layout).  Replaces reading the reference's
  * arrays (psi_local, rho_local, ...) and its slice writers (S:2323-2446). */
@@ -214,6 +221,42 @@ int smaiso_timing_enable(smaiso_handle* h, int32_t enable);
  * class kc since timing was enabled. */
 int smaiso_timing_get(smaiso_handle* h, int32_t kc, double* total_ms, int64_t* launches,
                       double* bytes);
+
+/*
+ * Ensembles: `count` independent simulations (each one the reference's whole
+ * program, S:1081-2263, with its own parameters and grid) stepped together on
+ * one device.  Members that share grid shape, spacing, dt, alpha, beta,
+ * gamma, epsilon, q0, the effective rho_m, divv_switch, load_mode, precision
+ * and slab_n0 form one group and run as one batch (every kernel launch covers
+ * all members of the group); nu, lambda, zeta and rho0 may differ inside a
+ * group.  Groups run on their own streams, and steady steps of all groups
+ * replay from one captured hipGraph (smaiso_ensemble_set_graph, on by
+ * default).  Every member's results equal those of a single handle with the
+ * same parameters and state.  No reference counterpart (the reference runs one
+ * simulation per MPI job).
+ */
+typedef struct smaiso_ensemble smaiso_ensemble;
+
+int smaiso_ensemble_create(const smaiso_params* members, int32_t count, smaiso_ensemble** out);
+int smaiso_ensemble_destroy(smaiso_ensemble* e);
+/* Text of the last error on e (or of the last failed smaiso_ensemble_create when e is NULL). */
+const char* smaiso_ensemble_last_error(const smaiso_ensemble* e);
+/* Number of batches the members were grouped into. */
+int smaiso_ensemble_groups(const smaiso_ensemble* e);
+/* As smaiso_set_state for one member; every member must be set before the
+ * first step.  Setting a member restarts the step count (nLoop, with quirk
+ * Q5's first-step history) of the batch it belongs to, so set every member of
+ * a batch before stepping it again. */
+int smaiso_ensemble_set_state(smaiso_ensemble* e, int32_t member, const double* psi,
+                              const double* rho);
+int smaiso_ensemble_set_graph(smaiso_ensemble* e, int32_t enable);
+/* nsteps of every member; synchronous. */
+int smaiso_ensemble_step(smaiso_ensemble* e, int32_t nsteps);
+/* Steps since the member's batch was last set (-1 for a bad member). */
+int64_t smaiso_ensemble_step_count(const smaiso_ensemble* e, int32_t member);
+int smaiso_ensemble_get_field(smaiso_ensemble* e, int32_t member, int32_t field, double* out);
+int smaiso_ensemble_diagnostics(smaiso_ensemble* e, int32_t member, double* l1, double* mass,
+                                double* energy);
 
 #ifdef __cplusplus
 }

@@ -1,8 +1,10 @@
-// C ABI (include/smaiso.h): handle, device buffers and the step schedule.
+// C ABI (include/smaiso.h): handles, device buffers, the step schedule,
+// hipGraph replay and ensembles.
 //
-// One call of run_step<T>() is one iteration of the reference's loop body
-// (smaiso-wcomp.cpp:1081-2198) on one GPU.  The schedule follows the
-// reference's stage order; stages are kernels on the handle's stream.
+// A handle is a group of `nmem` simulations of one grid shape that step
+// together: every field holds the members back to back and every kernel takes
+// the member as blockIdx.y.  One call of run_step<T>() is one iteration of the
+// reference's loop body (smaiso-wcomp.cpp:1081-2198) for all members.
 #include "../../include/smaiso.h"
 #include "common.h"
 #include "r2r.h"
@@ -46,22 +48,31 @@ struct PendingTiming {
   double bytes;
 };
 
+double derived_amp(const smaiso_params& p) {
+  return std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.epsilon * p.gamma)) / 5);
+}
+
 }  // namespace
 
 struct smaiso_handle {
-  smaiso_params p{};
-  Geom g{};
+  smaiso_params p{};               // parameters shared by the group (member 0's)
+  std::vector<smaiso_params> mp;   // per member
+  int nmem = 1;
+  Geom g{};                        // g.nb = nmem
   int device = 0;
   hipStream_t stream = nullptr;
   R2R r2r;
   std::string err = "no error";
-  bool have_state = false;
-  long long nloop = 0;
+  std::vector<char> member_set;
+  long long nloop = 0;  // steps since the last set_state (the reference's nLoop)
+  long long epoch = 0;  // steps since creation: its parity names the buffer rotation
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
 
-  // derived (S:250-255, S:278)
-  double amp_eq = 0, kp = 0, rho_s = 0, rho_m = 0, scale = 0;
+  // derived (S:250-255, S:278), shared by the group
+  double amp_eq = 0, rho_m = 0, scale = 0;
+  std::vector<MemberConsts> mc_host;
+  MemberConsts* mc = nullptr;  // device
 
   // device memory
   std::vector<void*> allocs;
@@ -69,14 +80,24 @@ struct smaiso_handle {
   double* vs[3] = {nullptr, nullptr, nullptr};
   // FP64 state
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
-  double *partials = nullptr, *diag_out = nullptr, *out64 = nullptr;
-  // work fields (T = double or float)
-  void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
   double* rhoq = nullptr;  // DCT of rho, FP64 in both precisions (rho is ~constant: its
                            // gradient is a small difference of large coefficients)
+  double *partials = nullptr, *diag_out = nullptr, *out64 = nullptr;
+  std::vector<double> diag_host;
+  long long diag_epoch = -1;  // epoch of the cached diagnostics (-1: none)
+  // work fields (T = double or float)
+  void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
   void *gp[3] = {}, *rd[3] = {}, *S[3] = {}, *f[3] = {}, *vsolq[3] = {}, *virrq[3] = {}, *v[3] = {};
   void *amp = nullptr, *lpsi = nullptr, *laprl = nullptr, *mu = nullptr, *fgrad = nullptr,
        *erho = nullptr, *divv = nullptr, *divf = nullptr, *nl = nullptr, *tmp = nullptr;
+
+  // hipGraph replay of two steps (the three buffer swaps of a step cancel
+  // after two, so one graph serves every pair that starts at the same epoch
+  // parity)
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_parity = -1;
 
   // timing
   bool timing = false;
@@ -89,6 +110,8 @@ struct smaiso_handle {
   ~smaiso_handle() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (auto& pt : pending) { ev_free.push_back(pt.a); ev_free.push_back(pt.b); }
     for (auto e : ev_free) (void)hipEventDestroy(e);
     for (void* a : allocs) (void)hipFree(a);
@@ -98,16 +121,35 @@ struct smaiso_handle {
   StepConsts consts() const {
     StepConsts c{};
     c.nx = g.nx; c.ny = g.ny; c.nz = g.nz;
+    c.nmem = nmem;
     c.slab_n0 = p.slab_n0;
     c.scale = scale;
     c.dt = p.dt; c.alpha = p.alpha; c.beta = p.beta; c.gamma = p.gamma; c.eps = p.epsilon;
-    c.q0 = p.q0; c.q02 = p.q0 * p.q0; c.nu = p.nu; c.lam = p.lambda; c.zeta = p.zeta;
-    c.rho0 = p.rho0; c.kp = kp; c.amp_eq = amp_eq; c.rho_m = rho_m;
+    c.q0 = p.q0; c.q02 = p.q0 * p.q0; c.rho_m = rho_m;
     for (int a = 0; a < 3; ++a) { c.vq[a] = vq[a]; c.vs[a] = vs[a]; }
+    c.mc = mc;
     return c;
   }
+  // one member as a batch of one (pointers are offset by the caller)
+  StepConsts consts_one(int m) const {
+    StepConsts c = consts();
+    c.nmem = 1;
+    c.mc = mc + m;
+    return c;
+  }
+  Geom g_one() const { Geom q = g; q.nb = 1; return q; }
 
   long long n3() const { return g.n3(); }
+  long long ntot() const { return g.n3() * nmem; }
+  bool all_set() const {
+    for (char c : member_set) if (!c) return false;
+    return true;
+  }
+  // does step number n (1-based) take the steady branch with the full history?
+  bool steady(long long n) const {
+    const bool relax = !(n > p.divv_switch || p.load_mode == 1);
+    return !relax && n > 1;
+  }
 
   // --- timing helpers ----------------------------------------------------
   hipEvent_t take_event() {
@@ -159,8 +201,8 @@ struct smaiso_handle {
     return f;
   }
   // Algorithmic bytes of one pass: its inputs and outputs, N^3 each.
-  double pass_bytes(DType dt, const PassIO& io, int axis) const {
-    const double es = dt == F64 ? 8 : 4, n = (double)n3();
+  double pass_bytes(DType dt, const PassIO& io, const Geom& gg) const {
+    const double es = dt == F64 ? 8 : 4, n = (double)gg.n3() * gg.nb;
     const int op = io.f.op;
     double in = es, out = es;
     if (op == OP_PLAIN) in = es * (io.in2 ? 2 : 1);
@@ -169,25 +211,26 @@ struct smaiso_handle {
     else if (op == OP_LD_PROD) in = 8 + es;
     else if (op == OP_LD_DIV) in = 3 * es;
     else if (op == OP_LD_PSI) { in = 16 + 2 * es; out = es + 8; }
-    (void)axis;
     return n * (in + out);
   }
-  hipError_t run_pass(DType dt, int kind, int axis, const PassIO& io) {
+  hipError_t run_pass(DType dt, int kind, int axis, const PassIO& io, const Geom& gg) {
     Scope sc(this, axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X,
-             pass_bytes(dt, io, axis));
-    return r2r.pass(dt, kind, axis, io, g, stream);
+             pass_bytes(dt, io, gg));
+    return r2r.pass(dt, kind, axis, io, gg, stream);
   }
-  // 3-D transform z, y, x; f's load op acts in the z pass
+  // 3-D transform z, y, x of all members (or of gg's batch); f's load op acts
+  // in the z pass
   hipError_t xform(DType dt, const int kinds[3], const void* in, const void* in2, void* out,
-                   const Fuse* f = nullptr) {
+                   const Fuse* f = nullptr, const Geom* gg = nullptr) {
+    const Geom& G = gg ? *gg : g;
     PassIO io;
     io.in = in; io.in2 = in2; io.out = out;
     if (f) io.f = *f;
-    hipError_t e = run_pass(dt, kinds[2], 2, io);
+    hipError_t e = run_pass(dt, kinds[2], 2, io, G);
     if (e != hipSuccess) return e;
     io.in = out; io.in2 = nullptr; io.f.op = OP_PLAIN;
-    if ((e = run_pass(dt, kinds[1], 1, io)) != hipSuccess) return e;
-    return run_pass(dt, kinds[0], 0, io);
+    if ((e = run_pass(dt, kinds[1], 1, io, G)) != hipSuccess) return e;
+    return run_pass(dt, kinds[0], 0, io, G);
   }
   hipError_t xw(const int kinds[3], const void* in, void* out, const void* in2 = nullptr) {
     return xform(mixed ? F32 : F64, kinds, in, in2, out);
@@ -202,17 +245,41 @@ struct smaiso_handle {
     PassIO io;
     io.out = out;
     io.f = f;
-    hipError_t e = run_pass(dt, REDFT10, 2, io);
+    hipError_t e = run_pass(dt, REDFT10, 2, io, g);
     if (e != hipSuccess) return e;
     io.in = out; io.f.op = OP_PLAIN;
-    if ((e = run_pass(dt, REDFT10, 1, io)) != hipSuccess) return e;
+    if ((e = run_pass(dt, REDFT10, 1, io, g)) != hipSuccess) return e;
     io.f.op = mid_op;
-    if ((e = run_pass(dt, REDFT10, 0, io)) != hipSuccess) return e;
+    if ((e = run_pass(dt, REDFT10, 0, io, g)) != hipSuccess) return e;
     io.f.op = OP_PLAIN;
-    if ((e = run_pass(dt, REDFT01, 1, io)) != hipSuccess) return e;
-    return run_pass(dt, REDFT01, 2, io);
+    if ((e = run_pass(dt, REDFT01, 1, io, g)) != hipSuccess) return e;
+    return run_pass(dt, REDFT01, 2, io, g);
   }
-  double fb(int nfields) const { return (double)n3() * tsz * nfields; }
+  double fb(int nfields) const { return (double)ntot() * tsz * nfields; }
+};
+
+struct smaiso_ensemble {
+  std::vector<std::unique_ptr<smaiso_handle>> groups;
+  std::vector<std::pair<int, int>> where;  // member -> (group, index in group)
+  int device = 0;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_parity = -1;
+  bool use_graph = true;
+  std::string err = "no error";
+  ~smaiso_ensemble() {
+    (void)hipSetDevice(device);
+    if (master) (void)hipStreamSynchronize(master);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto e : joins) (void)hipEventDestroy(e);
+    groups.clear();
+    if (master) (void)hipStreamDestroy(master);
+  }
 };
 
 namespace {
@@ -270,6 +337,118 @@ int validate(const smaiso_params* p, std::string* why) {
   return SMAISO_OK;
 }
 
+double rho_m_eff(const smaiso_params& p) {
+  const double amp = derived_amp(p);
+  const double kp = (1 - p.rho0) / amp;
+  const double rho_s = kp * amp + p.rho0;  // S:253
+  return p.rho_m > 0 ? p.rho_m : rho_s / 2;
+}
+
+// Parameters two members must share to step in one batch.
+bool same_group(const smaiso_params& a, const smaiso_params& b) {
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dx == b.dx && a.dy == b.dy &&
+         a.dz == b.dz && a.dt == b.dt && a.alpha == b.alpha && a.beta == b.beta &&
+         a.gamma == b.gamma && a.epsilon == b.epsilon && a.q0 == b.q0 &&
+         rho_m_eff(a) == rho_m_eff(b) && a.divv_switch == b.divv_switch &&
+         a.load_mode == b.load_mode && a.precision == b.precision && a.slab_n0 == b.slab_n0;
+}
+
+// Create a group of n members (all valid, all same_group).
+int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
+  *out = nullptr;
+  std::unique_ptr<smaiso_handle> h(new smaiso_handle());
+  const smaiso_params& p = ps[0];
+  h->p = p;
+  h->mp.assign(ps, ps + n);
+  h->nmem = n;
+  h->member_set.assign(n, 0);
+  h->g = Geom{p.nx, p.ny, p.nz, n};
+  h->mixed = p.precision == SMAISO_FP32_MIXED;
+  h->tsz = h->mixed ? 4 : 8;
+  h->amp_eq = derived_amp(p);
+  h->rho_m = rho_m_eff(p);
+  h->scale = 0.125 / ((double)p.nx * p.ny * p.nz);
+  for (int m = 0; m < n; ++m) {
+    MemberConsts c;
+    c.nu = ps[m].nu;
+    c.lam = ps[m].lambda;
+    c.zeta = ps[m].zeta;
+    c.rho0 = ps[m].rho0;
+    c.amp_eq = h->amp_eq;
+    c.kp = (1 - ps[m].rho0) / h->amp_eq;  // S:252
+    h->mc_host.push_back(c);
+  }
+  h->diag_host.assign(4 * n, 0.0);
+
+  auto hipfail = [&](hipError_t e, const char* what) {
+    return fail(nullptr, e == hipErrorOutOfMemory ? SMAISO_E_NOMEM : SMAISO_E_HIP,
+                fmt("%s: %s", what, hipGetErrorString(e)));
+  };
+  hipError_t e;
+  if ((e = hipGetDevice(&h->device)) != hipSuccess) return hipfail(e, "hipGetDevice");
+  if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+    return hipfail(e, "hipStreamCreate");
+  auto alloc = [&](size_t bytes, void** ptr) -> hipError_t {
+    hipError_t r = hipMalloc(ptr, bytes);
+    if (r == hipSuccess) h->allocs.push_back(*ptr);
+    return r;
+  };
+  const long long nt = h->ntot();
+  const size_t db = (size_t)nt * 8, tb = (size_t)nt * h->tsz;
+  void* tmpp = nullptr;
+  // wavenumbers, S:447-481 (same expressions as the reference)
+  const int nn[3] = {p.nx, p.ny, p.nz};
+  const double dd[3] = {p.dx, p.dy, p.dz};
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> q(nn[a]), w(nn[a]);
+    for (int i = 0; i < nn[a]; ++i) {
+      q[i] = M_PI * i / (dd[a] * nn[a]);
+      w[i] = M_PI * (i + 1) / (dd[a] * nn[a]);
+    }
+    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+    h->vq[a] = static_cast<double*>(tmpp);
+    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+    h->vs[a] = static_cast<double*>(tmpp);
+    if ((e = hipMemcpy(h->vq[a], q.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return hipfail(e, "hipMemcpy");
+    if ((e = hipMemcpy(h->vs[a], w.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return hipfail(e, "hipMemcpy");
+  }
+  if ((e = alloc(sizeof(MemberConsts) * n, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+  h->mc = static_cast<MemberConsts*>(tmpp);
+  if ((e = hipMemcpy(h->mc, h->mc_host.data(), sizeof(MemberConsts) * n, hipMemcpyHostToDevice)) != hipSuccess)
+    return hipfail(e, "hipMemcpy");
+  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rhoq};
+  for (double** b : dbufs) {
+    if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc (state)");
+    *b = static_cast<double*>(tmpp);
+  }
+  if ((e = alloc((size_t)8 * 8 * kDiagBlocks * n, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+  h->partials = static_cast<double*>(tmpp);
+  if ((e = alloc((size_t)8 * 4 * n, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+  h->diag_out = static_cast<double*>(tmpp);
+  std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
+                               &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
+                               &h->divf, &h->nl, &h->tmp};
+  for (int a = 0; a < 3; ++a) {
+    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
+    tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vsolq[a]); tbufs.push_back(&h->virrq[a]);
+    tbufs.push_back(&h->v[a]);
+  }
+  for (void** b : tbufs)
+    if ((e = alloc(tb, b)) != hipSuccess) return hipfail(e, "hipMalloc (work fields)");
+  if (h->mixed) {
+    if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+    h->out64 = static_cast<double*>(tmpp);
+  }
+  for (int a = 0; a < 3; ++a) {
+    if ((e = h->r2r.prepare(F64, nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
+    if (h->mixed && (e = h->r2r.prepare(F32, nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
+  }
+  *out = h.release();
+  return SMAISO_OK;
+}
+
 template <typename T>
 int run_step(smaiso_handle* h) {
   using K = StepKernels<T>;
@@ -283,10 +462,11 @@ int run_step(smaiso_handle* h) {
   T* const* virrq = reinterpret_cast<T* const*>(h->virrq);
   T* const* v = reinterpret_cast<T* const*>(h->v);
   const double fb1 = h->fb(1);
-  const double db1 = (double)h->n3() * 8;
+  const double db1 = (double)h->ntot() * 8;
   const Fuse fb = h->fuse_base();
 
   h->nloop += 1;
+  h->epoch += 1;
   std::swap(h->nlq, h->nlq_old);  // Nl_old_local = Nl_local, S:1100
 
   // a6 (S:1104-1250): rho^ = DCT(rho) in FP64; the DCT->DST shift and -Vs
@@ -362,11 +542,11 @@ int run_step(smaiso_handle* h) {
   for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vsolq[a], v[a], virrq[a]));
   // a15 + a16 (S:2121-2162)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 18 + db1 * 4);
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
     K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
-                  h->f_rho, static_cast<T*>(h->mu),
-                  static_cast<T*>(h->amp), static_cast<T*>(h->psi), static_cast<T*>(h->divf),
-                  gp[0], gp[1], gp[2], static_cast<T*>(h->nl), s);
+                  h->f_rho, static_cast<T*>(h->mu), static_cast<T*>(h->amp),
+                  static_cast<T*>(h->psi), static_cast<T*>(h->divf), gp[0], gp[1], gp[2],
+                  static_cast<T*>(h->nl), s);
   }
   // a16 tail (S:2166-2172)
   HIPTRY(h, h->xw(kCT, h->nl, h->nlq));
@@ -387,39 +567,45 @@ int run_step(smaiso_handle* h) {
   return SMAISO_OK;
 }
 
-int do_set_state(smaiso_handle* h, const double* psi, const double* rho, hipMemcpyKind kind) {
+int run_step_any(smaiso_handle* h) { return h->mixed ? run_step<float>(h) : run_step<double>(h); }
+
+// Pre-loop of member m (S:965-1001) in FP64 for both precisions:
+// psiq = scale DCT(psi), psi = iDCT(psiq), psiq_old = psiq, f_rho = 0.
+int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, hipMemcpyKind kind) {
   HIPTRY(h, hipSetDevice(h->device));
-  const long long n3 = h->n3();
+  const long long n3 = h->n3(), off = (long long)m * n3;
   const size_t db = (size_t)n3 * 8;
   hipStream_t s = h->stream;
-  const StepConsts c = h->consts();
-  // pre-loop S:965-1001 in FP64 for both precisions: psiq = scale DCT(psi),
-  // psi = iDCT(psiq), psiq_old = psiq; f_rho and the history reset (Q5)
-  HIPTRY(h, hipMemcpyAsync(h->f_rho, psi, db, kind, s));
-  HIPTRY(h, h->xform(F64, kCT, h->f_rho, nullptr, h->psiq));
-  StepKernels<double>::spec(c, SPEC_SCALE, h->psiq, h->psiq, s);
-  HIPTRY(h, h->xform(F64, kICT, h->psiq, nullptr, h->f_rho));
-  if (h->mixed) convert_d2f(h->f_rho, static_cast<float*>(h->psi), n3, s);
-  else HIPTRY(h, hipMemcpyAsync(h->psi, h->f_rho, db, hipMemcpyDeviceToDevice, s));
-  HIPTRY(h, hipMemcpyAsync(h->psiq_old, h->psiq, db, hipMemcpyDeviceToDevice, s));
-  HIPTRY(h, hipMemcpyAsync(h->rho, rho, db, kind, s));
-  fill_zero_d(h->f_rho, n3, s);
+  const StepConsts c = h->consts_one(m);
+  const Geom g1 = h->g_one();
+  double* psiq = h->psiq + off;
+  double* scratch = h->f_rho + off;
+  HIPTRY(h, hipMemcpyAsync(scratch, psi, db, kind, s));
+  HIPTRY(h, h->xform(F64, kCT, scratch, nullptr, psiq, nullptr, &g1));
+  StepKernels<double>::spec(c, SPEC_SCALE, psiq, psiq, s);
+  HIPTRY(h, h->xform(F64, kICT, psiq, nullptr, scratch, nullptr, &g1));
+  if (h->mixed) convert_d2f(scratch, static_cast<float*>(h->psi) + off, n3, s);
+  else HIPTRY(h, hipMemcpyAsync(static_cast<double*>(h->psi) + off, scratch, db, hipMemcpyDeviceToDevice, s));
+  HIPTRY(h, hipMemcpyAsync(h->psiq_old + off, psiq, db, hipMemcpyDeviceToDevice, s));
+  HIPTRY(h, hipMemcpyAsync(h->rho + off, rho, db, kind, s));
+  fill_zero_d(scratch, n3, s);
   HIPTRY(h, hipGetLastError());
   HIPTRY(h, hipStreamSynchronize(s));
   h->collect_timing();
-  h->nloop = 0;
-  h->have_state = true;
+  h->nloop = 0;  // the whole group restarts its step count (quirk Q5)
+  h->member_set[m] = 1;
+  h->diag_epoch = -1;
   return SMAISO_OK;
 }
 
-// Produce field `fid` as FP64 on the device; returns a device pointer
-// (possibly a handle scratch buffer).
+// Produce field `fid` of all members as FP64 on the device; returns the
+// device pointer of member 0 (possibly a handle scratch buffer).
 int field_device(smaiso_handle* h, int fid, const double** out) {
-  const long long n3 = h->n3();
+  const long long nt = h->ntot();
   hipStream_t s = h->stream;
   auto as64 = [&](void* work) -> const double* {
     if (!h->mixed) return static_cast<const double*>(work);
-    convert_f2d(static_cast<const float*>(work), h->out64, n3, s);
+    convert_f2d(static_cast<const float*>(work), h->out64, nt, s);
     return h->out64;
   };
   const StepConsts c = h->consts();
@@ -467,6 +653,158 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
   }
 }
 
+int get_field(smaiso_handle* h, int m, int field, void* out, hipMemcpyKind kind) {
+  if (!out) return fail(h, SMAISO_E_INVALID, "out is NULL");
+  if (!h->all_set()) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
+  HIPTRY(h, hipSetDevice(h->device));
+  const double* src = nullptr;
+  int rc = field_device(h, field, &src);
+  if (rc != SMAISO_OK) return rc;
+  HIPTRY(h, hipMemcpyAsync(out, src + (long long)m * h->n3(), (size_t)h->n3() * 8, kind, h->stream));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  h->collect_timing();
+  return SMAISO_OK;
+}
+
+int diagnostics(smaiso_handle* h, int m, double* l1, double* mass, double* energy) {
+  if (!h->all_set() || h->nloop < 1)
+    return fail(h, SMAISO_E_STATE, "diagnostics need at least one completed step");
+  HIPTRY(h, hipSetDevice(h->device));
+  if (h->diag_epoch != h->epoch) {
+    const StepConsts c = h->consts();
+    hipStream_t s = h->stream;
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_REDUCE, h->fb(4) + (double)h->ntot() * 8);
+      int nb;
+      if (h->mixed)
+        nb = StepKernels<float>::diag_partial(c, static_cast<float*>(h->psi), static_cast<float*>(h->psi_prev),
+                                              h->rho, static_cast<float*>(h->lpsi),
+                                              static_cast<float*>(h->amp), h->partials, s);
+      else
+        nb = StepKernels<double>::diag_partial(c, static_cast<double*>(h->psi),
+                                               static_cast<double*>(h->psi_prev), h->rho,
+                                               static_cast<double*>(h->lpsi),
+                                               static_cast<double*>(h->amp), h->partials, s);
+      diag_final(h->partials, nb, h->nmem, h->diag_out, s);
+    }
+    HIPTRY(h, hipMemcpyAsync(h->diag_host.data(), h->diag_out, sizeof(double) * 4 * h->nmem,
+                             hipMemcpyDeviceToHost, s));
+    HIPTRY(h, hipStreamSynchronize(s));
+    h->collect_timing();
+    h->diag_epoch = h->epoch;
+  }
+  const double* sums = h->diag_host.data() + 4 * m;
+  const smaiso_params& p = h->mp[m];
+  const double vol = p.dx * p.dy * p.dz;
+  const double a = sums[0] / (p.dt * sums[1]);  // S:2246
+  const double ms = sums[2] * vol;               // S:2252
+  const double en = sums[3] * vol;               // S:2258
+  if (l1) *l1 = a;
+  if (mass) *mass = ms;
+  if (energy) *energy = en;
+  if (!std::isfinite(a) || !std::isfinite(ms) || !std::isfinite(en))
+    return fail(h, SMAISO_E_NONFINITE, "non-finite diagnostics (state blew up)");
+  return SMAISO_OK;
+}
+
+// Streams of a set of groups stepped together: each group works on its own
+// stream; `master` orders them against graph launches.  For a single handle
+// master is the handle's stream and fork/join are no-ops.
+struct Lanes {
+  std::vector<smaiso_handle*>& hs;
+  hipStream_t master;
+  hipEvent_t fork;
+  std::vector<hipEvent_t>& joins;
+  // group streams wait for master's work so far
+  hipError_t fork_out() {
+    bool any = false;
+    for (smaiso_handle* h : hs) any |= h->stream != master;
+    if (!any) return hipSuccess;
+    hipError_t e = hipEventRecord(fork, master);
+    for (size_t q = 0; q < hs.size() && e == hipSuccess; ++q)
+      if (hs[q]->stream != master) e = hipStreamWaitEvent(hs[q]->stream, fork, 0);
+    return e;
+  }
+  // master waits for every group stream's work so far
+  hipError_t join_in() {
+    hipError_t e = hipSuccess;
+    for (size_t q = 0; q < hs.size() && e == hipSuccess; ++q) {
+      if (hs[q]->stream == master) continue;
+      e = hipEventRecord(joins[q], hs[q]->stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(master, joins[q], 0);
+    }
+    return e;
+  }
+};
+
+// Capture two steps of every group into one graph on lanes.master.
+int capture_two_steps(smaiso_handle* owner, Lanes& L, hipGraph_t* graph, hipGraphExec_t* exec) {
+  HIPTRY(owner, hipStreamBeginCapture(L.master, hipStreamCaptureModeRelaxed));
+  int rc = SMAISO_OK;
+  hipError_t e = L.fork_out();
+  for (int k = 0; k < 2 && rc == SMAISO_OK && e == hipSuccess; ++k)
+    for (size_t q = 0; q < L.hs.size() && rc == SMAISO_OK; ++q) {
+      rc = run_step_any(L.hs[q]);
+      if (rc != SMAISO_OK) owner->err = L.hs[q]->err;
+    }
+  if (rc == SMAISO_OK && e == hipSuccess) e = L.join_in();
+  hipGraph_t gr = nullptr;
+  hipError_t e2 = hipStreamEndCapture(L.master, &gr);
+  if (rc != SMAISO_OK || e != hipSuccess || e2 != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    if (rc != SMAISO_OK) return rc;
+    HIPTRY(owner, e);
+    HIPTRY(owner, e2);
+  }
+  hipGraphExec_t ex = nullptr;
+  hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  if (e3 != hipSuccess) {
+    (void)hipGraphDestroy(gr);
+    HIPTRY(owner, e3);
+  }
+  *graph = gr;
+  *exec = ex;
+  return SMAISO_OK;
+}
+
+// Advance every group by nsteps: eager steps, except that pairs of steps that
+// all take the steady branch replay one captured two-step graph when
+// `use_graph` is set.  The work is ordered after master's earlier work and
+// master is ordered after all of it on return.
+int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
+            hipGraphExec_t* gexec, int* parity, int nsteps) {
+  bool need_fork = true, need_join = false;
+  for (int i = 0; i < nsteps;) {
+    bool pair_ok = use_graph && nsteps - i >= 2;
+    for (smaiso_handle* h : L.hs) pair_ok = pair_ok && h->steady(h->nloop + 1) && h->steady(h->nloop + 2);
+    const int par = (int)(L.hs[0]->epoch & 1);
+    if (pair_ok && (!*gexec || par == *parity)) {
+      if (need_join) { HIPTRY(owner, L.join_in()); need_join = false; }
+      if (!*gexec) {
+        // capturing records the two steps (and advances the counters)
+        int rc = capture_two_steps(owner, L, graph, gexec);
+        if (rc != SMAISO_OK) return rc;
+        *parity = par;
+      } else {
+        for (smaiso_handle* h : L.hs) { h->nloop += 2; h->epoch += 2; }
+      }
+      HIPTRY(owner, hipGraphLaunch(*gexec, L.master));
+      need_fork = true;
+      i += 2;
+      continue;
+    }
+    if (need_fork) { HIPTRY(owner, L.fork_out()); need_fork = false; }
+    for (smaiso_handle* h : L.hs) {
+      int rc = run_step_any(h);
+      if (rc != SMAISO_OK) { owner->err = h->err; return rc; }
+    }
+    need_join = true;
+    i += 1;
+  }
+  if (need_join) HIPTRY(owner, L.join_in());
+  return SMAISO_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -502,76 +840,7 @@ int smaiso_create(const smaiso_params* p, smaiso_handle** out) {
   std::string why;
   int rc = validate(p, &why);
   if (rc != SMAISO_OK) return fail(nullptr, rc, why);
-  std::unique_ptr<smaiso_handle> h(new smaiso_handle());
-  h->p = *p;
-  h->g = Geom{p->nx, p->ny, p->nz};
-  h->mixed = p->precision == SMAISO_FP32_MIXED;
-  h->tsz = h->mixed ? 4 : 8;
-  h->amp_eq = std::sqrt((3 * p->beta + std::sqrt(9 * p->beta * p->beta + 40 * p->epsilon * p->gamma)) / 5);
-  h->kp = (1 - p->rho0) / h->amp_eq;
-  h->rho_s = h->kp * h->amp_eq + p->rho0;
-  h->rho_m = p->rho_m > 0 ? p->rho_m : h->rho_s / 2;
-  h->scale = 0.125 / ((double)p->nx * p->ny * p->nz);
-
-  auto hipfail = [&](hipError_t e, const char* what) {
-    return fail(nullptr, e == hipErrorOutOfMemory ? SMAISO_E_NOMEM : SMAISO_E_HIP,
-                fmt("%s: %s", what, hipGetErrorString(e)));
-  };
-  hipError_t e;
-  if ((e = hipGetDevice(&h->device)) != hipSuccess) return hipfail(e, "hipGetDevice");
-  if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
-    return hipfail(e, "hipStreamCreate");
-  const long long n3 = h->g.n3();
-  auto alloc = [&](size_t bytes, void** ptr) -> hipError_t {
-    hipError_t r = hipMalloc(ptr, bytes);
-    if (r == hipSuccess) h->allocs.push_back(*ptr);
-    return r;
-  };
-  const size_t db = (size_t)n3 * 8, tb = (size_t)n3 * h->tsz;
-  void* tmpp = nullptr;
-  // wavenumbers, S:447-481 (same expressions as the reference)
-  const int nn[3] = {p->nx, p->ny, p->nz};
-  const double dd[3] = {p->dx, p->dy, p->dz};
-  for (int a = 0; a < 3; ++a) {
-    std::vector<double> q(nn[a]), w(nn[a]);
-    for (int n = 0; n < nn[a]; ++n) {
-      q[n] = M_PI * n / (dd[a] * nn[a]);
-      w[n] = M_PI * (n + 1) / (dd[a] * nn[a]);
-    }
-    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
-    h->vq[a] = static_cast<double*>(tmpp);
-    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
-    h->vs[a] = static_cast<double*>(tmpp);
-    if ((e = hipMemcpy(h->vq[a], q.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
-      return hipfail(e, "hipMemcpy");
-    if ((e = hipMemcpy(h->vs[a], w.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
-      return hipfail(e, "hipMemcpy");
-  }
-  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rhoq};
-  for (double** b : dbufs) {
-    if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc (state)");
-    *b = static_cast<double*>(tmpp);
-  }
-  if ((e = alloc(8 * 8 * 1024, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
-  h->partials = static_cast<double*>(tmpp);
-  if ((e = alloc(8 * 4, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
-  h->diag_out = static_cast<double*>(tmpp);
-  std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
-                               &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
-                               &h->divf, &h->nl, &h->tmp};
-  for (int a = 0; a < 3; ++a) {
-    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
-    tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vsolq[a]); tbufs.push_back(&h->virrq[a]);
-    tbufs.push_back(&h->v[a]);
-  }
-  for (void** b : tbufs)
-    if ((e = alloc(tb, b)) != hipSuccess) return hipfail(e, "hipMalloc (work fields)");
-  if (h->mixed) {
-    if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
-    h->out64 = static_cast<double*>(tmpp);
-  }
-  *out = h.release();
-  return SMAISO_OK;
+  return create_group(p, 1, out);
 }
 
 int smaiso_destroy(smaiso_handle* h) {
@@ -586,25 +855,26 @@ const char* smaiso_last_error(const smaiso_handle* h) {
 int smaiso_set_state(smaiso_handle* h, const double* psi, const double* rho) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
   if (!psi || !rho) return fail(h, SMAISO_E_INVALID, "psi/rho is NULL");
-  return do_set_state(h, psi, rho, hipMemcpyHostToDevice);
+  return do_set_state(h, 0, psi, rho, hipMemcpyHostToDevice);
 }
 
 int smaiso_set_state_device(smaiso_handle* h, const void* psi, const void* rho) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
   if (!psi || !rho) return fail(h, SMAISO_E_INVALID, "psi/rho is NULL");
-  return do_set_state(h, static_cast<const double*>(psi), static_cast<const double*>(rho),
+  return do_set_state(h, 0, static_cast<const double*>(psi), static_cast<const double*>(rho),
                       hipMemcpyDeviceToDevice);
 }
 
 int smaiso_step_async(smaiso_handle* h, int32_t nsteps) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
-  if (!h->have_state) return fail(h, SMAISO_E_STATE, "smaiso_step before smaiso_set_state");
+  if (!h->all_set()) return fail(h, SMAISO_E_STATE, "smaiso_step before smaiso_set_state");
   if (nsteps < 0) return fail(h, SMAISO_E_INVALID, "nsteps < 0");
   HIPTRY(h, hipSetDevice(h->device));
-  for (int i = 0; i < nsteps; ++i) {
-    int rc = h->mixed ? run_step<float>(h) : run_step<double>(h);
-    if (rc != SMAISO_OK) return rc;
-  }
+  std::vector<smaiso_handle*> hs{h};
+  std::vector<hipEvent_t> none;
+  Lanes L{hs, h->stream, nullptr, none};
+  int rc = advance(h, L, h->use_graph && !h->timing, &h->graph, &h->gexec, &h->graph_parity, nsteps);
+  if (rc != SMAISO_OK) return rc;
   HIPTRY(h, hipGetLastError());
   return SMAISO_OK;
 }
@@ -623,73 +893,162 @@ int smaiso_step(smaiso_handle* h, int32_t nsteps) {
   return smaiso_synchronize(h);
 }
 
+int smaiso_set_graph(smaiso_handle* h, int32_t enable) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  h->use_graph = enable != 0;
+  return SMAISO_OK;
+}
+
 int64_t smaiso_step_count(const smaiso_handle* h) { return h ? h->nloop : -1; }
 
 int smaiso_get_field_device(smaiso_handle* h, int32_t field, void* out_dev) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
-  if (!out_dev) return fail(h, SMAISO_E_INVALID, "out is NULL");
-  if (!h->have_state) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
-  HIPTRY(h, hipSetDevice(h->device));
-  const double* src = nullptr;
-  int rc = field_device(h, field, &src);
-  if (rc != SMAISO_OK) return rc;
-  HIPTRY(h, hipMemcpyAsync(out_dev, src, (size_t)h->n3() * 8, hipMemcpyDeviceToDevice, h->stream));
-  HIPTRY(h, hipStreamSynchronize(h->stream));
-  h->collect_timing();
-  return SMAISO_OK;
+  return get_field(h, 0, field, out_dev, hipMemcpyDeviceToDevice);
 }
 
 int smaiso_get_field(smaiso_handle* h, int32_t field, double* out) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
-  if (!out) return fail(h, SMAISO_E_INVALID, "out is NULL");
-  if (!h->have_state) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
-  HIPTRY(h, hipSetDevice(h->device));
-  const double* src = nullptr;
-  int rc = field_device(h, field, &src);
-  if (rc != SMAISO_OK) return rc;
-  HIPTRY(h, hipMemcpyAsync(out, src, (size_t)h->n3() * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPTRY(h, hipStreamSynchronize(h->stream));
-  h->collect_timing();
-  return SMAISO_OK;
+  return get_field(h, 0, field, out, hipMemcpyDeviceToHost);
 }
 
 int smaiso_diagnostics(smaiso_handle* h, double* l1, double* mass, double* energy) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
-  if (!h->have_state || h->nloop < 1)
-    return fail(h, SMAISO_E_STATE, "diagnostics need at least one completed step");
-  HIPTRY(h, hipSetDevice(h->device));
-  const StepConsts c = h->consts();
-  hipStream_t s = h->stream;
-  int nb;
-  {
-    smaiso_handle::Scope sc(h, SMAISO_KC_REDUCE, h->fb(4) + (double)h->n3() * 8);
-    if (h->mixed)
-      nb = StepKernels<float>::diag_partial(c, static_cast<float*>(h->psi), static_cast<float*>(h->psi_prev),
-                                            h->rho, static_cast<float*>(h->lpsi),
-                                            static_cast<float*>(h->amp), h->partials, s);
-    else
-      nb = StepKernels<double>::diag_partial(c, static_cast<double*>(h->psi),
-                                             static_cast<double*>(h->psi_prev), h->rho,
-                                             static_cast<double*>(h->lpsi),
-                                             static_cast<double*>(h->amp), h->partials, s);
-    diag_final(h->partials, nb, h->diag_out, s);
+  return diagnostics(h, 0, l1, mass, energy);
+}
+
+// --- ensembles --------------------------------------------------------------
+namespace {
+int efail(smaiso_ensemble* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+int member_of(smaiso_ensemble* e, int32_t member, smaiso_handle** h, int* m) {
+  if (!e) return efail(nullptr, SMAISO_E_INVALID, "ensemble is NULL");
+  if (member < 0 || member >= (int32_t)e->where.size())
+    return efail(e, SMAISO_E_INVALID, fmt("member %d out of range", member));
+  *h = e->groups[e->where[member].first].get();
+  *m = e->where[member].second;
+  return SMAISO_OK;
+}
+}  // namespace
+
+int smaiso_ensemble_create(const smaiso_params* members, int32_t count, smaiso_ensemble** out) {
+  if (!out) return efail(nullptr, SMAISO_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!members || count < 1) return efail(nullptr, SMAISO_E_INVALID, "need at least one member");
+  for (int i = 0; i < count; ++i) {
+    std::string why;
+    int rc = validate(&members[i], &why);
+    if (rc != SMAISO_OK) return efail(nullptr, rc, fmt("member %d: %s", i, why.c_str()));
   }
-  double sums[4];
-  HIPTRY(h, hipMemcpyAsync(sums, h->diag_out, sizeof(sums), hipMemcpyDeviceToHost, s));
-  HIPTRY(h, hipStreamSynchronize(s));
-  h->collect_timing();
-  const double vol = h->p.dx * h->p.dy * h->p.dz;
-  const double a = sums[0] / (h->p.dt * sums[1]);  // S:2246
-  const double m = sums[2] * vol;                   // S:2252
-  const double en = sums[3] * vol;                  // S:2258
-  if (l1) *l1 = a;
-  if (mass) *mass = m;
-  if (energy) *energy = en;
-  if (!std::isfinite(a) || !std::isfinite(m) || !std::isfinite(en))
-    return fail(h, SMAISO_E_NONFINITE, "non-finite diagnostics (state blew up)");
+  std::unique_ptr<smaiso_ensemble> e(new smaiso_ensemble());
+  // group members that can step in one batch, in order of first appearance
+  std::vector<std::vector<int>> groups;
+  for (int i = 0; i < count; ++i) {
+    size_t q = 0;
+    for (; q < groups.size(); ++q)
+      if (same_group(members[groups[q][0]], members[i])) break;
+    if (q == groups.size()) groups.emplace_back();
+    groups[q].push_back(i);
+  }
+  e->where.resize(count);
+  for (size_t q = 0; q < groups.size(); ++q) {
+    std::vector<smaiso_params> ps;
+    for (size_t k = 0; k < groups[q].size(); ++k) {
+      ps.push_back(members[groups[q][k]]);
+      e->where[groups[q][k]] = {(int)q, (int)k};
+    }
+    smaiso_handle* h = nullptr;
+    int rc = create_group(ps.data(), (int)ps.size(), &h);
+    if (rc != SMAISO_OK) return efail(nullptr, rc, g_create_error);
+    e->groups.emplace_back(h);
+  }
+  hipError_t he = hipGetDevice(&e->device);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  e->joins.resize(e->groups.size(), nullptr);
+  for (auto& j : e->joins)
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+  if (he != hipSuccess) return efail(nullptr, SMAISO_E_HIP, hipGetErrorString(he));
+  *out = e.release();
   return SMAISO_OK;
 }
 
+int smaiso_ensemble_destroy(smaiso_ensemble* e) {
+  delete e;
+  return SMAISO_OK;
+}
+
+const char* smaiso_ensemble_last_error(const smaiso_ensemble* e) {
+  return e ? e->err.c_str() : g_create_error.c_str();
+}
+
+int smaiso_ensemble_groups(const smaiso_ensemble* e) { return e ? (int)e->groups.size() : -1; }
+
+int smaiso_ensemble_set_state(smaiso_ensemble* e, int32_t member, const double* psi,
+                              const double* rho) {
+  smaiso_handle* h;
+  int m;
+  int rc = member_of(e, member, &h, &m);
+  if (rc != SMAISO_OK) return rc;
+  if (!psi || !rho) return efail(e, SMAISO_E_INVALID, "psi/rho is NULL");
+  rc = do_set_state(h, m, psi, rho, hipMemcpyHostToDevice);
+  if (rc != SMAISO_OK) return efail(e, rc, h->err);
+  return SMAISO_OK;
+}
+
+int smaiso_ensemble_set_graph(smaiso_ensemble* e, int32_t enable) {
+  if (!e) return efail(nullptr, SMAISO_E_INVALID, "ensemble is NULL");
+  e->use_graph = enable != 0;
+  return SMAISO_OK;
+}
+
+int smaiso_ensemble_step(smaiso_ensemble* e, int32_t nsteps) {
+  if (!e) return efail(nullptr, SMAISO_E_INVALID, "ensemble is NULL");
+  if (nsteps < 0) return efail(e, SMAISO_E_INVALID, "nsteps < 0");
+  for (auto& g : e->groups)
+    if (!g->all_set()) return efail(e, SMAISO_E_STATE, "every member needs smaiso_ensemble_set_state");
+  hipError_t he = hipSetDevice(e->device);
+  if (he != hipSuccess) return efail(e, SMAISO_E_HIP, hipGetErrorString(he));
+  std::vector<smaiso_handle*> hs;
+  for (auto& g : e->groups) hs.push_back(g.get());
+  smaiso_handle* owner = hs[0];
+  Lanes L{hs, e->master, e->fork, e->joins};
+  int rc = advance(owner, L, e->use_graph, &e->graph, &e->gexec, &e->graph_parity, nsteps);
+  if (rc != SMAISO_OK) return efail(e, rc, owner->err);
+  he = hipStreamSynchronize(e->master);
+  if (he != hipSuccess) return efail(e, SMAISO_E_HIP, hipGetErrorString(he));
+  return SMAISO_OK;
+}
+
+int64_t smaiso_ensemble_step_count(const smaiso_ensemble* e, int32_t member) {
+  if (!e || member < 0 || member >= (int32_t)e->where.size()) return -1;
+  return e->groups[e->where[member].first]->nloop;
+}
+
+int smaiso_ensemble_get_field(smaiso_ensemble* e, int32_t member, int32_t field, double* out) {
+  smaiso_handle* h;
+  int m;
+  int rc = member_of(e, member, &h, &m);
+  if (rc != SMAISO_OK) return rc;
+  rc = get_field(h, m, field, out, hipMemcpyDeviceToHost);
+  if (rc != SMAISO_OK) return efail(e, rc, h->err);
+  return SMAISO_OK;
+}
+
+int smaiso_ensemble_diagnostics(smaiso_ensemble* e, int32_t member, double* l1, double* mass,
+                                double* energy) {
+  smaiso_handle* h;
+  int m;
+  int rc = member_of(e, member, &h, &m);
+  if (rc != SMAISO_OK) return rc;
+  rc = diagnostics(h, m, l1, mass, energy);
+  if (rc != SMAISO_OK) return efail(e, rc, h->err);
+  return SMAISO_OK;
+}
+
+// --- standalone transforms ----------------------------------------------------
 namespace {
 std::mutex g_r2r_mu;
 std::map<int, std::unique_ptr<R2R>> g_r2r;  // per device

@@ -28,9 +28,11 @@ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
 }
 
 // Grid geometry of one field, row-major (i*ny + j)*nz + k with k fastest,
-// the reference's layout (smaiso-wcomp.cpp:943).
+// the reference's layout (smaiso-wcomp.cpp:943); nb fields of this shape lie
+// back to back (a batch of ensemble members).
 struct Geom {
   int nx, ny, nz;
+  int nb = 1;
   long long n3() const { return (long long)nx * ny * nz; }
 };
 

@@ -71,6 +71,12 @@ class R2R {
                          hipStream_t s);
   // True when length n runs on the LDS FFT path (else the direct O(n^2) path).
   static bool fft_supported(int n);
+  // Build the twiddle tables of length n now (they are otherwise built on
+  // first use, with synchronous copies that a stream capture cannot hold).
+  hipError_t prepare(DType dt, int n) {
+    Tables* t = nullptr;
+    return tables(dt, n, &t);
+  }
 
  private:
   struct Tables {

@@ -380,10 +380,10 @@ __device__ __forceinline__ T load_elem(const PassArgs<T>& a, int i, int j, int m
 }
 
 template <typename T>
-__device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m) {
+__device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo) {
   int i, j;
   line_ij(g, a.ny, i, j);
-  const long long idx = g * a.nz + m;
+  const long long idx = mo + g * a.nz + m;
   switch (a.f.op) {
     case OP_LD_GRAD_D: return load_elem<T, OP_LD_GRAD_D>(a, i, j, m, idx);
     case OP_LD_GRAD_T: return load_elem<T, OP_LD_GRAD_T>(a, i, j, m, idx);
@@ -495,7 +495,9 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   T* rbuf = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x;
-  const TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
+  TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
+  g.base += mo;
   const bool fwd = MID || (a.kind == REDFT10 || a.kind == RODFT10);
   const bool sine = !MID && (a.kind == RODFT10 || a.kind == RODFT01);
 
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           int i, j;
           const long long gl = (long long)g.outer + l;
           line_ij(gl, a.ny, i, j);
-          v[it] = load_elem<T, MID ? OP_PLAIN : OP>(a, i, j, m, gl * N + m);
+          v[it] = load_elem<T, MID ? OP_PLAIN : OP>(a, i, j, m, mo + gl * N + m);
         }
       }
 #pragma unroll
@@ -574,14 +576,14 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           int i, j;
           line_ij(g0, a.ny, i, j);
           if (l0 < g.nvalid) {
-            rk[it].x = load_elem<T, OP>(a, i, j, pa, g0 * N + pa);
-            rn[it].x = load_elem<T, OP>(a, i, j, pb, g0 * N + pb);
+            rk[it].x = load_elem<T, OP>(a, i, j, pa, mo + g0 * N + pa);
+            rn[it].x = load_elem<T, OP>(a, i, j, pb, mo + g0 * N + pb);
           }
           if (l0 + 1 < g.nvalid) {
             int i1 = i, j1 = j + 1;
             if (j1 == a.ny) { j1 = 0; ++i1; }
-            rk[it].y = load_elem<T, OP>(a, i1, j1, pa, (g0 + 1) * N + pa);
-            rn[it].y = load_elem<T, OP>(a, i1, j1, pb, (g0 + 1) * N + pb);
+            rk[it].y = load_elem<T, OP>(a, i1, j1, pa, mo + (g0 + 1) * N + pa);
+            rn[it].y = load_elem<T, OP>(a, i1, j1, pb, mo + (g0 + 1) * N + pb);
           }
         } else {
           rk[it] = load_pair(a, g, 2 * c, pa);
@@ -775,6 +777,8 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     if (a.axis == 1) { g.base = (long long)o * a.ny * a.nz + k0; g.stride_m = a.nz; }
     else { g.base = (long long)o * a.nz + k0; g.stride_m = (long long)a.ny * a.nz; }
   }
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
+  g.base += mo;
   const int E = tl * n;
   for (int e = threadIdx.x; e < E; e += blockDim.x) {
     int l, m;
@@ -783,7 +787,7 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     T v = T(0);
     if (l < g.nvalid) {
       if (a.axis == 2) {
-        v = load_elem_rt(a, (long long)g.outer + l, m);
+        v = load_elem_rt(a, (long long)g.outer + l, m, mo);
       } else {
         const long long addr = g.base + l * g.line_step + m * g.stride_m;
         v = a.in[addr];
@@ -854,9 +858,12 @@ hipError_t R2R::tables(DType dt, int n, Tables** out) {
     std::vector<float> f(v.begin(), v.end());
     return hipMemcpy(*d, f.data(), f.size() * es, hipMemcpyHostToDevice);
   };
-  if ((e = upload(tw, &tb.tw)) != hipSuccess) return e;
-  if ((e = upload(mk, &tb.mk)) != hipSuccess) return e;
-  if ((e = upload(ct, &tb.costab)) != hipSuccess) return e;
+  if ((e = upload(tw, &tb.tw)) != hipSuccess || (e = upload(mk, &tb.mk)) != hipSuccess ||
+      (e = upload(ct, &tb.costab)) != hipSuccess) {
+    for (void* d : {tb.tw, tb.mk, tb.costab})
+      if (d) (void)hipFree(d);
+    return e;
+  }
   auto res = cache_.emplace(key, tb);
   *out = &res.first->second;
   return hipSuccess;
@@ -890,7 +897,7 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
   const size_t lds = (size_t)P * (N + 1) * sizeof(cplx<T>);
-  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 
@@ -954,7 +961,7 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
   if (axis == 2) blocks = ((long long)g.nx * g.ny + tl - 1) / tl;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + tl - 1) / tl);
   const size_t lds = (size_t)tl * n * sizeof(T) * (mid ? 2 : 1);
-  hipLaunchKernelGGL((direct_pass_kernel<T>), dim3((unsigned)blocks), dim3(kThreads), lds, s, d);
+  hipLaunchKernelGGL((direct_pass_kernel<T>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(kThreads), lds, s, d);
   return hipGetLastError();
 }
 

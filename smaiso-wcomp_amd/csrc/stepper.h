@@ -4,20 +4,30 @@
 // work fields (double for SMAISO_FP64, float for SMAISO_FP32_MIXED); the
 // time-integrated state psiq, psiq_old, rho and f_rho is always double, and the
 // pointwise kernels evaluate their expressions in FP64 registers.
+//
+// Every kernel works on a batch of `nmem` members of one grid shape: member m
+// of a field lives at offset m * Nx*Ny*Nz and blockIdx.y is the member.
 #pragma once
 
 #include "common.h"
 
 namespace smaiso {
 
+// Parameters that may differ between the members of a batch.
+struct MemberConsts {
+  double nu, lam, zeta, rho0, kp, amp_eq;
+};
+
 // Scalars and wavenumber tables shared by every kernel of a step.
 struct StepConsts {
   int nx, ny, nz;
-  int slab_n0;        // reference slab height for quirk Q3
-  double scale;       // 1/(8 Nx Ny Nz), S:278
-  double dt, alpha, beta, gamma, eps, q0, q02, nu, lam, zeta, rho0, kp, amp_eq, rho_m;
+  int nmem;             // members in the batch
+  int slab_n0;          // reference slab height for quirk Q3
+  double scale;         // 1/(8 Nx Ny Nz), S:278
+  double dt, alpha, beta, gamma, eps, q0, q02, rho_m;
   const double* vq[3];  // DCT-lattice wavenumbers pi n/(d N), S:447-463
   const double* vs[3];  // DST-lattice wavenumbers pi (n+1)/(d N), S:465-481
+  const MemberConsts* mc;  // device array, nmem entries
 };
 
 enum SpecOp : int {
@@ -47,16 +57,20 @@ struct StepKernels {
   // free-energy density (S:2227-2233) for field readout
   static void energy_density(const StepConsts& c, const T* psi, const T* lpsi, const double* rho,
                              const T* amp, T* out, hipStream_t s);
-  // a18: 4 block-partial compensated sums -> partials[4*nblocks]; returns nblocks
+  // a18: per member, block-partial compensated sums of the 4 diagnostics ->
+  // partials[member][block][4][2]; returns the number of blocks per member
   static int diag_partial(const StepConsts& c, const T* psi, const T* psi_old, const double* rho,
                           const T* lpsi, const T* amp, double* partials, hipStream_t s);
 };
 
-// sum partials[4*nb] -> out[4] with compensated summation (one workgroup)
-void diag_final(const double* partials, int nb, double* out, hipStream_t s);
-// element-wise conversions / fills
+// sum partials -> out[member][4] with compensated summation (one workgroup per member)
+void diag_final(const double* partials, int nb, int nmem, double* out, hipStream_t s);
+// element-wise conversions / fills over n elements
 void convert_d2f(const double* in, float* out, long long n, hipStream_t s);
 void convert_f2d(const float* in, double* out, long long n, hipStream_t s);
 void fill_zero_d(double* out, long long n, hipStream_t s);
+
+// Largest number of diagnostic blocks per member.
+constexpr int kDiagBlocks = 1024;
 
 }  // namespace smaiso

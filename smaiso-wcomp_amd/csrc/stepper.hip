@@ -1,11 +1,13 @@
 // Per-mode and pointwise kernels of the time step (smaiso-wcomp.cpp:1081-2263).
 //
-// Every kernel is a flat grid over the Nx*Ny*Nz points with the reference's
-// layout (i*Ny + j)*Nz + k.  The reference's halo exchanges between x-slabs
-// (S:1108-1164, S:1492-1579, S:1713-1753, S:1979-2035) are plain +-1 index
-// shifts here; a shifted read outside the grid is 0, as on the first/last rank.
-// The operator tables of S:936-963 (mq2c, aLin, C1, C2, CM1a, CM2a) are
-// evaluated in registers from the per-axis wavenumber vectors.
+// Every kernel is a flat grid over the Nx*Ny*Nz points of one member (x
+// dimension of the launch) times the members of the batch (y dimension), with
+// the reference's layout (i*Ny + j)*Nz + k inside a member.  The reference's
+// halo exchanges between x-slabs (S:1108-1164, S:1492-1579, S:1713-1753,
+// S:1979-2035) are plain +-1 index shifts here; a shifted read outside the grid
+// is 0, as on the first/last rank.  The operator tables of S:936-963 (mq2c,
+// aLin, C1, C2, CM1a, CM2a) are evaluated in registers from the per-axis
+// wavenumber vectors.
 #include "stepper.h"
 
 #include <cmath>
@@ -14,6 +16,10 @@ namespace smaiso {
 
 constexpr int kBlock = 256;
 
+static inline long long n3_of(const StepConsts& c) { return (long long)c.nx * c.ny * c.nz; }
+static inline dim3 grid_of(const StepConsts& c) {
+  return dim3((unsigned)((n3_of(c) + kBlock - 1) / kBlock), (unsigned)c.nmem);
+}
 static inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 struct Idx {
@@ -32,6 +38,11 @@ __device__ __forceinline__ Idx decompose(unsigned idx, int ny, int nz) {
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// member offset of this workgroup
+__device__ __forceinline__ long long member_offset(const StepConsts& c) {
+  return (long long)blockIdx.y * ((long long)c.nx * c.ny * c.nz);
+}
+
 // ---------------------------------------------------------------------------
 // pre-loop scaling (S:982)
 // ---------------------------------------------------------------------------
@@ -39,20 +50,23 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spec(StepConsts c, int op, const T* __restrict__ in,
                                                  T* out) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
+  const unsigned li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= n3) return;
+  const long long idx = member_offset(c) + li;
   (void)op;  // SPEC_SCALE
   out[idx] = (T)(c.scale * (double)in[idx]);
 }
 
 // a9 relaxation branch (S:1364-1378): quirk Q4, rho is overwritten
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, long long n3,
-                                                  const T* __restrict__ amp, T* laprl,
-                                                  double* rho) {
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
-  const double r = c.rho0 + ((1 - c.rho0) / c.amp_eq) * (double)amp[idx];
+__global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, const T* __restrict__ amp,
+                                                  T* laprl, double* rho) {
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= n3) return;
+  const long long idx = member_offset(c) + li;
+  const MemberConsts m = c.mc[blockIdx.y];
+  const double r = m.rho0 + ((1 - m.rho0) / m.amp_eq) * (double)amp[idx];
   rho[idx] = r;
   laprl[idx] = (T)(r * (double)laprl[idx]);
 }
@@ -62,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_relax(StepConsts c, long long n3,
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_forces(
-    StepConsts c, long long n3, const T* __restrict__ lpsi, const T* __restrict__ laprl,
+    StepConsts c, const T* __restrict__ lpsi, const T* __restrict__ laprl,
     const double* __restrict__ rho, const T* __restrict__ amp, const T* __restrict__ psi,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
     const T* __restrict__ rdx, const T* __restrict__ rdy, const T* __restrict__ rdz,
@@ -72,10 +86,13 @@ __global__ __launch_bounds__(kBlock) void k_forces(
   // FP64 arithmetic for both storage types: rho^2 - (rho0 + kp amp)^2 and
   // rho - rho0 - kp amp cancel to a few 1e-3 inside the smectic
   using A = double;
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= n3) return;
+  const long long idx = member_offset(c) + li;
+  const MemberConsts m = c.mc[blockIdx.y];
   const A L = lpsi[idx], lr = laprl[idx], r = rho[idx], am = amp[idx], ps = psi[idx];
-  const A kp = (A)c.kp, zeta = (A)c.zeta, rho0 = (A)c.rho0;
+  const A kp = m.kp, zeta = m.zeta, rho0 = m.rho0;
   const A e = A(0.5) * (L * L);
   const A ra = rho0 + kp * am;
   fgrad[idx] = (T)(A(0.5) * zeta * (r * r - ra * ra));
@@ -98,8 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_velocity(
-    StepConsts c, const T* __restrict__ sxq, const T* __restrict__ syq,
-    const T* __restrict__ szq, const T* __restrict__ fgq, const T* __restrict__ erq,
+    StepConsts c, const T* __restrict__ sxq_, const T* __restrict__ syq_,
+    const T* __restrict__ szq_, const T* __restrict__ fgq_, const T* __restrict__ erq_,
     T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
     T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
@@ -111,12 +128,19 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
   const unsigned idx = b * kBlock + threadIdx.x;
   if (idx >= n3) return;
+  const long long mo = member_offset(c);
+  const T* sxq = sxq_ + mo;
+  const T* syq = syq_ + mo;
+  const T* szq = szq_ + mo;
+  const T* fgq = fgq_ + mo;
+  const T* erq = erq_ + mo;
+  const MemberConsts mm = c.mc[blockIdx.y];
   const Idx p = decompose(idx, c.ny, c.nz);
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
   const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
   const double wx = c.vs[0][p.i], wy = c.vs[1][p.j], wz = c.vs[2][p.k];
   const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
-  const double inu = c.nu, ilam = c.lam + 2 * c.nu;
+  const double inu = mm.nu, ilam = mm.lam + 2 * mm.nu;
   const double gfac = -1.57 * 1.57 * 2 * c.scale;
   const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
 
@@ -165,9 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
       rz = (cm1 / ilam) * (wz * fg_at(idx + 1, p.i, p.j, p.k + 1) + cm2 * dot * exp(gfac / cme));
     }
   }
-  vsolx[idx] = (T)ox; virrx[idx] = (T)rx;
-  vsoly[idx] = (T)oy; virry[idx] = (T)ry;
-  vsolz[idx] = (T)oz; virrz[idx] = (T)rz;
+  const long long o = mo + idx;
+  vsolx[o] = (T)ox; virrx[o] = (T)rx;
+  vsoly[o] = (T)oy; virry[o] = (T)ry;
+  vsolz[o] = (T)oz; virrz[o] = (T)rz;
 }
 
 // ---------------------------------------------------------------------------
@@ -175,15 +200,18 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_density_nl(
-    StepConsts c, long long n3, const T* __restrict__ vx, const T* __restrict__ vy,
+    StepConsts c, const T* __restrict__ vx, const T* __restrict__ vy,
     const T* __restrict__ vz, const T* __restrict__ rdx, const T* __restrict__ rdy,
     const T* __restrict__ rdz, const T* __restrict__ divv, double* __restrict__ rho,
     double* __restrict__ f_rho, const T* __restrict__ mu,
     const T* __restrict__ amp, const T* __restrict__ psi, const T* __restrict__ divf,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
     T* __restrict__ nl) {
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= n3) return;
+  const long long idx = member_offset(c) + li;
+  const MemberConsts m = c.mc[blockIdx.y];
   // density in FP64 (both precisions)
   const double ux = vx[idx], uy = vy[idx], uz = vz[idx];
   const double r0 = rho[idx];
@@ -194,26 +222,28 @@ __global__ __launch_bounds__(kBlock) void k_density_nl(
   f_rho[idx] = src;
   // FP64 arithmetic (rho_new - rho0 - kp amp cancels inside the smectic)
   const double am = amp[idx];
-  const double v = -(double)mu[idx] + ((c.kp * c.zeta) / (am + 0.001)) * (r1 - c.rho0 - c.kp * am) * (double)psi[idx] -
+  const double v = -(double)mu[idx] + ((m.kp * m.zeta) / (am + 0.001)) * (r1 - m.rho0 - m.kp * am) * (double)psi[idx] -
                    (double)divf[idx] - ux * (double)gpx[idx] - uy * (double)gpy[idx] - uz * (double)gpz[idx];
   nl[idx] = (T)v;
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_energy_density(StepConsts c, long long n3,
-                                                           const T* __restrict__ psi,
+__global__ __launch_bounds__(kBlock) void k_energy_density(StepConsts c, const T* __restrict__ psi,
                                                            const T* __restrict__ lpsi,
                                                            const double* __restrict__ rho,
                                                            const T* __restrict__ amp,
                                                            T* __restrict__ out) {
-  const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= n3) return;
+  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
+  const unsigned li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= n3) return;
+  const long long idx = member_offset(c) + li;
+  const MemberConsts m = c.mc[blockIdx.y];
   const double ps = psi[idx], L = lpsi[idx], r = rho[idx], am = amp[idx];
   const double p2 = ps * ps;
   const double e = -0.5 * c.eps * p2 + 0.5 * (L * L) - 0.25 * c.beta * (p2 * p2) +
                    (c.gamma / 6) * (p2 * p2 * p2);
-  const double d = r - c.rho0 - c.kp * am;
-  out[idx] = (T)(r * e + 0.5 * c.zeta * (d * d));
+  const double d = r - m.rho0 - m.kp * am;
+  out[idx] = (T)(r * e + 0.5 * m.zeta * (d * d));
 }
 
 // ---------------------------------------------------------------------------
@@ -238,19 +268,20 @@ __device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
   return r;
 }
 
-constexpr int kDiagBlocks = 1024;
-
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_diag(StepConsts c, long long n3,
-                                                 const T* __restrict__ psi,
+__global__ __launch_bounds__(kBlock) void k_diag(StepConsts c, const T* __restrict__ psi,
                                                  const T* __restrict__ psi_old,
                                                  const double* __restrict__ rho,
                                                  const T* __restrict__ lpsi,
                                                  const T* __restrict__ amp,
                                                  double* __restrict__ partials) {
+  const long long n3 = (long long)c.nx * c.ny * c.nz;
+  const long long mo = member_offset(c);
+  const MemberConsts m = c.mc[blockIdx.y];
   CSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  for (long long idx = (long long)blockIdx.x * kBlock + threadIdx.x; idx < n3;
-       idx += (long long)gridDim.x * kBlock) {
+  for (long long li = (long long)blockIdx.x * kBlock + threadIdx.x; li < n3;
+       li += (long long)gridDim.x * kBlock) {
+    const long long idx = mo + li;
     const double ps = psi[idx], po = psi_old[idx], r = rho[idx], L = lpsi[idx], am = amp[idx];
     csum_add(acc[0], fabs(ps - po));
     csum_add(acc[1], fabs(ps));
@@ -258,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_diag(StepConsts c, long long n3,
     const double p2 = ps * ps;
     const double e = -0.5 * c.eps * p2 + 0.5 * (L * L) - 0.25 * c.beta * (p2 * p2) +
                      (c.gamma / 6) * (p2 * p2 * p2);
-    const double d = r - c.rho0 - c.kp * am;
-    csum_add(acc[3], r * e + 0.5 * c.zeta * (d * d));
+    const double d = r - m.rho0 - m.kp * am;
+    csum_add(acc[3], r * e + 0.5 * m.zeta * (d * d));
   }
   __shared__ CSum sh[4][kBlock];
 #pragma unroll
@@ -274,18 +305,20 @@ __global__ __launch_bounds__(kBlock) void k_diag(StepConsts c, long long n3,
   }
   if (threadIdx.x < 4) {
     const CSum r = sh[threadIdx.x][0];
-    partials[2 * (4 * blockIdx.x + threadIdx.x)] = r.s;
-    partials[2 * (4 * blockIdx.x + threadIdx.x) + 1] = r.c;
+    double* pm = partials + (size_t)blockIdx.y * kDiagBlocks * 8;
+    pm[2 * (4 * blockIdx.x + threadIdx.x)] = r.s;
+    pm[2 * (4 * blockIdx.x + threadIdx.x) + 1] = r.c;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_diag_final(const double* __restrict__ partials, int nb,
                                                        double* __restrict__ out) {
+  const double* pm = partials + (size_t)blockIdx.x * kDiagBlocks * 8;
   CSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   for (int b = threadIdx.x; b < nb; b += kBlock) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      acc[q] = csum_merge(acc[q], CSum{partials[2 * (4 * b + q)], partials[2 * (4 * b + q) + 1]});
+      acc[q] = csum_merge(acc[q], CSum{pm[2 * (4 * b + q)], pm[2 * (4 * b + q) + 1]});
   }
   __shared__ CSum sh[4][kBlock];
 #pragma unroll
@@ -298,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(const double* __restrict_
     }
     __syncthreads();
   }
-  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0].s + sh[threadIdx.x][0].c;
+  if (threadIdx.x < 4) out[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0].s + sh[threadIdx.x][0].c;
 }
 
 __global__ void k_d2f(const double* __restrict__ in, float* __restrict__ out, long long n) {
@@ -317,33 +350,28 @@ __global__ void k_zero_d(double* __restrict__ out, long long n) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline long long n3_of(const StepConsts& c) { return (long long)c.nx * c.ny * c.nz; }
-
 template <typename T>
 void StepKernels<T>::spec(const StepConsts& c, int op, const T* in, T* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_spec<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, op, in, out);
+  hipLaunchKernelGGL(k_spec<T>, grid_of(c), dim3(kBlock), 0, s, c, op, in, out);
 }
 template <typename T>
-void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_relax<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), amp,
-                     laprl, rho);
+void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho, hipStream_t s) {
+  hipLaunchKernelGGL(k_relax<T>, grid_of(c), dim3(kBlock), 0, s, c, amp, laprl, rho);
 }
 template <typename T>
 void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                             const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
                             const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu,
                             T* sx, T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s) {
-  hipLaunchKernelGGL(k_forces<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), lpsi,
-                     laprl, rho, amp, psi, gpx, gpy, gpz, rdx, rdy, rdz, fgrad, erho, mu, sx, sy,
-                     sz, fx, fy, fz);
+  hipLaunchKernelGGL(k_forces<T>, grid_of(c), dim3(kBlock), 0, s, c, lpsi, laprl, rho, amp, psi,
+                     gpx, gpy, gpz, rdx, rdy, rdz, fgrad, erho, mu, sx, sy, sz, fx, fy, fz);
 }
 template <typename T>
 void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                               const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
                               T* virry, T* virrz, hipStream_t s) {
-  hipLaunchKernelGGL(k_velocity<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, sxq, syq, szq,
-                     fgq, erq, vsolx, vsoly, vsolz, virrx, virry, virrz);
+  hipLaunchKernelGGL(k_velocity<T>, grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
+                     vsolx, vsoly, vsolz, virrx, virry, virrz);
 }
 template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
@@ -351,15 +379,13 @@ void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, c
                                 double* rho, double* f_rho, const T* mu, const T* amp,
                                 const T* psi, const T* divf, const T* gpx, const T* gpy,
                                 const T* gpz, T* nl, hipStream_t s) {
-  hipLaunchKernelGGL(k_density_nl<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c), vx,
-                     vy, vz, rdx, rdy, rdz, divv, rho, f_rho, mu, amp, psi, divf, gpx, gpy,
-                     gpz, nl);
+  hipLaunchKernelGGL(k_density_nl<T>, grid_of(c), dim3(kBlock), 0, s, c, vx, vy, vz, rdx, rdy, rdz,
+                     divv, rho, f_rho, mu, amp, psi, divf, gpx, gpy, gpz, nl);
 }
 template <typename T>
 void StepKernels<T>::energy_density(const StepConsts& c, const T* psi, const T* lpsi, const double* rho,
                                     const T* amp, T* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_density<T>, dim3(grid_for(n3_of(c))), dim3(kBlock), 0, s, c, n3_of(c),
-                     psi, lpsi, rho, amp, out);
+  hipLaunchKernelGGL(k_energy_density<T>, grid_of(c), dim3(kBlock), 0, s, c, psi, lpsi, rho, amp, out);
 }
 template <typename T>
 int StepKernels<T>::diag_partial(const StepConsts& c, const T* psi, const T* psi_old,
@@ -367,13 +393,13 @@ int StepKernels<T>::diag_partial(const StepConsts& c, const T* psi, const T* psi
                                  hipStream_t s) {
   long long nb = grid_for(n3_of(c));
   if (nb > kDiagBlocks) nb = kDiagBlocks;
-  hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, c, n3_of(c), psi, psi_old,
-                     rho, lpsi, amp, partials);
+  hipLaunchKernelGGL(k_diag<T>, dim3((unsigned)nb, (unsigned)c.nmem), dim3(kBlock), 0, s, c, psi,
+                     psi_old, rho, lpsi, amp, partials);
   return (int)nb;
 }
 
-void diag_final(const double* partials, int nb, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, partials, nb, out);
+void diag_final(const double* partials, int nb, int nmem, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_final, dim3((unsigned)nmem), dim3(kBlock), 0, s, partials, nb, out);
 }
 void convert_d2f(const double* in, float* out, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_d2f, dim3(grid_for(n)), dim3(kBlock), 0, s, in, out, n);

@@ -97,11 +97,27 @@ def lib():
         L.smaiso_make_interface_ic.argtypes = [C.POINTER(CParams), C.POINTER(CIC), dp, dp, dp]
         L.smaiso_timing_enable.argtypes = [vp, C.c_int32]
         L.smaiso_timing_get.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
+        L.smaiso_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(vp)]
+        L.smaiso_ensemble_destroy.argtypes = [vp]
+        L.smaiso_ensemble_last_error.argtypes = [vp]
+        L.smaiso_ensemble_last_error.restype = C.c_char_p
+        L.smaiso_ensemble_groups.argtypes = [vp]
+        L.smaiso_ensemble_set_state.argtypes = [vp, C.c_int32, dp, dp]
+        L.smaiso_ensemble_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_ensemble_step.argtypes = [vp, C.c_int32]
+        L.smaiso_ensemble_step_count.argtypes = [vp, C.c_int32]
+        L.smaiso_ensemble_step_count.restype = C.c_int64
+        L.smaiso_ensemble_get_field.argtypes = [vp, C.c_int32, C.c_int32, dp]
+        L.smaiso_ensemble_diagnostics.argtypes = [vp, C.c_int32, dp, dp, dp]
         for name in ("smaiso_create", "smaiso_destroy", "smaiso_set_state", "smaiso_set_state_device",
                      "smaiso_step", "smaiso_step_async", "smaiso_synchronize", "smaiso_get_field",
                      "smaiso_get_field_device", "smaiso_diagnostics", "smaiso_r2r3d",
                      "smaiso_r2r3d_device", "smaiso_make_interface_ic", "smaiso_timing_enable",
-                     "smaiso_timing_get"):
+                     "smaiso_timing_get", "smaiso_set_graph", "smaiso_ensemble_create",
+                     "smaiso_ensemble_destroy", "smaiso_ensemble_groups", "smaiso_ensemble_set_state",
+                     "smaiso_ensemble_set_graph", "smaiso_ensemble_step", "smaiso_ensemble_get_field",
+                     "smaiso_ensemble_diagnostics"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -255,6 +271,10 @@ class Stepper:
             _check(rc, self.h)
         return a.value, b.value, c.value
 
+    def set_graph(self, enable: bool):
+        """Replay pairs of steady steps from one captured hipGraph (smaiso_set_graph)."""
+        _check(lib().smaiso_set_graph(self.h, 1 if enable else 0), self.h)
+
     def timing(self, enable: bool):
         _check(lib().smaiso_timing_enable(self.h, 1 if enable else 0), self.h)
 
@@ -262,6 +282,81 @@ class Stepper:
         ms, n, by = C.c_double(), C.c_int64(), C.c_double()
         _check(lib().smaiso_timing_get(self.h, KERNEL_CLASSES[kc], C.byref(ms), C.byref(n), C.byref(by)), self.h)
         return ms.value, n.value, by.value
+
+
+def _echeck(rc: int, e=None):
+    if rc != OK:
+        raise SmaisoError(rc, lib().smaiso_ensemble_last_error(e).decode())
+
+
+class Ensemble:
+    """Independent simulations stepped together (smaiso_ensemble_*).
+
+    Members with the same grid and shared scalars run as one batch; nu,
+    lambda, zeta and rho0 may differ per member.
+    """
+
+    def __init__(self, members):
+        self.params = list(members)
+        arr = (CParams * len(self.params))(*[p.to_c() for p in self.params])
+        e = C.c_void_p()
+        _echeck(lib().smaiso_ensemble_create(arr, len(self.params), C.byref(e)))
+        self.e = e
+
+    def close(self):
+        if getattr(self, "e", None):
+            lib().smaiso_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.params)
+
+    @property
+    def groups(self) -> int:
+        return int(lib().smaiso_ensemble_groups(self.e))
+
+    def _shape(self, member: int):
+        # out-of-range members are reported by the library itself
+        return self.params[member].shape if 0 <= member < len(self.params) else (-1,)
+
+    def set_state(self, member: int, psi: np.ndarray, rho: np.ndarray):
+        shape = self._shape(member)
+        psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(shape)
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(shape)
+        _echeck(lib().smaiso_ensemble_set_state(self.e, int(member), _dp(psi), _dp(rho)), self.e)
+
+    def set_graph(self, enable: bool):
+        _echeck(lib().smaiso_ensemble_set_graph(self.e, 1 if enable else 0), self.e)
+
+    def step(self, n: int = 1):
+        _echeck(lib().smaiso_ensemble_step(self.e, int(n)), self.e)
+
+    def nloop(self, member: int = 0) -> int:
+        return int(lib().smaiso_ensemble_step_count(self.e, int(member)))
+
+    def field(self, member: int, name: str) -> np.ndarray:
+        out = np.empty(self._shape(member) if 0 <= member < len(self.params) else 1, dtype=np.float64)
+        _echeck(lib().smaiso_ensemble_get_field(self.e, int(member), FIELDS[name], _dp(out)), self.e)
+        return out
+
+    def diagnostics(self, member: int, allow_nonfinite: bool = False):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        rc = lib().smaiso_ensemble_diagnostics(self.e, int(member), C.byref(a), C.byref(b), C.byref(c))
+        if not (allow_nonfinite and rc == E_NONFINITE):
+            _echeck(rc, self.e)
+        return a.value, b.value, c.value
 
 
 def r2r3d(x: np.ndarray, kinds, dtype: int = FP64) -> np.ndarray:

@@ -90,3 +90,20 @@ def test_splitmix64_known_answers():
     psi, _ = S.make_interface_ic(S.Params(nx=2, ny=2, nz=2), S.InterfaceIC(amplitude=0.0, noise=1.0, seed=0))
     u8 = O.splitmix64_uniform(0, 8)
     assert np.array_equal(psi.ravel(), 2.0 * u8 - 1.0)
+
+
+def test_ensemble_create_rejects_bad_members():
+    good = S.Params(nx=8, ny=8, nz=8)
+    bad = S.Params(nx=8, ny=8, nz=8, nu=-1.0)
+    with pytest.raises(S.SmaisoError) as e:
+        S.Ensemble([good, bad])
+    assert e.value.code == S.E_INVALID
+    assert "member 1" in str(e.value) and "nu" in str(e.value)
+    with pytest.raises(S.SmaisoError) as e:
+        S.Ensemble([])
+    assert e.value.code == S.E_INVALID
+    L = S.lib()
+    assert L.smaiso_ensemble_step_count(None, 0) == -1
+    assert L.smaiso_ensemble_groups(None) == -1
+    assert L.smaiso_set_graph(None, 1) == S.E_INVALID
+    assert L.smaiso_ensemble_step(None, 1) == S.E_INVALID
