@@ -53,8 +53,9 @@ def make_params(S, n, precision):
                     precision=precision)
 
 
-def time_stepper(S, params, psi, rho, steps, warmup, cadence=50):
+def time_stepper(S, params, psi, rho, steps, warmup, cadence=50, graph=True):
     st = S.Stepper(params)
+    st.set_graph(graph)
     st.set_state(psi, rho)
     st.step(warmup)
     S.device_synchronize()
@@ -113,6 +114,57 @@ def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000)):
     return out
 
 
+ENSEMBLE_SHAPES = ((64, 64, 64), (96, 96, 96), (128, 128, 64))
+
+
+def ensemble_members(S, count, seed=4):
+    """Batched-ensemble workload: `count` simulations split evenly over 64^3,
+    96^3 and 128x128x64 grids, FP32-mixed, dt = 0.01, with seeded draws of the
+    density ratio (rho_iso = rho0 in [0.7, 0.95]), viscosity nu in [0.1, 2],
+    coupling zeta in [0.5, 2] and interface undulation in [0, 1]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    members, ics = [], []
+    for m in range(count):
+        shape = ENSEMBLE_SHAPES[m * len(ENSEMBLE_SHAPES) // count]
+        ratio = float(rng.uniform(0.7, 0.95))
+        p = S.Params(nx=shape[0], ny=shape[1], nz=shape[2], dt=0.01, rho0=ratio, rho_m=1.0,
+                     divv_switch=0, nu=float(rng.uniform(0.1, 2.0)), zeta=float(rng.uniform(0.5, 2.0)),
+                     precision=S.FP32_MIXED)
+        ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=ratio, rho_sm=1.0,
+                           undulation=float(rng.uniform(0.0, 1.0)), noise=1e-3,
+                           seed=int(rng.integers(1 << 62)))
+        members.append(p)
+        ics.append(ic)
+    return members, ics
+
+
+def time_ensemble(S, count, steps, warmup=4):
+    members, ics = ensemble_members(S, count)
+    points = sum(p.nx * p.ny * p.nz for p in members)
+    en = S.Ensemble(members)
+    for m, (p, ic) in enumerate(zip(members, ics)):
+        psi, rho = S.make_interface_ic(p, ic)
+        en.set_state(m, psi, rho)
+    en.step(warmup)
+    S.device_synchronize()
+    t0 = time.perf_counter()
+    en.step(steps)
+    S.device_synchronize()
+    sec = time.perf_counter() - t0
+    finite = all(math.isfinite(en.diagnostics(m, allow_nonfinite=True)[0]) for m in range(count))
+    eps = steps / sec
+    gbs = BYTES_PER_POINT["mixed"] * points * eps / 1e9
+    res = dict(members=count, groups=en.groups, points=points, steps=steps,
+               ensemble_steps_per_s=eps, member_steps_per_s=eps * count,
+               point_steps_per_s=eps * points, algorithmic_gbs=gbs, hbm_frac=gbs / HBM_PEAK_GBS,
+               graph=True, finite=finite,
+               workload="16 x 64^3 + 16 x 96^3 + 16 x 128x128x64 (per 48 members), FP32-mixed, "
+                        "dt=0.01, seeded rho0/nu/zeta/undulation draws")
+    en.close()
+    return res
+
+
 def cpu_baseline(n, psi, rho, seconds_budget=20.0):
     """The FP64 oracle (numpy + scipy.fft) on the host on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -148,6 +200,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--accuracy-steps", type=str, default="100,500,2000",
                     help="checkpoints of the FP32-mixed vs FP64 comparison ('' to skip)")
+    ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly")
+    ap.add_argument("--ensemble", type=int, default=48, help="members of the ensemble leg (0 to skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -165,7 +220,7 @@ def main():
     psi, rho = S.make_interface_ic(p64, ic)
 
     log(f"[bench] FP64: warmup {args.warmup}, timed {args.steps}")
-    st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup)
+    st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup, graph=not args.no_graph)
     sps64 = args.steps / sec64
     log(f"[bench] FP64 {sps64:.2f} steps/s, diag {diag64}")
     kt64 = kernel_timing(st64, args.profile_steps)
@@ -175,7 +230,7 @@ def main():
     if not args.no_mixed:
         pm = make_params(S, n, S.FP32_MIXED)
         log(f"[bench] FP32-mixed: warmup {args.warmup}, timed {args.steps}")
-        stm, secm, diagm = time_stepper(S, pm, psi, rho, args.steps, args.warmup)
+        stm, secm, diagm = time_stepper(S, pm, psi, rho, args.steps, args.warmup, graph=not args.no_graph)
         spsm = args.steps / secm
         ktm = kernel_timing(stm, args.profile_steps)
         del stm
@@ -217,6 +272,13 @@ def main():
                      achieved=step_bytes * sps64 / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=step_bytes * sps64 / (HBM_PEAK_GBS * 1e9))
 
+    ens = None
+    if args.ensemble > 0:
+        log(f"[bench] ensemble: {args.ensemble} members, {args.ensemble_steps} steps")
+        ens = time_ensemble(S, args.ensemble, args.ensemble_steps)
+        log(f"[bench] ensemble {ens['member_steps_per_s']:.1f} member-steps/s, "
+            f"{ens['algorithmic_gbs']:.0f} GB/s")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -238,7 +300,8 @@ def main():
         "data": "synthetic: seeded perturbed-interface initial state built on the host",
         "config": {"workload": f"plain FP64 stepper, {n}^3 grid, interface h=Lz/2+0.5cos(2pi x/Lx)cos(2pi y/Ly), "
                                f"rho 1.0/0.8, dt=0.01, rho_m=1, diagnostics every 50 steps",
-                   "grid": [n, n, n], "precision": "fp64", "dt": 0.01, "rho_m": 1.0},
+                   "grid": [n, n, n], "precision": "fp64", "dt": 0.01, "rho_m": 1.0,
+                   "hipgraph": not args.no_graph},
         "roofline": roofline,
         "step_roofline": step_roof,
         "kernels_fp64": {k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
@@ -246,6 +309,7 @@ def main():
                                  gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
                          for k, v in kt64.items()},
         "fp32_mixed": mixed,
+        "ensemble": ens,
         "cpu_baseline": cpu,
         "diagnostics_fp64": diag64,
     }
