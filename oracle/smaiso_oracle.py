@@ -68,7 +68,11 @@ def r2r_1d(x: np.ndarray, kind: int, axis: int) -> np.ndarray:
 
 
 def r2r3d(x: np.ndarray, kinds) -> np.ndarray:
-    """3-D r2r with one kind per axis, like ``fftw_mpi_plan_r2r_3d`` (S:499-545)."""
+    """3-D r2r with one kind per axis, like ``fftw_mpi_plan_r2r_3d`` (S:499-545).
+    A complex array is transformed as real and imaginary parts (the transforms
+    are real-linear), which is what complex-step differentiation needs."""
+    if np.iscomplexobj(x):
+        return r2r3d(np.real(x), kinds) + 1j * r2r3d(np.imag(x), kinds)
     y = np.asarray(x, dtype=np.float64)
     for ax in (2, 1, 0):
         y = r2r_1d(y, kinds[ax], ax)
@@ -266,7 +270,7 @@ def make_interface_ic(p: Params, ic: InterfaceIC):
     nx, ny, nz = p.shape
     lx, ly, lz = nx * p.dx, ny * p.dy, nz * p.dz
     if ic.height is not None:
-        h = np.asarray(ic.height, dtype=np.float64).reshape(nx, ny)
+        h = np.asarray(ic.height).reshape(nx, ny)  # complex allowed (complex-step)
     else:
         h0 = lz / 2 if ic.h0 is None else ic.h0
         x = np.arange(nx, dtype=np.float64) * p.dx
@@ -317,13 +321,15 @@ class Oracle:
     # The pre-loop Nl of S:996-1008 is overwritten before use (S:2170-2172).
     def set_state(self, psi: np.ndarray, rho: np.ndarray):
         p = self.p
-        psi = np.asarray(psi, dtype=np.float64).reshape(p.shape)
-        rho = np.asarray(rho, dtype=np.float64).reshape(p.shape)
+        # complex states are allowed for complex-step derivatives
+        dt = np.result_type(psi, rho, np.float64)
+        psi = np.asarray(psi, dtype=dt).reshape(p.shape)
+        rho = np.asarray(rho, dtype=dt).reshape(p.shape)
         psiq = r2r3d(psi, CT)
         psiq = p.scale * psiq
         psi = r2r3d(psiq, ICT)
         self.f = Fields(psi=psi, psiq=psiq, psiq_old=psiq.copy(), rho=rho.copy(),
-                        f_rho=np.zeros(p.shape))
+                        f_rho=np.zeros(p.shape, dtype=dt))
         self.nloop = 0
 
     def step(self, nsteps: int = 1):
@@ -405,8 +411,9 @@ def gradients(p: Params, t: Tables, psiq, rho):
     sc = p.scale
     sx, sy, sz = (_b(t.vs[a], a) for a in range(3))
     rhoq = r2r3d(rho, CT)                            # S:1104-1106
-    gpx = np.zeros(p.shape); gpy = np.zeros(p.shape); gpz = np.zeros(p.shape)
-    rdx = np.zeros(p.shape); rdy = np.zeros(p.shape); rdz = np.zeros(p.shape)
+    dt = np.result_type(psiq, rhoq)
+    gpx = np.zeros(p.shape, dt); gpy = np.zeros(p.shape, dt); gpz = np.zeros(p.shape, dt)
+    rdx = np.zeros(p.shape, dt); rdy = np.zeros(p.shape, dt); rdz = np.zeros(p.shape, dt)
     gpx[:-1] = -sx[:-1] * psiq[1:]                   # S:1166-1192
     rdx[:-1] = -sc * sx[:-1] * rhoq[1:]
     gpy[:, :-1] = -sy[:, :-1] * psiq[:, 1:]          # S:1196-1206
@@ -517,11 +524,15 @@ def velocity_solve(p: Params, t: Tables, Sxq, Syq, Szq, fgq):
     nu, lam = p.nu, p.lam
     shape = p.shape
     gfac = -1.57 * 1.57 * 2 * sc
+    dt = np.result_type(Sxq, Syq, Szq, fgq, nu, lam)
+
+    def zeros(sh):
+        return np.zeros(sh, dt)
 
     # x component: i <= Nx-2 (S:1583-1709)
-    Sy_s = np.zeros(shape); Sy_s[:-1, 1:, :] = Syq[1:, :-1, :]
-    Sz_s = np.zeros(shape); Sz_s[:-1, :, 1:] = Szq[1:, :, :-1]
-    fg_s = np.zeros(shape); fg_s[:-1] = fgq[1:]
+    Sy_s = zeros(shape); Sy_s[:-1, 1:, :] = Syq[1:, :-1, :]
+    Sz_s = zeros(shape); Sz_s[:-1, :, 1:] = Szq[1:, :, :-1]
+    fg_s = zeros(shape); fg_s[:-1] = fgq[1:]
     dot = sx * Sxq + qy * Sy_s + qz * Sz_s
     vsolx = (t.CM1x / nu) * (Sxq - t.CM2x * dot)
     virrx = (t.CM1x / (lam + 2 * nu)) * (sx * fg_s + t.CM2x * dot * np.exp(gfac / t.CM1x))
@@ -529,9 +540,9 @@ def velocity_solve(p: Params, t: Tables, Sxq, Syq, Szq, fgq):
     virrx[-1] = 0.0
 
     # y component: j <= Ny-2, k <= Nz-2 (S:1759-1974)
-    Sx_s = np.zeros(shape); Sx_s[1:, :-1, :] = Sxq[:-1, 1:, :]
-    Sz_s = np.zeros(shape); Sz_s[:, :-1, 1:] = Szq[:, 1:, :-1]
-    fg_s = np.zeros(shape); fg_s[:, :-1] = fgq[:, 1:]
+    Sx_s = zeros(shape); Sx_s[1:, :-1, :] = Sxq[:-1, 1:, :]
+    Sz_s = zeros(shape); Sz_s[:, :-1, 1:] = Szq[:, 1:, :-1]
+    fg_s = zeros(shape); fg_s[:, :-1] = fgq[:, 1:]
     dot = qx * Sx_s + sy * Syq + qz * Sz_s
     vsoly = (t.CM1y / nu) * (Syq - t.CM2y * dot)
     virry = (t.CM1y / (lam + 2 * nu)) * (sy * fg_s + t.CM2y * dot * np.exp(gfac / t.CM1y))
@@ -539,9 +550,9 @@ def velocity_solve(p: Params, t: Tables, Sxq, Syq, Szq, fgq):
     virry[:, -1, :] = 0.0; virry[:, :, -1] = 0.0
 
     # z component: j <= Ny-2, k <= Nz-2
-    Sx_s = np.zeros(shape); Sx_s[1:, :, :-1] = Sxq[:-1, :, 1:]
-    Sy_s = np.zeros(shape); Sy_s[:, 1:, :-1] = Syq[:, :-1, 1:]
-    fg_s = np.zeros(shape); fg_s[:, :, :-1] = fgq[:, :, 1:]
+    Sx_s = zeros(shape); Sx_s[1:, :, :-1] = Sxq[:-1, :, 1:]
+    Sy_s = zeros(shape); Sy_s[:, 1:, :-1] = Syq[:, :-1, 1:]
+    fg_s = zeros(shape); fg_s[:, :, :-1] = fgq[:, :, 1:]
     dot = qx * Sx_s + qy * Sy_s + sz * Szq
     vsolz = (t.CM1z / nu) * (Szq - t.CM2z * dot)
     cm_exp = t.CM1z.copy()
