@@ -223,6 +223,33 @@ int smaiso_timing_get(smaiso_handle* h, int32_t kc, double* total_ms, int64_t* l
                       double* bytes);
 
 /*
+ * Adjoint (no reference counterpart): reverse-mode gradients of
+ *   L = sum over the grid of (psi(T) - target)^2
+ * after nsteps steps from smaiso_set_state(psi0, rho0), with respect to psi0,
+ * rho0 (host FP64 arrays of Nx*Ny*Nz), zeta and nu.  The forward trajectory
+ * is stored every checkpoint_every steps and each segment is recomputed from
+ * its checkpoint before its steps are transposed (hand-written adjoints of
+ * S:1081-2198 with transposed DCT/DST passes), so device memory grows with
+ * nsteps/checkpoint_every + checkpoint_every stored states.  The adjoint
+ * fields are FP64; with SMAISO_FP32_MIXED the trajectory it differentiates is
+ * the mixed-precision one.  Needs the steady branch for every step
+ * (divv_switch = 0 or load_mode = 1; SMAISO_E_INVALID otherwise).  dpsi0 and
+ * drho0 may be NULL.  The handle's state is consumed: call smaiso_set_state
+ * before stepping it again.
+ */
+int smaiso_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const double* target,
+                   int32_t nsteps, int32_t checkpoint_every, double* loss, double* dpsi0,
+                   double* drho0, double* dzeta, double* dnu);
+
+/* dL/dh(x, y) of the interface initial condition (smaiso_make_interface_ic)
+ * from dL/dpsi and dL/drho of its output: sum over z of (amplitude cos(q0 z)
+ * dpsi + (rho_sm - rho_iso) drho) * d profile/dh, d profile/dh =
+ * sech^2((z - h)/width)/(2 width).  height may be NULL (the analytic h0 +
+ * undulation map).  dheight: Nx*Ny.  Host only. */
+int smaiso_interface_ic_vjp(const smaiso_params* p, const smaiso_ic* ic, const double* height,
+                            const double* dpsi, const double* drho, double* dheight);
+
+/*
  * Ensembles: `count` independent simulations (each one the reference's whole
  * program, S:1081-2263, with its own parameters and grid) stepped together on
  * one device.  Members that share grid shape, spacing, dt, alpha, beta,

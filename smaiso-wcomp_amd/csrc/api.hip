@@ -7,9 +7,11 @@
 // reference's loop body (smaiso-wcomp.cpp:1081-2198) for all members.
 #include "../../include/smaiso.h"
 #include "common.h"
+#include "adjoint.h"
 #include "r2r.h"
 #include "stepper.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -98,6 +100,11 @@ struct smaiso_handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_parity = -1;
+
+  // adjoint work fields (FP64, allocated by the first smaiso_adjoint call)
+  std::vector<double*> adj;
+  double* adj_partials = nullptr;
+  double* adj_acc = nullptr;
 
   // timing
   bool timing = false;
@@ -241,8 +248,12 @@ struct smaiso_handle {
   // DCT-II (z pass with f's load op), DCT-II along y, then one x pass doing
   // DCT-II, the diagonal operator mid_op and DCT-III, then DCT-III along y, z.
   hipError_t xmid(const Fuse& f, int mid_op, void* out) {
-    const DType dt = mixed ? F32 : F64;
+    return xmid_dt(mixed ? F32 : F64, nullptr, f, mid_op, out);
+  }
+  // as xmid in precision dt; `in` is read by the z pass when f.op is OP_PLAIN
+  hipError_t xmid_dt(DType dt, const void* in, const Fuse& f, int mid_op, void* out) {
     PassIO io;
+    io.in = in;
     io.out = out;
     io.f = f;
     hipError_t e = run_pass(dt, REDFT10, 2, io, g);
@@ -1160,6 +1171,294 @@ int smaiso_make_interface_ic(const smaiso_params* p, const smaiso_ic* ic, const 
         psi[idx] = ic->amplitude * std::cos(p->q0 * z) * prof + ic->noise * (2.0 * u - 1.0);
         rho[idx] = ic->rho_iso + (ic->rho_sm - ic->rho_iso) * prof;
       }
+    }
+  }
+  return SMAISO_OK;
+}
+
+}  // extern "C"
+
+// --- adjoint -------------------------------------------------------------------
+namespace {
+
+// One stored step input: psiq, psiq_old, rho, f_rho (FP64), the previous
+// step's N^l and psi (work precision).
+struct StateStore {
+  std::vector<void*> mem;
+  size_t n3 = 0, tsz = 8;
+  int count = 0;
+  ~StateStore() {
+    for (void* m : mem) (void)hipFree(m);
+  }
+  hipError_t alloc(int n, long long points, size_t es) {
+    count = n;
+    n3 = (size_t)points;
+    tsz = es;
+    for (int k = 0; k < n; ++k) {
+      void* m = nullptr;
+      hipError_t e = hipMalloc(&m, n3 * (32 + 2 * tsz));
+      if (e != hipSuccess) return e;
+      mem.push_back(m);
+    }
+    return hipSuccess;
+  }
+  double* field(int k, int f) const {  // f: 0 psiq, 1 psiq_old, 2 rho, 3 f_rho
+    return static_cast<double*>(mem[k]) + (size_t)f * n3;
+  }
+  void* nlq(int k) const { return static_cast<char*>(mem[k]) + 32 * n3; }
+  void* psi(int k) const { return static_cast<char*>(mem[k]) + (32 + tsz) * n3; }
+};
+
+hipError_t save_state(smaiso_handle* h, const StateStore& st, int k) {
+  const size_t db = st.n3 * 8, tb = st.n3 * st.tsz;
+  hipStream_t s = h->stream;
+  hipError_t e = hipMemcpyAsync(st.field(k, 0), h->psiq, db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.field(k, 1), h->psiq_old, db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.field(k, 2), h->rho, db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.field(k, 3), h->f_rho, db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.nlq(k), h->nlq, tb, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.psi(k), h->psi, tb, hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
+hipError_t restore_state(smaiso_handle* h, const StateStore& st, int k, long long nloop) {
+  const size_t db = st.n3 * 8, tb = st.n3 * st.tsz;
+  hipStream_t s = h->stream;
+  hipError_t e = hipMemcpyAsync(h->psiq, st.field(k, 0), db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->psiq_old, st.field(k, 1), db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->rho, st.field(k, 2), db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->f_rho, st.field(k, 3), db, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->nlq, st.nlq(k), tb, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h->psi, st.psi(k), tb, hipMemcpyDeviceToDevice, s);
+  h->nloop = nloop;
+  return e;
+}
+
+enum AdjField {
+  AF_P, AF_PO, AF_R, AF_FR, AF_NO, AF_NL, AF_PSI, AF_AMP, AF_LAP, AF_LPSI, AF_YBAR, AF_A0BAR,
+  AF_TMP, AF_DIVV, AF_DIVF, AF_WFG, AF_WGFG, AF_TARGET,
+  AF_GP, AF_RD = AF_GP + 3, AF_V = AF_RD + 3, AF_DOT = AF_V + 3, AF_CFG = AF_DOT + 3,
+  AF_DIR = AF_CFG + 3, AF_WFQ = AF_DIR + 3, AF_WSQ = AF_WFQ + 3, AF_COUNT = AF_WSQ + 3
+};
+
+int adj_alloc(smaiso_handle* h) {
+  if (!h->adj.empty()) return SMAISO_OK;
+  const size_t db = (size_t)h->n3() * 8;
+  for (int f = 0; f < AF_COUNT; ++f) {
+    void* m = nullptr;
+    HIPTRY(h, hipMalloc(&m, db));
+    h->allocs.push_back(m);
+    h->adj.push_back(static_cast<double*>(m));
+  }
+  const StepConsts c = h->consts();
+  void* m = nullptr;
+  HIPTRY(h, hipMalloc(&m, sizeof(double) * kAdjSlots * adj_blocks(c)));
+  h->allocs.push_back(m);
+  h->adj_partials = static_cast<double*>(m);
+  HIPTRY(h, hipMalloc(&m, sizeof(double) * 2 * kAdjSlots));
+  h->allocs.push_back(m);
+  h->adj_acc = static_cast<double*>(m);
+  return SMAISO_OK;
+}
+
+template <typename T>
+AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
+  AdjArgs<T> A{};
+  auto tp = [](void* p) { return static_cast<const T*>(p); };
+  A.psi = tp(h->psi_prev);  // the step swapped psi and psi_prev
+  A.amp = tp(h->amp);
+  A.lpsi = tp(h->lpsi);
+  A.laprl = tp(h->laprl);
+  A.divv = tp(h->divv);
+  for (int a = 0; a < 3; ++a) {
+    A.gp[a] = tp(h->gp[a]);
+    A.rd[a] = tp(h->rd[a]);
+    A.v[a] = tp(h->v[a]);
+    A.vsolq[a] = tp(h->vsolq[a]);
+    A.virrq[a] = tp(h->virrq[a]);
+  }
+  A.rho = rho_n;
+  A.rho_new = h->rho;
+  double* const* f = h->adj.data();
+  A.bP = f[AF_P]; A.bPo = f[AF_PO]; A.bR = f[AF_R]; A.bFr = f[AF_FR]; A.bNo = f[AF_NO];
+  A.bnl = f[AF_NL]; A.bpsi = f[AF_PSI]; A.bamp = f[AF_AMP]; A.blap = f[AF_LAP];
+  A.blpsi = f[AF_LPSI]; A.ybar = f[AF_YBAR]; A.a0bar = f[AF_A0BAR]; A.tmp = f[AF_TMP];
+  A.bdivv = f[AF_DIVV]; A.bdivf = f[AF_DIVF]; A.wfg = f[AF_WFG]; A.wGfg = f[AF_WGFG];
+  for (int a = 0; a < 3; ++a) {
+    A.bgp[a] = f[AF_GP + a]; A.brd[a] = f[AF_RD + a]; A.bv[a] = f[AF_V + a];
+    A.dotb[a] = f[AF_DOT + a]; A.cfgb[a] = f[AF_CFG + a]; A.dirb[a] = f[AF_DIR + a];
+    A.wfq[a] = f[AF_WFQ + a]; A.wSq[a] = f[AF_WSQ + a];
+  }
+  A.partials = h->adj_partials;
+  A.first = first ? 1 : 0;
+  return A;
+}
+
+// Transpose of the step that the handle has just recomputed from the stored
+// input state (rho at its start is rho_n): the adjoint of the step's outputs,
+// held in the adjoint state fields, becomes that of its inputs.
+template <typename T>
+int adjoint_step(smaiso_handle* h, const double* rho_n, bool first) {
+  using K = AdjKernels<T>;
+  const StepConsts c = h->consts();
+  hipStream_t s = h->stream;
+  const AdjArgs<T> A = adj_args<T>(h, rho_n, first);
+  const int nb = (int)adj_blocks(c);
+  Fuse fb = h->fuse_base();
+  fb.op = OP_PLAIN;
+  K::update(c, A, s);
+  HIPTRY(h, h->xform(F64, kICT, A.tmp, nullptr, A.bnl));
+  K::nl_density(c, A, s);
+  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xform(F64, kST[a], A.bv[a], nullptr, A.bv[a]));
+  HIPTRY(h, h->xform(F64, kCT, A.bdivv, nullptr, A.bdivv));
+  HIPTRY(h, h->xform(F64, kCT, A.bdivf, nullptr, A.bdivf));
+  K::velocity_a(c, A, s);
+  K::velocity_b(c, A, s);
+  for (int a = 0; a < 3; ++a) {
+    HIPTRY(h, h->xform(F64, kIST[a], A.wSq[a], nullptr, A.wSq[a]));
+    HIPTRY(h, h->xform(F64, kIST[a], A.wfq[a], nullptr, A.wfq[a]));
+  }
+  HIPTRY(h, h->xform(F64, kICT, A.wfg, nullptr, A.wfg));
+  HIPTRY(h, h->xform(F64, kICT, A.wGfg, nullptr, A.wGfg));
+  K::forces(c, A, s);
+  HIPTRY(h, h->xmid_dt(F64, A.blap, fb, OP_MID_NEGLAP, A.ybar));
+  HIPTRY(h, h->xmid_dt(F64, A.bamp, fb, OP_MID_AMP, A.a0bar));
+  K::amp_lap(c, A, s);
+  HIPTRY(h, h->xform(F64, kCT, A.bpsi, nullptr, A.bpsi));
+  HIPTRY(h, h->xform(F64, kCT, A.blpsi, nullptr, A.blpsi));
+  for (int a = 0; a < 3; ++a) {
+    HIPTRY(h, h->xform(F64, kST[a], A.bgp[a], nullptr, A.bgp[a]));
+    HIPTRY(h, h->xform(F64, kST[a], A.brd[a], nullptr, A.brd[a]));
+  }
+  K::grad(c, A, s);
+  HIPTRY(h, h->xform(F64, kICT, A.tmp, nullptr, A.tmp));
+  adj_add(c, A.bR, A.tmp, s);
+  adj_reduce(h->adj_partials, 0, nb, h->adj_acc, 0, s);  // dzeta
+  adj_reduce(h->adj_partials, 2, nb, h->adj_acc, 0, s);
+  adj_reduce(h->adj_partials, 1, nb, h->adj_acc, 1, s);  // dnu
+  HIPTRY(h, hipGetLastError());
+  return SMAISO_OK;
+}
+
+int adjoint_step_any(smaiso_handle* h, const double* rho_n, bool first) {
+  return h->mixed ? adjoint_step<float>(h, rho_n, first) : adjoint_step<double>(h, rho_n, first);
+}
+
+int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const double* target,
+                int nsteps, int every, double* loss, double* dpsi0, double* drho0, double* dzeta,
+                double* dnu) {
+  if (h->nmem != 1) return fail(h, SMAISO_E_INVALID, "the adjoint runs on single handles");
+  if (!(h->p.divv_switch == 0 || h->p.load_mode == 1))
+    return fail(h, SMAISO_E_INVALID,
+                "the adjoint covers the steady branch only: set divv_switch = 0 or load_mode = 1");
+  if (nsteps < 0 || every < 1) return fail(h, SMAISO_E_INVALID, "need nsteps >= 0 and checkpoint_every >= 1");
+  HIPTRY(h, hipSetDevice(h->device));
+  int rc = adj_alloc(h);
+  if (rc != SMAISO_OK) return rc;
+  hipStream_t s = h->stream;
+  const StepConsts c = h->consts();
+  const long long n3 = h->n3();
+  const size_t db = (size_t)n3 * 8;
+  const int nb = (int)adj_blocks(c);
+  double* const* F = h->adj.data();
+
+  rc = do_set_state(h, 0, psi0, rho0, hipMemcpyHostToDevice);
+  if (rc != SMAISO_OK) return rc;
+  HIPTRY(h, hipMemcpyAsync(F[AF_TARGET], target, db, hipMemcpyHostToDevice, s));
+  HIPTRY(h, hipMemsetAsync(h->adj_acc, 0, sizeof(double) * 2 * kAdjSlots, s));
+
+  // forward sweep with a checkpoint every `every` steps
+  const int ncp = nsteps > 0 ? (nsteps + every - 1) / every : 0;
+  StateStore cps, seg;
+  HIPTRY(h, cps.alloc(ncp, n3, h->tsz));
+  HIPTRY(h, seg.alloc(std::min(every, std::max(nsteps, 1)), n3, h->tsz));
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % every == 0) HIPTRY(h, save_state(h, cps, n / every));
+    rc = run_step_any(h);
+    if (rc != SMAISO_OK) return rc;
+  }
+  // loss = sum (psi_T - target)^2; psiq_T bar = ICT^T(2 (psi_T - target))
+  if (h->mixed)
+    AdjKernels<float>::loss(c, static_cast<const float*>(h->psi), F[AF_TARGET], F[AF_TMP], h->adj_partials, s);
+  else
+    AdjKernels<double>::loss(c, static_cast<const double*>(h->psi), F[AF_TARGET], F[AF_TMP], h->adj_partials, s);
+  adj_reduce(h->adj_partials, 3, nb, h->adj_acc, 3, s);
+  HIPTRY(h, h->xform(F64, kCT, F[AF_TMP], nullptr, F[AF_TMP]));
+  adj_div_wc(c, F[AF_TMP], F[AF_P], s);
+  for (int f : {AF_PO, AF_R, AF_FR, AF_NO}) fill_zero_d(F[f], n3, s);
+
+  // reverse sweep: recompute each segment from its checkpoint, then
+  // transpose its steps last to first
+  for (int sg = ncp - 1; sg >= 0; --sg) {
+    const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
+    HIPTRY(h, restore_state(h, cps, sg, n0));
+    for (int n = n0; n < n1; ++n) {
+      HIPTRY(h, save_state(h, seg, n - n0));
+      if (n + 1 < n1) {
+        rc = run_step_any(h);
+        if (rc != SMAISO_OK) return rc;
+      }
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      HIPTRY(h, restore_state(h, seg, n - n0, n));
+      rc = run_step_any(h);
+      if (rc != SMAISO_OK) return rc;
+      rc = adjoint_step_any(h, seg.field(n - n0, 2), n == 0);
+      if (rc != SMAISO_OK) return rc;
+    }
+  }
+  // set_state: psiq = scale CT(psi0), psiq_old = psiq, rho = rho0
+  adj_psi0(c, F[AF_P], F[AF_PO], F[AF_TMP], s);
+  HIPTRY(h, h->xform(F64, kICT, F[AF_TMP], nullptr, F[AF_TMP]));
+  double acc[2 * kAdjSlots];
+  HIPTRY(h, hipMemcpyAsync(acc, h->adj_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+  if (dpsi0) HIPTRY(h, hipMemcpyAsync(dpsi0, F[AF_TMP], db, hipMemcpyDeviceToHost, s));
+  if (drho0) HIPTRY(h, hipMemcpyAsync(drho0, F[AF_R], db, hipMemcpyDeviceToHost, s));
+  HIPTRY(h, hipStreamSynchronize(s));
+  h->collect_timing();
+  // the stepper holds an intermediate state now
+  h->member_set[0] = 0;
+  h->diag_epoch = -1;
+  if (loss) *loss = acc[6] + acc[7];
+  if (dzeta) *dzeta = acc[0] + acc[1];
+  if (dnu) *dnu = acc[2] + acc[3];
+  return SMAISO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smaiso_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const double* target,
+                   int32_t nsteps, int32_t checkpoint_every, double* loss, double* dpsi0,
+                   double* drho0, double* dzeta, double* dnu) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (!psi0 || !rho0 || !target) return fail(h, SMAISO_E_INVALID, "psi0/rho0/target is NULL");
+  return run_adjoint(h, psi0, rho0, target, nsteps, checkpoint_every, loss, dpsi0, drho0, dzeta, dnu);
+}
+
+int smaiso_interface_ic_vjp(const smaiso_params* p, const smaiso_ic* ic, const double* height,
+                            const double* dpsi, const double* drho, double* dheight) {
+  if (!p || !ic || !dpsi || !drho || !dheight) return fail(nullptr, SMAISO_E_INVALID, "NULL argument");
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(nullptr, SMAISO_E_INVALID, "bad grid");
+  if (!(ic->width > 0)) return fail(nullptr, SMAISO_E_INVALID, "width must be > 0");
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double lx = nx * p->dx, ly = ny * p->dy, lz = nz * p->dz;
+  const double h0 = ic->h0 < 0 ? lz / 2 : ic->h0;
+  for (int i = 0; i < nx; ++i) {
+    for (int j = 0; j < ny; ++j) {
+      const double hh = height ? height[(size_t)i * ny + j]
+                               : h0 + ic->undulation * std::cos(2 * M_PI * i * p->dx / lx) *
+                                          std::cos(2 * M_PI * j * p->dy / ly);
+      double acc = 0.0;
+      for (int k = 0; k < nz; ++k) {
+        const double z = k * p->dz;
+        const double ch = std::cosh((z - hh) / ic->width);
+        const double dprof = 0.5 / (ic->width * ch * ch);
+        const size_t idx = ((size_t)i * ny + j) * nz + k;
+        acc += (ic->amplitude * std::cos(p->q0 * z) * dpsi[idx] + (ic->rho_sm - ic->rho_iso) * drho[idx]) * dprof;
+      }
+      dheight[(size_t)i * ny + j] = acc;
     }
   }
   return SMAISO_OK;

@@ -98,6 +98,8 @@ def lib():
         L.smaiso_timing_enable.argtypes = [vp, C.c_int32]
         L.smaiso_timing_get.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
         L.smaiso_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_adjoint.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp, dp, dp, dp]
+        L.smaiso_interface_ic_vjp.argtypes = [C.POINTER(CParams), C.POINTER(CIC), dp, dp, dp, dp]
         L.smaiso_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(vp)]
         L.smaiso_ensemble_destroy.argtypes = [vp]
         L.smaiso_ensemble_last_error.argtypes = [vp]
@@ -117,7 +119,7 @@ def lib():
                      "smaiso_timing_get", "smaiso_set_graph", "smaiso_ensemble_create",
                      "smaiso_ensemble_destroy", "smaiso_ensemble_groups", "smaiso_ensemble_set_state",
                      "smaiso_ensemble_set_graph", "smaiso_ensemble_step", "smaiso_ensemble_get_field",
-                     "smaiso_ensemble_diagnostics"):
+                     "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_interface_ic_vjp"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -275,6 +277,21 @@ class Stepper:
         """Replay pairs of steady steps from one captured hipGraph (smaiso_set_graph)."""
         _check(lib().smaiso_set_graph(self.h, 1 if enable else 0), self.h)
 
+    def adjoint(self, psi0: np.ndarray, rho0: np.ndarray, target: np.ndarray, nsteps: int,
+                checkpoint_every: int = 50) -> "AdjointResult":
+        """Loss sum (psi(T) - target)^2 after nsteps from set_state(psi0, rho0)
+        and its gradients (smaiso_adjoint).  Consumes the state."""
+        sh = self.params.shape
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64).reshape(sh)
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64).reshape(sh)
+        target = np.ascontiguousarray(target, dtype=np.float64).reshape(sh)
+        dpsi, drho = np.empty(sh), np.empty(sh)
+        loss, dz, dn = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().smaiso_adjoint(self.h, _dp(psi0), _dp(rho0), _dp(target), int(nsteps),
+                                    int(checkpoint_every), C.byref(loss), _dp(dpsi), _dp(drho),
+                                    C.byref(dz), C.byref(dn)), self.h)
+        return AdjointResult(loss.value, dpsi, drho, dz.value, dn.value)
+
     def timing(self, enable: bool):
         _check(lib().smaiso_timing_enable(self.h, 1 if enable else 0), self.h)
 
@@ -282,6 +299,15 @@ class Stepper:
         ms, n, by = C.c_double(), C.c_int64(), C.c_double()
         _check(lib().smaiso_timing_get(self.h, KERNEL_CLASSES[kc], C.byref(ms), C.byref(n), C.byref(by)), self.h)
         return ms.value, n.value, by.value
+
+
+@dataclass
+class AdjointResult:
+    loss: float
+    psi0: np.ndarray   # dL/dpsi0
+    rho0: np.ndarray   # dL/drho0
+    zeta: float        # dL/dzeta
+    nu: float          # dL/dnu
 
 
 def _echeck(rc: int, e=None):
@@ -404,6 +430,21 @@ def make_interface_ic(params: Params, ic: InterfaceIC, height: np.ndarray | None
         hp = _dp(height)
     _check(lib().smaiso_make_interface_ic(C.byref(cp), C.byref(ci), hp, _dp(psi), _dp(rho)))
     return psi, rho
+
+
+def interface_ic_vjp(params: Params, ic: InterfaceIC, dpsi: np.ndarray, drho: np.ndarray,
+                     height: np.ndarray | None = None) -> np.ndarray:
+    """dL/dh(x, y) through the interface IC (smaiso_interface_ic_vjp, host code)."""
+    cp, ci = params.to_c(), ic.to_c()
+    dpsi = np.ascontiguousarray(dpsi, dtype=np.float64).reshape(params.shape)
+    drho = np.ascontiguousarray(drho, dtype=np.float64).reshape(params.shape)
+    hp = None
+    if height is not None:
+        height = np.ascontiguousarray(height, dtype=np.float64).reshape(params.nx, params.ny)
+        hp = _dp(height)
+    out = np.empty((params.nx, params.ny))
+    _check(lib().smaiso_interface_ic_vjp(C.byref(cp), C.byref(ci), hp, _dp(dpsi), _dp(drho), _dp(out)))
+    return out
 
 
 def create_error(params: Params) -> tuple[int, str]:

@@ -107,3 +107,23 @@ def test_ensemble_create_rejects_bad_members():
     assert L.smaiso_ensemble_groups(None) == -1
     assert L.smaiso_set_graph(None, 1) == S.E_INVALID
     assert L.smaiso_ensemble_step(None, 1) == S.E_INVALID
+
+
+@pytest.mark.parametrize("explicit", [False, True])
+def test_interface_ic_vjp_matches_oracle(explicit):
+    import smaiso_adjoint as A
+    p = S.Params(nx=6, ny=5, nz=12)
+    op = O.Params(nx=6, ny=5, nz=12)
+    rng = np.random.default_rng(3)
+    dpsi, drho = rng.standard_normal(p.shape), rng.standard_normal(p.shape)
+    ic = S.InterfaceIC(amplitude=0.8, width=1.5, rho_iso=0.85, rho_sm=1.0, undulation=0.7, noise=0.0)
+    x = np.arange(6) * op.dx
+    y = np.arange(5) * op.dy
+    h = op.nz * op.dz / 2 + 0.7 * np.cos(2 * np.pi * x / (6 * op.dx))[:, None] * np.cos(
+        2 * np.pi * y / (5 * op.dy))[None, :]
+    if explicit:
+        h = h + 0.3 * rng.standard_normal(h.shape)
+    got = S.interface_ic_vjp(p, ic, dpsi, drho, height=h if explicit else None)
+    oic = O.InterfaceIC(amplitude=0.8, width=1.5, rho_iso=0.85, rho_sm=1.0, noise=0.0, height=h)
+    want = A.interface_ic_vjp(op, oic, h, dpsi, drho)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
