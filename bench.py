@@ -165,6 +165,43 @@ def time_ensemble(S, count, steps, warmup=4):
     return res
 
 
+def time_adjoint(S, n=128, nsteps=500, every=50):
+    """Adjoint workload: 128^3, loss ||psi(T) - target||^2 after 500 steps
+    (target from a perturbed interface height), checkpoint every 50; the whole
+    smaiso_adjoint call (forward sweep, segment recompute, transposed steps)
+    is timed, and a plain forward run of the same steps beside it."""
+    import dataclasses
+    import numpy as np
+    out = {}
+    for name, prec in (("fp64", S.FP64), ("fp32_mixed", S.FP32_MIXED)):
+        p = S.Params(nx=n, ny=n, nz=n, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0, nu=0.6, zeta=1.4,
+                     precision=prec)
+        x = np.arange(n) * p.dx
+        h = n * p.dz / 2 + 0.5 * np.cos(2 * np.pi * x / (n * p.dx))[:, None] * np.cos(2 * np.pi * x / (n * p.dy))[None, :]
+        ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=0.8, rho_sm=1.0, noise=1e-3, seed=4)
+        psi0, rho0 = S.make_interface_ic(p, ic, height=h)
+        target = S.make_interface_ic(p, dataclasses.replace(ic, seed=11), height=h + 0.6)[0]
+        st = S.Stepper(p)
+        st.adjoint(psi0, rho0, target, 4, 2)  # warm-up (allocations, tables)
+        S.device_synchronize()
+        t0 = time.perf_counter()
+        g = st.adjoint(psi0, rho0, target, nsteps, every)
+        S.device_synchronize()
+        sec = time.perf_counter() - t0
+        st.set_state(psi0, rho0)
+        S.device_synchronize()
+        t0 = time.perf_counter()
+        st.step(nsteps)
+        S.device_synchronize()
+        fsec = time.perf_counter() - t0
+        st.close()
+        out[name] = dict(seconds=sec, adjoint_steps_per_s=nsteps / sec, forward_steps_per_s=nsteps / fsec,
+                         adjoint_over_forward=sec / fsec, loss=g.loss, dzeta=g.zeta, dnu=g.nu)
+        log(f"[bench] adjoint {name}: {nsteps} steps in {sec:.3f} s ({sec / fsec:.2f}x the forward run)")
+    out["workload"] = f"{n}^3, {nsteps} steps, checkpoint every {every}, loss ||psi(T)-target||^2"
+    return out
+
+
 def cpu_baseline(n, psi, rho, seconds_budget=20.0):
     """The FP64 oracle (numpy + scipy.fft) on the host on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -203,6 +240,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly")
     ap.add_argument("--ensemble", type=int, default=48, help="members of the ensemble leg (0 to skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--adjoint-steps", type=int, default=500, help="0 to skip the adjoint leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -279,6 +317,10 @@ def main():
         log(f"[bench] ensemble {ens['member_steps_per_s']:.1f} member-steps/s, "
             f"{ens['algorithmic_gbs']:.0f} GB/s")
 
+    adj = None
+    if args.adjoint_steps > 0:
+        adj = time_adjoint(S, nsteps=args.adjoint_steps)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -310,6 +352,7 @@ def main():
                          for k, v in kt64.items()},
         "fp32_mixed": mixed,
         "ensemble": ens,
+        "adjoint": adj,
         "cpu_baseline": cpu,
         "diagnostics_fp64": diag64,
     }
