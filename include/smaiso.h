@@ -250,6 +250,28 @@ int smaiso_interface_ic_vjp(const smaiso_params* p, const smaiso_ic* ic, const d
                             const double* dpsi, const double* drho, double* dheight);
 
 /*
+ * Autotuner (no reference counterpart).  For each axis whose length runs on
+ * the LDS FFT path, times a forward + inverse plain pass of the handle's
+ * batch geometry with each of SMAISO_TILE_VARIANTS tile shapes (lines per
+ * tile, threads per workgroup; variant 0 is the built-in default) and keeps
+ * the fastest for that handle's later plain passes.  FP64 is tuned always
+ * (the density transforms), FP32 for SMAISO_FP32_MIXED handles.  ms
+ * (optional): 2 x 3 x SMAISO_TILE_VARIANTS milliseconds [dtype FP64, FP32]
+ * [axis x, y, z][variant], -1 where not timed; chosen (optional): 2 x 3
+ * variants.  Results change only by rounding (the FFT radix plan depends on
+ * the tile).  Allocates scratch and synchronises; a captured step graph is
+ * recaptured afterwards.
+ */
+#define SMAISO_TILE_VARIANTS 4
+int smaiso_autotune(smaiso_handle* h, float* ms, int32_t* chosen);
+
+/* smaiso_r2r3d_device with tile variant `variant` forced on every axis
+ * (SMAISO_E_INVALID if the variant is not built for a length); synchronous.
+ * For validating the autotuner's variants. */
+int smaiso_r2r3d_tiles(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz, const void* in_dev,
+                       void* out_dev, int32_t dtype, int32_t variant, void* stream);
+
+/*
  * Ensembles: `count` independent simulations (each one the reference's whole
  * program, S:1081-2263, with its own parameters and grid) stepped together on
  * one device.  Members that share grid shape, spacing, dt, alpha, beta,
@@ -284,6 +306,8 @@ int64_t smaiso_ensemble_step_count(const smaiso_ensemble* e, int32_t member);
 int smaiso_ensemble_get_field(smaiso_ensemble* e, int32_t member, int32_t field, double* out);
 int smaiso_ensemble_diagnostics(smaiso_ensemble* e, int32_t member, double* l1, double* mass,
                                 double* energy);
+/* smaiso_autotune for every batch of the ensemble. */
+int smaiso_ensemble_autotune(smaiso_ensemble* e);
 
 #ifdef __cplusplus
 }

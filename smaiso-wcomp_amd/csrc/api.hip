@@ -1176,6 +1176,78 @@ int smaiso_make_interface_ic(const smaiso_params* p, const smaiso_ic* ic, const 
   return SMAISO_OK;
 }
 
+// --- autotuner -----------------------------------------------------------------
+namespace {
+void drop_graph(hipGraph_t* graph, hipGraphExec_t* gexec, int* parity) {
+  if (*gexec) (void)hipGraphExecDestroy(*gexec);
+  if (*graph) (void)hipGraphDestroy(*graph);
+  *gexec = nullptr;
+  *graph = nullptr;
+  *parity = -1;
+}
+
+// tune the plain passes of the handle's batch geometry in FP64 (the rho
+// transforms, and every transform of an FP64 handle) and, for FP32-mixed,
+// in FP32
+int tune_handle(smaiso_handle* h, float* ms, int32_t* chosen) {
+  HIPTRY(h, hipSetDevice(h->device));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  const DType dts[2] = {F64, F32};
+  for (int d = 0; d < 2; ++d) {
+    float* m = ms ? ms + d * 3 * R2R::kTileVariants : nullptr;
+    int c[3] = {0, 0, 0};
+    if (d == 1 && !h->mixed) {
+      if (m) for (int k = 0; k < 3 * R2R::kTileVariants; ++k) m[k] = -1.0f;
+    } else {
+      HIPTRY(h, h->r2r.autotune(dts[d], h->g, h->stream, c, m));
+    }
+    if (chosen) for (int a = 0; a < 3; ++a) chosen[d * 3 + a] = c[a];
+  }
+  drop_graph(&h->graph, &h->gexec, &h->graph_parity);
+  return SMAISO_OK;
+}
+}  // namespace
+
+int smaiso_autotune(smaiso_handle* h, float* ms, int32_t* chosen) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  return tune_handle(h, ms, chosen);
+}
+
+int smaiso_ensemble_autotune(smaiso_ensemble* e) {
+  if (!e) return efail(nullptr, SMAISO_E_INVALID, "ensemble is NULL");
+  for (auto& g : e->groups) {
+    int rc = tune_handle(g.get(), nullptr, nullptr);
+    if (rc != SMAISO_OK) return efail(e, rc, g->err);
+  }
+  drop_graph(&e->graph, &e->gexec, &e->graph_parity);
+  return SMAISO_OK;
+}
+
+int smaiso_r2r3d_tiles(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz, const void* in_dev,
+                       void* out_dev, int32_t dtype, int32_t variant, void* stream) {
+  int rc = check_r2r_args(kinds, nx, ny, nz, dtype);
+  if (rc != SMAISO_OK) return rc;
+  if (!in_dev || !out_dev) return fail(nullptr, SMAISO_E_INVALID, "in/out is NULL");
+  if (variant < 0 || variant >= R2R::kTileVariants) return fail(nullptr, SMAISO_E_INVALID, "bad tile variant");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(nullptr, SMAISO_E_HIP, hipGetErrorString(e));
+  // a private transform object so that the shared one keeps its defaults
+  R2R r;
+  const Geom g{nx, ny, nz};
+  const DType dt = dtype == SMAISO_FP64 ? F64 : F32;
+  for (int a = 0; a < 3; ++a) r.set_variant(dt, a, g, variant);
+  PassIO io;
+  io.in = in_dev;
+  io.out = out_dev;
+  const int k[3] = {kinds[0], kinds[1], kinds[2]};
+  e = r.transform3d(dt, k, io, g, static_cast<hipStream_t>(stream));
+  if (e == hipErrorNotSupported) return fail(nullptr, SMAISO_E_INVALID, "tile variant not built for this length");
+  if (e == hipSuccess) e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(nullptr, SMAISO_E_HIP, fmt("r2r: %s", hipGetErrorString(e)));
+  return SMAISO_OK;
+}
+
 }  // extern "C"
 
 // --- adjoint -------------------------------------------------------------------

@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -78,7 +79,25 @@ class R2R {
     return tables(dt, n, &t);
   }
 
+  // Autotuner: for each axis of geometry g whose length runs on the FFT
+  // path, time a forward+inverse plain pass with every tile variant
+  // (kTileVariants, variant 0 = the built-in default) on scratch buffers and
+  // keep the fastest; later plain passes of (dt, axis, g) use it.  ms
+  // (optional, 3 x kTileVariants) receives the timings, -1 where a variant
+  // is not built for the length; chosen (optional, 3) the variants kept.
+  // Not capturable (allocates and synchronises).
+  hipError_t autotune(DType dt, const Geom& g, hipStream_t s, int* chosen, float* ms);
+  // Use tile variant v for plain passes of (dt, axis, g) (0 = default).
+  void set_variant(DType dt, int axis, const Geom& g, int v);
+  int variant(DType dt, int axis, const Geom& g) const;
+  static constexpr int kTileVariants = 4;
+
  private:
+  using TuneKey = std::tuple<int, int, int, int, int, int>;  // dtype, axis, nx, ny, nz, nb
+  static TuneKey tune_key(DType dt, int axis, const Geom& g) {
+    return TuneKey((int)dt, axis, g.nx, g.ny, g.nz, g.nb);
+  }
+  std::map<TuneKey, int> tuned_;
   struct Tables {
     void* tw = nullptr;      // n complex: exp(-2 pi i m / n)
     void* mk = nullptr;      // n complex: exp(-i pi m / (2n))

@@ -1,0 +1,895 @@
+// Device code of the r2r axis passes (included by r2r.hip and by the
+// translation units of the autotuner's tile variants).
+//
+// LDS-staged batched r2r transforms for gfx950.
+//
+// Method (per axis pass):
+//  * A workgroup owns a tile of TL = 2P lines.  For the contiguous z axis the
+//    tile is TL consecutive lines; for the strided y/x axes it is TL
+//    consecutive k columns at one outer index, so every global access is a
+//    run of TL*sizeof(T) bytes.
+//  * Two real lines form one complex sequence z = a + i b (two real FFTs for
+//    the price of one).
+//  * DCT-II (REDFT10) uses Makhoul's reordering (J. Makhoul, "A fast cosine
+//    transform in one and two dimensions", IEEE TASSP 28(1), 1980):
+//    v = (x0, x2, x4, ..., x5, x3, x1), Y_k = 2 Re(e^{-i pi k/2N} V_k) with
+//    V = DFT_N(v).  DCT-III (REDFT01) is its transpose:
+//    U_k = e^{+i pi k/2N}(Y_k - i Y_{N-k}), u = IDFT_N(U) (unnormalised),
+//    y_{2n} = u_n, y_{2n+1} = u_{N-1-n}.
+//  * DST-II (RODFT10)  = reverse(DCT-II((-1)^n x_n)) and
+//    DST-III (RODFT01) = (-1)^k DCT-III(reverse(x))_k, folded into the load
+//    and store index maps.
+//  * The complex DFT_N is a self-sorting Stockham FFT in LDS with radix 2, 3,
+//    4, 8 and 16 butterflies in registers and a per-N twiddle table.
+//  * Lengths outside the FFT list use a direct O(N^2) LDS kernel that
+//    evaluates FFTW's defining sums with a cosine table.
+#pragma once
+
+#include "r2r.h"
+
+#include <cmath>
+#include <cstdint>
+
+namespace smaiso {
+
+// ---------------------------------------------------------------------------
+// butterflies
+// ---------------------------------------------------------------------------
+template <typename T, int SIGN>
+__device__ __forceinline__ cplx<T> mul_i(cplx<T> a) {
+  // a * (SIGN * i)
+  if constexpr (SIGN < 0) return {a.y, -a.x};
+  else return {-a.y, a.x};
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft2(cplx<T>& a0, cplx<T>& a1) {
+  cplx<T> t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3) {
+  cplx<T> t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  cplx<T> t2 = cadd(a1, a3), t3 = mul_i<T, SIGN>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft3(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2) {
+  const T h = T(0.8660254037844386);  // sqrt(3)/2
+  cplx<T> t = cadd(a1, a2);
+  cplx<T> d = csub(a1, a2);
+  cplx<T> u = mul_i<T, SIGN>(cplx<T>{d.x * h, d.y * h});
+  cplx<T> m = {a0.x - T(0.5) * t.x, a0.y - T(0.5) * t.y};
+  a0 = cadd(a0, t);
+  a1 = cadd(m, u);
+  a2 = csub(m, u);
+}
+
+// w8^s = exp(SIGN i pi s / 4)
+template <typename T, int SIGN, int S>
+__device__ __forceinline__ cplx<T> tw8(cplx<T> a) {
+  const T r = T(0.7071067811865476);
+  if constexpr (S == 0) return a;
+  else if constexpr (S == 1) return cmul(a, cplx<T>{r, T(SIGN) * r});
+  else if constexpr (S == 2) return mul_i<T, SIGN>(a);
+  else return cmul(a, cplx<T>{-r, T(SIGN) * r});
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft8(cplx<T>* a) {
+  cplx<T> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  cplx<T> o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<T, SIGN>(e0, e1, e2, e3);
+  dft4<T, SIGN>(o0, o1, o2, o3);
+  o1 = tw8<T, SIGN, 1>(o1);
+  o2 = tw8<T, SIGN, 2>(o2);
+  o3 = tw8<T, SIGN, 3>(o3);
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft16(cplx<T>* a) {
+  cplx<T> e[8], o[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { e[q] = a[2 * q]; o[q] = a[2 * q + 1]; }
+  dft8<T, SIGN>(e);
+  dft8<T, SIGN>(o);
+  // w16^s = exp(SIGN i pi s / 8)
+  const T c1 = T(0.9238795325112867), s1 = T(0.3826834323650898), r = T(0.7071067811865476);
+  const T sg = T(SIGN);
+  o[1] = cmul(o[1], cplx<T>{c1, sg * s1});
+  o[2] = cmul(o[2], cplx<T>{r, sg * r});
+  o[3] = cmul(o[3], cplx<T>{s1, sg * c1});
+  o[4] = mul_i<T, SIGN>(o[4]);
+  o[5] = cmul(o[5], cplx<T>{-s1, sg * c1});
+  o[6] = cmul(o[6], cplx<T>{-r, sg * r});
+  o[7] = cmul(o[7], cplx<T>{-c1, sg * s1});
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    a[q] = cadd(e[q], o[q]);
+    a[q + 8] = csub(e[q], o[q]);
+  }
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft(cplx<T>* a) {
+  if constexpr (R == 2) dft2<T, SIGN>(a[0], a[1]);
+  else if constexpr (R == 3) dft3<T, SIGN>(a[0], a[1], a[2]);
+  else if constexpr (R == 4) dft4<T, SIGN>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<T, SIGN>(a);
+  else if constexpr (R == 16) dft16<T, SIGN>(a);
+}
+
+// ---------------------------------------------------------------------------
+// compile-time FFT plans: radix of stage s for length n when no radix may
+// exceed maxr (16, 8 or 4); 0 ends the plan
+// ---------------------------------------------------------------------------
+constexpr int fft_radix(int n, int maxr, int s) {
+  if (maxr >= 16) {
+    switch (n) {
+      case 16: { const int r[] = {16, 0}; return r[s < 1 ? s : 1]; }
+      case 32: { const int r[] = {16, 2, 0}; return r[s < 2 ? s : 2]; }
+      case 48: { const int r[] = {16, 3, 0}; return r[s < 2 ? s : 2]; }
+      case 64: { const int r[] = {16, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 96: { const int r[] = {16, 3, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 128: { const int r[] = {16, 8, 0}; return r[s < 2 ? s : 2]; }
+      case 192: { const int r[] = {16, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 256: { const int r[] = {16, 16, 0}; return r[s < 2 ? s : 2]; }
+      case 384: { const int r[] = {16, 8, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 512: { const int r[] = {16, 8, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 1024: { const int r[] = {16, 16, 4, 0}; return r[s < 3 ? s : 3]; }
+      default: return 0;
+    }
+  } else if (maxr >= 8) {
+    switch (n) {
+      case 16: { const int r[] = {8, 2, 0}; return r[s < 2 ? s : 2]; }
+      case 32: { const int r[] = {8, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 48: { const int r[] = {8, 3, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 64: { const int r[] = {8, 8, 0}; return r[s < 2 ? s : 2]; }
+      case 96: { const int r[] = {8, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 128: { const int r[] = {8, 4, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 192: { const int r[] = {8, 8, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 256: { const int r[] = {8, 8, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 384: { const int r[] = {8, 8, 3, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 512: { const int r[] = {8, 8, 8, 0}; return r[s < 3 ? s : 3]; }
+      case 1024: { const int r[] = {8, 8, 4, 4, 0}; return r[s < 4 ? s : 4]; }
+      default: return 0;
+    }
+  } else {
+    switch (n) {
+      case 16: { const int r[] = {4, 4, 0}; return r[s < 2 ? s : 2]; }
+      case 32: { const int r[] = {4, 4, 2, 0}; return r[s < 3 ? s : 3]; }
+      case 48: { const int r[] = {4, 4, 3, 0}; return r[s < 3 ? s : 3]; }
+      case 64: { const int r[] = {4, 4, 4, 0}; return r[s < 3 ? s : 3]; }
+      case 96: { const int r[] = {4, 4, 3, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 128: { const int r[] = {4, 4, 4, 2, 0}; return r[s < 4 ? s : 4]; }
+      case 192: { const int r[] = {4, 4, 4, 3, 0}; return r[s < 4 ? s : 4]; }
+      case 256: { const int r[] = {4, 4, 4, 4, 0}; return r[s < 4 ? s : 4]; }
+      case 384: { const int r[] = {4, 4, 4, 3, 2, 0}; return r[s < 5 ? s : 5]; }
+      case 512: { const int r[] = {4, 4, 4, 4, 2, 0}; return r[s < 5 ? s : 5]; }
+      case 1024: { const int r[] = {4, 4, 4, 4, 4, 0}; return r[s < 5 ? s : 5]; }
+      default: return 0;
+    }
+  }
+}
+
+constexpr int ns_before(int n, int maxr, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= fft_radix(n, maxr, i);
+  return p;
+}
+
+// Largest radix that still gives every thread of the block a butterfly.
+constexpr int pick_maxr(int n, int p, int nt) {
+  return (p * n / 16 >= nt) ? 16 : (p * n / 8 >= nt) ? 8 : 4;
+}
+
+template <typename T>
+struct PassArgs {
+  const T* in;
+  const T* in2;
+  T* out;
+  const cplx<T>* tw;
+  const cplx<T>* mk;
+  int nx, ny, nz;
+  int axis;
+  int kind;
+  int vec;  // 16-byte pair accesses allowed (nz even, pointers 16-byte aligned)
+  Fuse f;
+};
+
+struct TileGeo {
+  long long base;
+  long long line_step;
+  long long stride_m;
+  int nvalid;
+  int outer;  // y/x passes: the fixed index (i for y, j for x); z pass: first line
+  int k0;     // y/x passes: first column
+};
+
+template <int TL, int N>
+__device__ __forceinline__ TileGeo tile_geometry(int axis, int nx, int ny, int nz) {
+  TileGeo t;
+  if (axis == 2) {
+    long long first = (long long)blockIdx.x * TL;
+    long long nl = (long long)nx * ny;
+    t.nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
+    t.base = first * N;
+    t.line_step = N;
+    t.stride_m = 1;
+    t.outer = (int)first;
+    t.k0 = 0;
+  } else {
+    int tpo = (nz + TL - 1) / TL;
+    int o = blockIdx.x / tpo;
+    int k0 = (blockIdx.x - o * tpo) * TL;
+    t.nvalid = (nz - k0) < TL ? (nz - k0) : TL;
+    t.line_step = 1;
+    t.outer = o;
+    t.k0 = k0;
+    if (axis == 1) {
+      t.base = (long long)o * ny * nz + k0;
+      t.stride_m = nz;
+    } else {
+      t.base = (long long)o * nz + k0;
+      t.stride_m = (long long)ny * nz;
+    }
+  }
+  return t;
+}
+
+// Sequence p of the tile lives at buf[p * SEQ], SEQ = N + 1 complex values:
+// the pad keeps lanes that walk across sequences off a single LDS bank.
+template <typename T, int N, int P, int NT, int MAXR, int SIGN, int S, int SEQ = N + 1>
+__device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+  constexpr int R = fft_radix(N, MAXR, S);
+  if constexpr (R != 0) {
+    constexpr int NS = ns_before(N, MAXR, S);
+    constexpr int J = N / R;
+    constexpr int TOT = P * J;
+    constexpr int IT = (TOT + NT - 1) / NT;
+    constexpr int STRIDE = N / (NS * R);
+    cplx<T> v[IT][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int w = tid + it * NT;
+      if (TOT % NT == 0 || w < TOT) {
+        const int p = w / J;
+        const int j = w - p * J;
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + j + q * J];
+        if constexpr (NS > 1) {
+          const int k = j % NS;
+#pragma unroll
+          for (int q = 1; q < R; ++q) {
+            cplx<T> t = tw[q * k * STRIDE];
+            if constexpr (SIGN > 0) t.y = -t.y;
+            v[it][q] = cmul(v[it][q], t);
+          }
+        }
+        dft<T, R, SIGN>(v[it]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int w = tid + it * NT;
+      if (TOT % NT == 0 || w < TOT) {
+        const int p = w / J;
+        const int j = w - p * J;
+        const int k = j % NS;
+        const int idx = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[p * SEQ + idx + q * NS] = v[it][q];
+      }
+    }
+    __syncthreads();
+    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1, SEQ>(buf, tw);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused element operations
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+__device__ __forceinline__ double mq2c(const Fuse& f, int i, int j, int k) {
+  return sqd(f.vq[0][i]) + sqd(f.vq[1][j]) + sqd(f.vq[2][k]);
+}
+
+// psi update at one mode (S:2177-2190) with FP64 C1, C2: the new psiq from
+// psiq, psiq_old and the transformed nonlinear terms of this and the last step.
+template <typename T>
+__device__ __forceinline__ double psi_update_value(const Fuse& f, long long idx, int i, int j, int k) {
+  const double m2 = mq2c(f, i, j, k);
+  const double alin = f.eps - f.alpha * sqd(f.q02 - m2);
+  const double c1 = 2.0 - f.dt * f.rho_m * alin;
+  const double c2 = 1.5 - f.dt * f.rho_m * alin;
+  const double n1 = (double)static_cast<const T*>(f.src[0])[idx];
+  const double nold = (double)static_cast<const T*>(f.nlq_old)[idx];  // read even on the first step
+  const double n0 = f.first ? n1 : nold;
+  return (c1 * f.psiq[idx] - 0.5 * f.psiq_new[idx] + (f.dt / 2) * f.scale * (3.0 * n1 - n0)) / c2;
+}
+
+// z-line index g = i*ny + j -> (i, j) with 32-bit arithmetic
+__device__ __forceinline__ void line_ij(long long g, int ny, int& i, int& j) {
+  const unsigned gu = (unsigned)g;
+  const unsigned iu = gu / (unsigned)ny;
+  i = (int)iu;
+  j = (int)(gu - iu * (unsigned)ny);
+}
+
+// Element m of z-line g (= i*ny + j) of the first pass under load op OP;
+// idx = g*nz + m.
+template <typename T, int OP>
+__device__ __forceinline__ T load_elem(const PassArgs<T>& a, int i, int j, int m, long long idx) {
+  if constexpr (OP == OP_PLAIN) {
+    T v = a.in[idx];
+    if (a.in2) v += a.in2[idx];
+    return v;
+  } else if constexpr (OP == OP_LD_PROD) {
+    // rho (FP64 state) times L psi
+    return (T)(static_cast<const double*>(a.f.src[0])[idx] * (double)static_cast<const T*>(a.f.src[1])[idx]);
+  } else if constexpr (OP == OP_LD_AMP) {
+    const T ps = static_cast<const T*>(a.f.src[0])[idx];
+    const T x = static_cast<const T*>(a.f.src[1])[idx];
+    const T y = static_cast<const T*>(a.f.src[2])[idx];
+    const T z = static_cast<const T*>(a.f.src[3])[idx];
+    return sqrt((T)a.f.q02 * (ps * ps) + x * x + y * y + z * z);
+  } else if constexpr (OP == OP_LD_GRAD_D || OP == OP_LD_GRAD_T) {
+    // branch-free: the load always happens (at a clamped address) so that the
+    // compiler can keep every load of the tile in flight
+    const int ax = a.f.axis;
+    const int na = ax == 0 ? i : (ax == 1 ? j : m);
+    const int nmax = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+    const long long off = ax == 0 ? (long long)a.ny * a.nz : (ax == 1 ? (long long)a.nz : 1LL);
+    const bool ok = na < nmax - 1;
+    const double w = a.f.vs[ax][ok ? na : 0];
+    const long long src_idx = ok ? idx + off : idx;
+    double src;
+    if constexpr (OP == OP_LD_GRAD_D) src = static_cast<const double*>(a.f.src[0])[src_idx];
+    else src = (double)static_cast<const T*>(a.f.src[0])[src_idx];
+    return ok ? (T)((-a.f.c0 * w) * src) : T(0);
+  } else if constexpr (OP == OP_LD_SH || OP == OP_LD_RELAXLAP) {
+    const double m2 = mq2c(a.f, i, j, m);
+    const double ps = static_cast<const double*>(a.f.src[0])[idx];
+    if constexpr (OP == OP_LD_SH) return (T)(a.f.alpha * (a.f.q02 - m2) * ps);
+    else return (T)(-m2 * a.f.alpha * (a.f.q02 - m2) * ps);
+  } else if constexpr (OP == OP_LD_DIV) {
+    const T* sx = static_cast<const T*>(a.f.src[0]);
+    const T* sy = static_cast<const T*>(a.f.src[1]);
+    const T* sz = static_cast<const T*>(a.f.src[2]);
+    const double vx = (double)sx[i > 0 ? idx - (long long)a.ny * a.nz : idx];
+    const double vy = (double)sy[j > 0 ? idx - a.nz : idx];
+    const double vz = (double)sz[m > 0 ? idx - 1 : idx];
+    return (T)(a.f.c0 * (a.f.vq[0][i] * (i > 0 ? vx : 0.0) + a.f.vq[1][j] * (j > 0 ? vy : 0.0) +
+                         a.f.vq[2][m] * (m > 0 ? vz : 0.0)));
+  } else if constexpr (OP == OP_LD_PSI) {
+    // the new psiq is the transform's input and is also stored (over psiq_old)
+    const double r = psi_update_value<T>(a.f, idx, i, j, m);
+    a.f.psiq_new[idx] = r;
+    return (T)r;
+  } else {
+    return T(0);
+  }
+}
+
+template <typename T>
+__device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo) {
+  int i, j;
+  line_ij(g, a.ny, i, j);
+  const long long idx = mo + g * a.nz + m;
+  switch (a.f.op) {
+    case OP_LD_GRAD_D: return load_elem<T, OP_LD_GRAD_D>(a, i, j, m, idx);
+    case OP_LD_GRAD_T: return load_elem<T, OP_LD_GRAD_T>(a, i, j, m, idx);
+    case OP_LD_AMP: return load_elem<T, OP_LD_AMP>(a, i, j, m, idx);
+    case OP_LD_SH: return load_elem<T, OP_LD_SH>(a, i, j, m, idx);
+    case OP_LD_RELAXLAP: return load_elem<T, OP_LD_RELAXLAP>(a, i, j, m, idx);
+    case OP_LD_PROD: return load_elem<T, OP_LD_PROD>(a, i, j, m, idx);
+    case OP_LD_DIV: return load_elem<T, OP_LD_DIV>(a, i, j, m, idx);
+    case OP_LD_PSI: return load_elem<T, OP_LD_PSI>(a, i, j, m, idx);
+    default: return load_elem<T, OP_PLAIN>(a, i, j, m, idx);
+  }
+}
+
+// Diagonal multiplier of the mid ops at mode (i, j, k); y is the forward
+// output, evaluated in the reference's order: (scale y) G, (-scale |q|^2) y.
+template <typename T, int OP>
+__device__ __forceinline__ T mid_apply(const Fuse& f, int i, int j, int k, T y) {
+  const double m2 = mq2c(f, i, j, k);
+  if constexpr (OP == OP_MID_AMP) return (T)(f.scale * (double)y * exp(-1.57 * 1.57 * m2 * 2));
+  else return (T)((-f.scale * m2) * (double)y);
+}
+
+// Adjacent columns l0, l0+1 of a strided tile at row m (zero outside the grid).
+template <typename T>
+__device__ __forceinline__ cplx<T> load_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m) {
+  const long long addr = g.base + l0 + (long long)m * g.stride_m;
+  cplx<T> v = {T(0), T(0)};
+  if (a.vec && l0 + 1 < g.nvalid) {
+    v = *reinterpret_cast<const cplx<T>*>(a.in + addr);
+    if (a.in2) {
+      const cplx<T> w = *reinterpret_cast<const cplx<T>*>(a.in2 + addr);
+      v.x += w.x;
+      v.y += w.y;
+    }
+  } else {
+    if (l0 < g.nvalid) { v.x = a.in[addr]; if (a.in2) v.x += a.in2[addr]; }
+    if (l0 + 1 < g.nvalid) { v.y = a.in[addr + 1]; if (a.in2) v.y += a.in2[addr + 1]; }
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m, T y0,
+                                           T y1) {
+  T* o = a.out + g.base + l0 + (long long)m * g.stride_m;
+  if (a.vec && l0 + 1 < g.nvalid) {
+    *reinterpret_cast<cplx<T>*>(o) = cplx<T>{y0, y1};
+  } else {
+    if (l0 < g.nvalid) o[0] = y0;
+    if (l0 + 1 < g.nvalid) o[1] = y1;
+  }
+}
+
+// Makhoul post-processing of item k of one sequence: Z at k and kb (kb = N-k,
+// or N/2 for k = 0) -> the forward outputs of both lines at k and at kb.
+template <typename T, int N>
+__device__ __forceinline__ void makhoul_post(const cplx<T>* __restrict__ mk, int k, int kb, cplx<T> za,
+                                             cplx<T> zb, T& y0a, T& y1a, T& y0b, T& y1b) {
+  // partner of k is N-k: Z_k pairs with conj Z_{N-k}; item 0 pairs 0 with 0 and N/2 with N/2
+  const cplx<T> pa = (k == 0) ? za : zb;
+  const cplx<T> qa = (k == 0) ? zb : za;
+  {
+    const cplx<T> m = mk[k];
+    const cplx<T> S = {za.x + pa.x, za.y - pa.y};
+    const cplx<T> D = {za.x - pa.x, za.y + pa.y};
+    y0a = m.x * S.x - m.y * S.y;
+    y1a = m.x * D.y + m.y * D.x;
+  }
+  {
+    const cplx<T> m = mk[kb];
+    const cplx<T> S = {zb.x + qa.x, zb.y - qa.y};
+    const cplx<T> D = {zb.x - qa.x, zb.y + qa.y};
+    y0b = m.x * S.x - m.y * S.y;
+    y1b = m.x * D.y + m.y * D.x;
+  }
+}
+
+// Makhoul pre-processing of the inverse: (Y0, Y1) at k (rk) and at kb (rn) ->
+// z_k, z_kb of U0 + i U1 with U_k = e^{+i pi k/2N} (Y_k - i Y_{N-k}).
+template <typename T, int N>
+__device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int k, cplx<T> rk, cplx<T> rn,
+                                            cplx<T>& zk, cplx<T>& zb) {
+  constexpr int HN = N / 2;
+  if (k == 0) {
+    zk = rk;
+    cplx<T> cc = mk[HN];
+    cc.y = -cc.y;
+    zb = cmul(cc, cplx<T>{rn.x + rn.y, rn.y - rn.x});
+  } else {
+    cplx<T> cc = mk[k];
+    cc.y = -cc.y;
+    zk = cmul(cc, cplx<T>{rk.x + rn.y, rk.y - rn.x});
+    cc = mk[N - k];
+    cc.y = -cc.y;
+    zb = cmul(cc, cplx<T>{rn.x + rk.y, rn.y - rk.x});
+  }
+}
+
+// One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
+// form complex sequence c.  OP selects a fused operation (FuseOp).
+template <typename T, int N, int P, int NT, int OP>
+__global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
+  constexpr int TL = 2 * P;
+  constexpr int MAXR = pick_maxr(N, P, NT);
+  constexpr int SEQ = N + 1;
+  constexpr int HN = N / 2;
+  constexpr bool MID = (OP == OP_MID_AMP || OP == OP_MID_NEGLAP);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
+  T* rbuf = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x;
+  TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
+  g.base += mo;
+  const bool fwd = MID || (a.kind == REDFT10 || a.kind == RODFT10);
+  const bool sine = !MID && (a.kind == RODFT10 || a.kind == RODFT01);
+
+  if (fwd) {
+    // ---- load: every global load of the tile is issued before the first LDS
+    // write, so each thread keeps all of its loads in flight ---------------
+    if (a.axis == 2) {
+      constexpr int E = TL * N;
+      constexpr int IT = (E + NT - 1) / NT;
+      T v[IT];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        const int l = e / N, m = e - (e / N) * N;
+        v[it] = T(0);
+        if ((E % NT == 0 || e < E) && l < g.nvalid) {
+          int i, j;
+          const long long gl = (long long)g.outer + l;
+          line_ij(gl, a.ny, i, j);
+          v[it] = load_elem<T, MID ? OP_PLAIN : OP>(a, i, j, m, mo + gl * N + m);
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        if (E % NT == 0 || e < E) {
+          const int l = e / N, m = e - (e / N) * N;
+          T x = v[it];
+          if (sine && (m & 1)) x = -x;
+          const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+          rbuf[2 * ((l >> 1) * SEQ + pos) + (l & 1)] = x;
+        }
+      }
+    } else {
+      constexpr int E = P * N;  // pairs of adjacent columns
+      constexpr int IT = (E + NT - 1) / NT;
+      cplx<T> v[IT];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        v[it] = (E % NT == 0 || e < E) ? load_pair(a, g, 2 * (e % P), e / P) : cplx<T>{T(0), T(0)};
+      }
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        if (E % NT == 0 || e < E) {
+          const int c = e % P, m = e / P;
+          cplx<T> x = v[it];
+          if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
+          const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+          buf[c * SEQ + pos] = x;
+        }
+      }
+    }
+  } else {
+    // ---- load straight into Makhoul's pre-twiddled sequence: item k < N/2
+    // loads Y_k and its mirror Y_{N-k} (item 0 takes Y_0 and Y_{N/2}) for both
+    // lines of a pair.  DST-III reads the input reversed, Y_k = x_{N-1-k}.
+    constexpr int W = P * HN;
+    constexpr int ITW = (W + NT - 1) / NT;
+    cplx<T> rk[ITW], rn[ITW];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int w = tid + it * NT;
+      rk[it] = rn[it] = cplx<T>{T(0), T(0)};
+      if (W % NT == 0 || w < W) {
+        int c, k;
+        if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+        else { k = w / P; c = w - k * P; }
+        const int kb = (k == 0) ? HN : N - k;
+        const int pa = sine ? N - 1 - k : k;
+        const int pb = sine ? N - 1 - kb : kb;
+        if (a.axis == 2) {
+          const int l0 = 2 * c;
+          const long long g0 = (long long)g.outer + l0;
+          int i, j;
+          line_ij(g0, a.ny, i, j);
+          if (l0 < g.nvalid) {
+            rk[it].x = load_elem<T, OP>(a, i, j, pa, mo + g0 * N + pa);
+            rn[it].x = load_elem<T, OP>(a, i, j, pb, mo + g0 * N + pb);
+          }
+          if (l0 + 1 < g.nvalid) {
+            int i1 = i, j1 = j + 1;
+            if (j1 == a.ny) { j1 = 0; ++i1; }
+            rk[it].y = load_elem<T, OP>(a, i1, j1, pa, mo + (g0 + 1) * N + pa);
+            rn[it].y = load_elem<T, OP>(a, i1, j1, pb, mo + (g0 + 1) * N + pb);
+          }
+        } else {
+          rk[it] = load_pair(a, g, 2 * c, pa);
+          rn[it] = load_pair(a, g, 2 * c, pb);
+        }
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int w = tid + it * NT;
+      if (W % NT == 0 || w < W) {
+        int c, k;
+        if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+        else { k = w / P; c = w - k * P; }
+        const int kb = (k == 0) ? HN : N - k;
+        cplx<T> zk, zb;
+        makhoul_pre<T, N>(a.mk, k, rk[it], rn[it], zk, zb);
+        buf[c * SEQ + k] = zk;
+        buf[c * SEQ + kb] = zb;
+      }
+    }
+  }
+  __syncthreads();
+
+  if (fwd) {
+    fft_stage<T, N, P, NT, MAXR, -1, 0>(buf, a.tw);
+    constexpr int W = P * HN;
+    constexpr int ITW = (W + NT - 1) / NT;
+    if constexpr (MID) {
+      // forward outputs -> diagonal operator -> inverse pre-twiddle, in place:
+      // item k owns positions k and kb of its sequence, so no barrier is needed
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          T y0a, y1a, y0b, y1b;
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
+          const int jj = g.outer;
+          const int kc0 = g.k0 + 2 * c < a.nz ? g.k0 + 2 * c : a.nz - 1;
+          const int kc1 = g.k0 + 2 * c + 1 < a.nz ? g.k0 + 2 * c + 1 : a.nz - 1;
+          const cplx<T> rk = {mid_apply<T, OP>(a.f, k, jj, kc0, y0a), mid_apply<T, OP>(a.f, k, jj, kc1, y1a)};
+          const cplx<T> rn = {mid_apply<T, OP>(a.f, kb, jj, kc0, y0b), mid_apply<T, OP>(a.f, kb, jj, kc1, y1b)};
+          cplx<T> zk, zb;
+          makhoul_pre<T, N>(a.mk, k, rk, rn, zk, zb);
+          buf[c * SEQ + k] = zk;
+          buf[c * SEQ + kb] = zb;
+        }
+      }
+      __syncthreads();
+    } else {
+      // ---- post: item k < N/2 writes outputs k and N-k (item 0: 0 and N/2)
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          T y0a, y1a, y0b, y1b;
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          const int oa = sine ? (N - 1 - k) : k;
+          const int ob = sine ? (N - 1 - kb) : kb;
+          const int l0 = 2 * c;
+          if (a.axis == 2) {
+            if (l0 < g.nvalid) {
+              T* o = a.out + g.base + (long long)l0 * N;
+              o[oa] = y0a;
+              o[ob] = y0b;
+            }
+            if (l0 + 1 < g.nvalid) {
+              T* o = a.out + g.base + (long long)(l0 + 1) * N;
+              o[oa] = y1a;
+              o[ob] = y1b;
+            }
+          } else {
+            store_pair(a, g, l0, oa, y0a, y1a);
+            store_pair(a, g, l0, ob, y0b, y1b);
+          }
+        }
+      }
+      return;
+    }
+  }
+
+  // ---- inverse FFT and store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n}
+  fft_stage<T, N, P, NT, MAXR, +1, 0>(buf, a.tw);
+  constexpr int WO = P * HN;
+  constexpr int ITO = (WO + NT - 1) / NT;
+  const T s = sine ? T(-1) : T(1);
+#pragma unroll
+  for (int it = 0; it < ITO; ++it) {
+    const int w = tid + it * NT;
+    if (WO % NT == 0 || w < WO) {
+      int c, n;
+      if (a.axis == 2) { c = w / HN; n = w - c * HN; }
+      else { n = w / P; c = w - n * P; }
+      const cplx<T> ua = buf[c * SEQ + n];
+      const cplx<T> ub = buf[c * SEQ + N - 1 - n];
+      const int l0 = 2 * c;
+      if (a.axis == 2) {
+        if (l0 < g.nvalid) {
+          T* o = a.out + g.base + (long long)l0 * N + 2 * n;
+          if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.x, s * ub.x};
+          else { o[0] = ua.x; o[1] = s * ub.x; }
+        }
+        if (l0 + 1 < g.nvalid) {
+          T* o = a.out + g.base + (long long)(l0 + 1) * N + 2 * n;
+          if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.y, s * ub.y};
+          else { o[0] = ua.y; o[1] = s * ub.y; }
+        }
+      } else {
+        store_pair(a, g, l0, 2 * n, ua.x, ua.y);
+        store_pair(a, g, l0, 2 * n + 1, s * ub.x, s * ub.y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// direct O(N^2) fallback for any length, with the same fused operations
+// ---------------------------------------------------------------------------
+constexpr int kThreads = 256;
+
+template <typename T>
+struct DirectArgs {
+  PassArgs<T> p;
+  const T* costab;  // 4n entries: cos(pi t/(2n))
+  int n;
+  int tl;
+};
+
+// FFTW's defining sum of `kind` at output k for one line xl of length n; the
+// angle index t (units of pi/(2n)) is advanced incrementally mod 4n.
+template <typename T>
+__device__ T direct_sum(int kind, const T* xl, int n, int k, const T* __restrict__ ct) {
+  const int n4 = 4 * n;
+  T acc = T(0);
+  if (kind == REDFT10) {            // 2 sum x_m cos(pi (2m+1) k / 2n)
+    int t = k % n4, step = (2 * k) % n4;
+    for (int m = 0; m < n; ++m) { acc += xl[m] * ct[t]; t += step; if (t >= n4) t -= n4; }
+    return T(2) * acc;
+  } else if (kind == REDFT01) {     // x_0 + 2 sum_{m>=1} x_m cos(pi m (2k+1)/2n)
+    int step = (2 * k + 1) % n4, t = step;
+    for (int m = 1; m < n; ++m) { acc += xl[m] * ct[t]; t += step; if (t >= n4) t -= n4; }
+    return xl[0] + T(2) * acc;
+  } else if (kind == RODFT10) {     // 2 sum x_m sin(pi (2m+1)(k+1)/2n)
+    int step = (2 * (k + 1)) % n4;
+    int t = ((k + 1) - n) % n4; if (t < 0) t += n4;   // sin(u) = cos(u - pi/2)
+    for (int m = 0; m < n; ++m) { acc += xl[m] * ct[t]; t += step; if (t >= n4) t -= n4; }
+    return T(2) * acc;
+  } else {                          // (-1)^k x_{n-1} + 2 sum_{m<n-1} x_m sin(pi (m+1)(2k+1)/2n)
+    int step = (2 * k + 1) % n4;
+    int t = (step - n) % n4; if (t < 0) t += n4;
+    for (int m = 0; m < n - 1; ++m) { acc += xl[m] * ct[t]; t += step; if (t >= n4) t -= n4; }
+    return ((k & 1) ? -xl[n - 1] : xl[n - 1]) + T(2) * acc;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const PassArgs<T>& a = d.p;
+  const int n = d.n, tl = d.tl;
+  T* x = reinterpret_cast<T*>(smem_raw);
+  T* y = x + tl * n;  // second buffer of the mid ops
+  const bool mid = a.f.op == OP_MID_AMP || a.f.op == OP_MID_NEGLAP;
+  TileGeo g;
+  if (a.axis == 2) {
+    long long first = (long long)blockIdx.x * tl;
+    long long nl = (long long)a.nx * a.ny;
+    g.nvalid = (int)((nl - first) < tl ? (nl - first) : tl);
+    g.base = first * n;
+    g.line_step = n;
+    g.stride_m = 1;
+    g.outer = (int)first;
+    g.k0 = 0;
+  } else {
+    int tpo = (a.nz + tl - 1) / tl;
+    int o = blockIdx.x / tpo;
+    int k0 = (blockIdx.x - o * tpo) * tl;
+    g.nvalid = (a.nz - k0) < tl ? (a.nz - k0) : tl;
+    g.line_step = 1;
+    g.outer = o;
+    g.k0 = k0;
+    if (a.axis == 1) { g.base = (long long)o * a.ny * a.nz + k0; g.stride_m = a.nz; }
+    else { g.base = (long long)o * a.nz + k0; g.stride_m = (long long)a.ny * a.nz; }
+  }
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
+  g.base += mo;
+  const int E = tl * n;
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int l, m;
+    if (a.axis == 2) { l = e / n; m = e - l * n; }
+    else { m = e / tl; l = e - m * tl; }
+    T v = T(0);
+    if (l < g.nvalid) {
+      if (a.axis == 2) {
+        v = load_elem_rt(a, (long long)g.outer + l, m, mo);
+      } else {
+        const long long addr = g.base + l * g.line_step + m * g.stride_m;
+        v = a.in[addr];
+        if (a.in2) v += a.in2[addr];
+      }
+    }
+    x[l * n + m] = v;
+  }
+  __syncthreads();
+  if (mid) {
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+      const int k = e / tl, l = e - (e / tl) * tl;
+      if (l >= g.nvalid) continue;
+      const T v = direct_sum<T>(REDFT10, x + l * n, n, k, d.costab);
+      const int kc = g.k0 + l;
+      y[l * n + k] = (a.f.op == OP_MID_AMP) ? mid_apply<T, OP_MID_AMP>(a.f, k, g.outer, kc, v)
+                                            : mid_apply<T, OP_MID_NEGLAP>(a.f, k, g.outer, kc, v);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < E; e += blockDim.x) {
+    int l, k;
+    if (a.axis == 2) { l = e / n; k = e - l * n; }
+    else { k = e / tl; l = e - k * tl; }
+    if (l >= g.nvalid) continue;
+    const T r = mid ? direct_sum<T>(REDFT01, y + l * n, n, k, d.costab)
+                    : direct_sum<T>(a.kind, x + l * n, n, k, d.costab);
+    const long long oi = g.base + l * g.line_step + k * g.stride_m;
+    a.out[oi] = r;
+  }
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <typename T>
+PassArgs<T> make_args(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g) {
+  PassArgs<T> a;
+  a.in = static_cast<const T*>(io.in);
+  a.in2 = static_cast<const T*>(io.in2);
+  a.out = static_cast<T*>(io.out);
+  a.tw = static_cast<const cplx<T>*>(tw);
+  a.mk = static_cast<const cplx<T>*>(mk);
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+  a.axis = axis;
+  a.kind = kind;
+  a.vec = (g.nz % 2 == 0) && aligned16(io.in) && aligned16(io.out) &&
+          (io.in2 == nullptr || aligned16(io.in2));
+  a.f = io.f;
+  return a;
+}
+
+template <typename T, int N, int P, int NT, int OP>
+hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
+                         hipStream_t s) {
+  constexpr int TL = 2 * P;
+  PassArgs<T> a = make_args<T>(io, tw, mk, kind, axis, g);
+  long long blocks;
+  if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
+  else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
+  const size_t lds = (size_t)P * (N + 1) * sizeof(cplx<T>);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// tile variants that R2R::autotune times for plain passes (variant 0 is the
+// default TileChoice of r2r.hip): lines per tile 2P and threads NT
+// ---------------------------------------------------------------------------
+template <typename T, bool Z, int V>
+struct TileVariant;
+template <typename T> struct TileVariant<T, true, 1> { static constexpr int P = 1, NT = 64; };
+template <typename T> struct TileVariant<T, true, 2> { static constexpr int P = 2, NT = 128; };
+template <typename T> struct TileVariant<T, true, 3> { static constexpr int P = 4, NT = 128; };
+template <typename T> struct TileVariant<T, false, 1> {
+  static constexpr int P = sizeof(T) == 8 ? 4 : 8, NT = sizeof(T) == 8 ? 128 : 256;
+};
+template <typename T> struct TileVariant<T, false, 2> {
+  static constexpr int P = sizeof(T) == 8 ? 8 : 16, NT = sizeof(T) == 8 ? 512 : 256;
+};
+template <typename T> struct TileVariant<T, false, 3> {
+  static constexpr int P = sizeof(T) == 8 ? 16 : 32, NT = sizeof(T) == 8 ? 256 : 512;
+};
+
+// A variant is built for length N when its LDS tile fits 64 KiB and each
+// thread holds at most 256 bytes of the tile in registers.
+template <typename T, int N, int P, int NT>
+constexpr bool tile_fits() {
+  return (long long)P * (N + 1) * (long long)sizeof(cplx<T>) <= 65536 &&
+         (2LL * P * N / NT) * (long long)sizeof(T) <= 256 && NT % 64 == 0 && NT <= 1024;
+}
+
+// Plain pass of length n with tile variant v (1..kTileVariants-1);
+// hipErrorNotSupported when the variant is not built for n.  Defined in
+// r2r_tune_f64.hip / r2r_tune_f32.hip.
+template <typename T>
+hipError_t launch_plain_variant(int n, int v, const PassIO& io, void* tw, void* mk, int kind, int axis,
+                                const Geom& g, hipStream_t s);
+
+}  // namespace smaiso

@@ -31,6 +31,7 @@ FIELDS = {
     "nlq": 18, "f_rho": 19, "divf": 20, "gpx": 21, "gpy": 22, "gpz": 23,
     "rdx": 24, "rdy": 25, "rdz": 26, "lap_rl": 27, "psi_old": 28,
 }
+TILE_VARIANTS = 4
 KERNEL_CLASSES = {"r2r_z": 0, "r2r_y": 1, "r2r_x": 2, "spectral": 3, "pointwise": 4, "reduce": 5}
 
 
@@ -98,6 +99,10 @@ def lib():
         L.smaiso_timing_enable.argtypes = [vp, C.c_int32]
         L.smaiso_timing_get.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
         L.smaiso_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_autotune.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
+        L.smaiso_ensemble_autotune.argtypes = [vp]
+        L.smaiso_r2r3d_tiles.argtypes = [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, vp, vp,
+                                         C.c_int32, C.c_int32, vp]
         L.smaiso_adjoint.argtypes = [vp, dp, dp, dp, C.c_int32, C.c_int32, dp, dp, dp, dp, dp]
         L.smaiso_interface_ic_vjp.argtypes = [C.POINTER(CParams), C.POINTER(CIC), dp, dp, dp, dp]
         L.smaiso_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(vp)]
@@ -119,7 +124,8 @@ def lib():
                      "smaiso_timing_get", "smaiso_set_graph", "smaiso_ensemble_create",
                      "smaiso_ensemble_destroy", "smaiso_ensemble_groups", "smaiso_ensemble_set_state",
                      "smaiso_ensemble_set_graph", "smaiso_ensemble_step", "smaiso_ensemble_get_field",
-                     "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_interface_ic_vjp"):
+                     "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_interface_ic_vjp",
+                     "smaiso_autotune", "smaiso_ensemble_autotune", "smaiso_r2r3d_tiles"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -277,6 +283,15 @@ class Stepper:
         """Replay pairs of steady steps from one captured hipGraph (smaiso_set_graph)."""
         _check(lib().smaiso_set_graph(self.h, 1 if enable else 0), self.h)
 
+    def autotune(self):
+        """Pick tile variants for this handle's plain passes (smaiso_autotune).
+        Returns (ms[dtype][axis][variant], chosen[dtype][axis]), dtype 0 = FP64, 1 = FP32."""
+        ms = (C.c_float * (2 * 3 * TILE_VARIANTS))()
+        ch = (C.c_int32 * 6)()
+        _check(lib().smaiso_autotune(self.h, ms, ch), self.h)
+        return (np.array(ms[:], dtype=np.float64).reshape(2, 3, TILE_VARIANTS),
+                np.array(ch[:], dtype=np.int32).reshape(2, 3))
+
     def adjoint(self, psi0: np.ndarray, rho0: np.ndarray, target: np.ndarray, nsteps: int,
                 checkpoint_every: int = 50) -> "AdjointResult":
         """Loss sum (psi(T) - target)^2 after nsteps from set_state(psi0, rho0)
@@ -366,6 +381,9 @@ class Ensemble:
     def set_graph(self, enable: bool):
         _echeck(lib().smaiso_ensemble_set_graph(self.e, 1 if enable else 0), self.e)
 
+    def autotune(self):
+        _echeck(lib().smaiso_ensemble_autotune(self.e), self.e)
+
     def step(self, n: int = 1):
         _echeck(lib().smaiso_ensemble_step(self.e, int(n)), self.e)
 
@@ -399,6 +417,13 @@ def r2r3d_device(in_ptr: int, out_ptr: int, shape, kinds, dtype: int = FP64, str
     k = (C.c_int32 * 3)(*[int(v) for v in kinds])
     _check(lib().smaiso_r2r3d_device(k, shape[0], shape[1], shape[2], C.c_void_p(in_ptr),
                                      C.c_void_p(out_ptr), int(dtype), C.c_void_p(stream)))
+
+
+def r2r3d_tiles(in_ptr: int, out_ptr: int, shape, kinds, dtype: int, variant: int, stream: int = 0):
+    """r2r3d_device with one tile variant forced on every axis (smaiso_r2r3d_tiles)."""
+    k = (C.c_int32 * 3)(*[int(v) for v in kinds])
+    _check(lib().smaiso_r2r3d_tiles(k, shape[0], shape[1], shape[2], C.c_void_p(in_ptr),
+                                    C.c_void_p(out_ptr), int(dtype), int(variant), C.c_void_p(stream)))
 
 
 @dataclass

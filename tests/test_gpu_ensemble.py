@@ -210,3 +210,49 @@ def test_ensemble_errors():
         en.set_state(1, psi, rho)
         en.step(2)
         assert en.diagnostics(0) == en.diagnostics(1)
+
+
+@pytest.mark.parametrize("precision", [S.FP64, S.FP32_MIXED])
+def test_autotune_picks_fastest_and_keeps_parity(precision):
+    sp, op = params_pair(nx=64, ny=48, nz=32, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0,
+                         precision=precision)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(undulation=0.5, seed=8))
+    st = S.Stepper(sp)
+    ms, chosen = st.autotune()
+    assert ms.shape == (2, 3, S.TILE_VARIANTS) and chosen.shape == (2, 3)
+    for d in range(2 if precision == S.FP32_MIXED else 1):
+        for a in range(3):
+            timed = ms[d, a] >= 0
+            assert timed[0] and timed.sum() >= 2
+            assert ms[d, a, chosen[d, a]] == ms[d, a][timed].min()
+    if precision == S.FP64:
+        assert np.all(ms[1] == -1)
+    st.set_graph(True)
+    st.set_state(psi, rho)
+    st.step(10)
+    o = O.Oracle(op)
+    o.set_state(psi, rho)
+    o.step(10)
+    tol = 1e-10 if precision == S.FP64 else 1e-4
+    for name, ref in (("psi", o.f.psi), ("rho", o.f.rho), ("vx", o.f.extra["vx"])):
+        mx, l2 = rel_err(st.field(name), ref)
+        assert mx < tol and l2 < tol, (name, mx, l2)
+
+
+def test_ensemble_autotune():
+    members = _mixed_members()
+    ics = [ic_for(p, 30 + m) for m, p in enumerate(members)]
+    with S.Ensemble(members) as en:
+        for m, (psi, rho) in enumerate(ics):
+            en.set_state(m, psi, rho)
+        en.step(3)
+        en.autotune()  # mid-run: the captured graph is dropped and recaptured
+        en.step(5)
+        for m, p in enumerate(members):
+            st = S.Stepper(p)
+            st.set_state(*ics[m])
+            st.step(8)
+            tol = 1e-12 if p.precision == S.FP64 else 1e-5
+            for k in ("psi", "rho", "vz"):
+                mx, l2 = rel_err(en.field(m, k), st.field(k))
+                assert mx < tol and l2 < tol, (m, k, mx, l2)

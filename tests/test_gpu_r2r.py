@@ -102,3 +102,26 @@ def test_device_api_in_place():
     S.r2r3d_device(a.ptr, b.ptr, x.shape, O.CT, S.FP32_MIXED)
     mx, _ = rel_err(b.get().astype(np.float64), O.r2r3d(xf.astype(np.float64), O.CT))
     assert mx < 2e-5
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("dtype", [S.FP64, S.FP32_MIXED])
+@pytest.mark.parametrize("shape", [(64, 96, 128), (256, 32, 48), (16, 512, 32)])
+def test_tile_variants(variant, dtype, shape):
+    """Every autotuner tile variant gives the transform (all four kinds on
+    the three axes)."""
+    from hipmem import DeviceArray
+    rng = np.random.default_rng(variant + 10 * dtype)
+    x = rng.standard_normal(shape)
+    npdt = np.float64 if dtype == S.FP64 else np.float32
+    tol = 1e-12 if dtype == S.FP64 else 2e-5
+    for kinds in (O.CT, O.ICT, O.ST[0], O.IST[2]):
+        xin = x.astype(npdt)
+        a, b = DeviceArray(xin), DeviceArray.empty(shape, npdt)
+        try:
+            S.r2r3d_tiles(a.ptr, b.ptr, shape, kinds, dtype, variant)
+        except S.SmaisoError as e:
+            assert e.code == S.E_INVALID and "not built" in str(e)
+            continue
+        mx, _ = rel_err(b.get().astype(np.float64), O.r2r3d(xin.astype(np.float64), kinds))
+        assert mx < tol, (kinds, mx)
