@@ -26,7 +26,7 @@ struct AdjArgs {
   const T* lpsi;      // L psi
   const T* laprl;     // lap(rho L psi)
   const T* gp[3];     // grad psi
-  const T* rd[3];     // grad rho
+  const double* rd[3];  // grad rho (FP64 in both precisions)
   const T* v[3];      // total velocity (real space)
   const T* divv;      // div v (real space)
   const T* vsolq[3];  // solenoidal velocity on the staggered lattices (spectral)

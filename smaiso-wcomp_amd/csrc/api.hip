@@ -82,14 +82,14 @@ struct smaiso_handle {
   double* vs[3] = {nullptr, nullptr, nullptr};
   // FP64 state
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
-  double* rhoq = nullptr;  // DCT of rho, FP64 in both precisions (rho is ~constant: its
-                           // gradient is a small difference of large coefficients)
+  double* rd[3] = {nullptr, nullptr, nullptr};  // grad rho, FP64 in both precisions (rho is
+                                               // ~constant: its gradient is a small difference)
   double *partials = nullptr, *diag_out = nullptr, *out64 = nullptr;
   std::vector<double> diag_host;
   long long diag_epoch = -1;  // epoch of the cached diagnostics (-1: none)
   // work fields (T = double or float)
   void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
-  void *gp[3] = {}, *rd[3] = {}, *S[3] = {}, *f[3] = {}, *vsolq[3] = {}, *virrq[3] = {}, *v[3] = {};
+  void *gp[3] = {}, *S[3] = {}, *f[3] = {}, *vsolq[3] = {}, *virrq[3] = {}, *v[3] = {};
   void *amp = nullptr, *lpsi = nullptr, *laprl = nullptr, *mu = nullptr, *fgrad = nullptr,
        *erho = nullptr, *divv = nullptr, *divf = nullptr, *nl = nullptr, *tmp = nullptr;
 
@@ -218,6 +218,8 @@ struct smaiso_handle {
     else if (op == OP_LD_PROD) in = 8 + es;
     else if (op == OP_LD_DIV) in = 3 * es;
     else if (op == OP_LD_PSI) { in = 16 + 2 * es; out = es + 8; }
+    else if (op == OP_AX_GAUSS_AMP) in = 4 * es;
+    else if (op == OP_AX_DIV && io.f.accumulate) in = 2 * es;
     return n * (in + out);
   }
   hipError_t run_pass(DType dt, int kind, int axis, const PassIO& io, const Geom& gg) {
@@ -265,6 +267,14 @@ struct smaiso_handle {
     io.f.op = OP_PLAIN;
     if ((e = run_pass(dt, REDFT01, 1, io, g)) != hipSuccess) return e;
     return run_pass(dt, REDFT01, 2, io, g);
+  }
+  // one-axis operator pass (OP_AX_*) along `axis`
+  hipError_t axpass(DType dt, int axis, const void* in, void* out, const Fuse& f) {
+    PassIO io;
+    io.in = in;
+    io.out = out;
+    io.f = f;
+    return run_pass(dt, REDFT10, axis, io, g);
   }
   double fb(int nfields) const { return (double)ntot() * tsz * nfields; }
 };
@@ -429,7 +439,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->mc = static_cast<MemberConsts*>(tmpp);
   if ((e = hipMemcpy(h->mc, h->mc_host.data(), sizeof(MemberConsts) * n, hipMemcpyHostToDevice)) != hipSuccess)
     return hipfail(e, "hipMemcpy");
-  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rhoq};
+  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rd[0], &h->rd[1], &h->rd[2]};
   for (double** b : dbufs) {
     if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc (state)");
     *b = static_cast<double*>(tmpp);
@@ -442,7 +452,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
                                &h->divf, &h->nl, &h->tmp};
   for (int a = 0; a < 3; ++a) {
-    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
+    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->S[a]);
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vsolq[a]); tbufs.push_back(&h->virrq[a]);
     tbufs.push_back(&h->v[a]);
   }
@@ -466,7 +476,7 @@ int run_step(smaiso_handle* h) {
   hipStream_t s = h->stream;
   const StepConsts c = h->consts();
   T* const* gp = reinterpret_cast<T* const*>(h->gp);
-  T* const* rd = reinterpret_cast<T* const*>(h->rd);
+  double* const* rd = h->rd;
   T* const* S = reinterpret_cast<T* const*>(h->S);
   T* const* f = reinterpret_cast<T* const*>(h->f);
   T* const* vsolq = reinterpret_cast<T* const*>(h->vsolq);
@@ -480,25 +490,39 @@ int run_step(smaiso_handle* h) {
   h->epoch += 1;
   std::swap(h->nlq, h->nlq_old);  // Nl_old_local = Nl_local, S:1100
 
-  // a6 (S:1104-1250): rho^ = DCT(rho) in FP64; the DCT->DST shift and -Vs
-  // (times scale for rho) are applied on load by the z pass of each inverse
-  // transform
-  HIPTRY(h, h->xform(F64, kCT, h->rho, nullptr, h->rhoq));
+  // a6 (S:1104-1250): grad_a = IST_a(-Vs_a shift(scale CT(.))) acts on one
+  // axis only (iDCT.DCT = 2N on the others), so each component is one pass
+  // along its axis: from rho in FP64, and in FP64 from psi (= ICT(psiq)).
+  // The FP32-mixed path keeps the 3-D form for grad psi: the shift is
+  // applied on load to the FP64 coefficients psiq, each rounded relative to
+  // its own size, where a pass over the FP32 copy of psi would amplify that
+  // copy's white rounding noise by |q|
   for (int a = 0; a < 3; ++a) {
     Fuse g = fb;
     g.axis = a;
-    g.op = OP_LD_GRAD_D; g.src[0] = h->psiq; g.c0 = 1.0;
-    HIPTRY(h, h->xwf(kIST[a], g, gp[a]));
-    g.op = OP_LD_GRAD_D; g.src[0] = h->rhoq; g.c0 = h->scale;
-    HIPTRY(h, h->xwf(kIST[a], g, rd[a]));
+    g.op = OP_AX_GRAD;
+    HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
+    if (h->mixed) {
+      g.op = OP_LD_GRAD_D; g.src[0] = h->psiq; g.c0 = 1.0;
+      HIPTRY(h, h->xwf(kIST[a], g, gp[a]));
+    } else {
+      HIPTRY(h, h->axpass(F64, a, h->psi, gp[a], g));
+    }
   }
-  // a7 (S:1254-1289): |amp| computed on load, Gaussian applied between the
-  // forward and inverse x passes
+  // a7 (S:1254-1289): the Gaussian exp(-c |q|^2) is a product over the axes,
+  // one pass per axis; |amp| is computed on load by the first
   {
+    const DType dt = h->mixed ? F32 : F64;
     Fuse g = fb;
-    g.op = OP_LD_AMP;
+    g.op = OP_AX_GAUSS_AMP;
+    g.axis = 2;
     g.src[0] = h->psi; g.src[1] = gp[0]; g.src[2] = gp[1]; g.src[3] = gp[2];
-    HIPTRY(h, h->xmid(g, OP_MID_AMP, h->amp));
+    HIPTRY(h, h->axpass(dt, 2, nullptr, h->amp, g));
+    g.op = OP_AX_GAUSS;
+    for (int a = 1; a >= 0; --a) {
+      g.axis = a;
+      HIPTRY(h, h->axpass(dt, a, h->amp, h->amp, g));
+    }
   }
   // a8 (S:1293-1308)
   {
@@ -520,7 +544,7 @@ int run_step(smaiso_handle* h) {
   }
   // a10 (S:1385-1425)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 19 + db1);
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
     K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), h->rho,
               static_cast<T*>(h->amp), static_cast<T*>(h->psi), gp[0], gp[1], gp[2], rd[0],
               rd[1], rd[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
@@ -529,31 +553,34 @@ int run_step(smaiso_handle* h) {
   // a11 (S:1427-1457)
   HIPTRY(h, h->xw(kCT, h->fgrad, h->fgrad));
   HIPTRY(h, h->xw(kCT, h->erho, h->erho));
-  for (int a = 0; a < 3; ++a) {
-    HIPTRY(h, h->xw(kST[a], S[a], S[a]));
-    HIPTRY(h, h->xw(kST[a], f[a], f[a]));
-  }
+  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kST[a], S[a], S[a]));
   // a11 tail + a12 (S:1479-1974)
   {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 11);
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
                 vsolq[0], vsolq[1], vsolq[2], virrq[0], virrq[1], virrq[2], s);
   }
-  // a13 + a14 (S:1979-2117): the divergences are formed on load by the z
-  // pass of their inverse transforms; v = iST(vsol^ + virr^) by linearity
+  // a13 + a14 (S:1979-2117): div v is formed on load by the z pass of its
+  // inverse transform; div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum
+  // of one-axis operators, one accumulating pass per component;
+  // v = iST(vsol^ + virr^) by linearity
   {
     Fuse g = fb;
     g.op = OP_LD_DIV; g.c0 = 1.0;
     g.src[0] = virrq[0]; g.src[1] = virrq[1]; g.src[2] = virrq[2];
     HIPTRY(h, h->xwf(kICT, g, h->divv));
-    g.c0 = h->scale;
-    g.src[0] = f[0]; g.src[1] = f[1]; g.src[2] = f[2];
-    HIPTRY(h, h->xwf(kICT, g, h->divf));
+    Fuse d = fb;
+    d.op = OP_AX_DIV;
+    for (int a = 2; a >= 0; --a) {
+      d.axis = a;
+      d.accumulate = a != 2;
+      HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, f[a], h->divf, d));
+    }
   }
   for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vsolq[a], v[a], virrq[a]));
   // a15 + a16 (S:2121-2162)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
+    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
     K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
                   h->f_rho, static_cast<T*>(h->mu), static_cast<T*>(h->amp),
                   static_cast<T*>(h->psi), static_cast<T*>(h->divf), gp[0], gp[1], gp[2],
@@ -657,7 +684,7 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_GPSIX: case SMAISO_FIELD_GPSIY: case SMAISO_FIELD_GPSIZ:
       *out = as64(h->gp[fid - SMAISO_FIELD_GPSIX]); return SMAISO_OK;
     case SMAISO_FIELD_GRHOX: case SMAISO_FIELD_GRHOY: case SMAISO_FIELD_GRHOZ:
-      *out = as64(h->rd[fid - SMAISO_FIELD_GRHOX]); return SMAISO_OK;
+      *out = h->rd[fid - SMAISO_FIELD_GRHOX]; return SMAISO_OK;
     case SMAISO_FIELD_LAPRHOL: *out = as64(h->laprl); return SMAISO_OK;
     case SMAISO_FIELD_PSI_OLD: *out = as64(h->psi_prev); return SMAISO_OK;
     default: return fail(h, SMAISO_E_INVALID, fmt("unknown field id %d", fid));
@@ -1344,7 +1371,7 @@ AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
   A.divv = tp(h->divv);
   for (int a = 0; a < 3; ++a) {
     A.gp[a] = tp(h->gp[a]);
-    A.rd[a] = tp(h->rd[a]);
+    A.rd[a] = h->rd[a];
     A.v[a] = tp(h->v[a]);
     A.vsolq[a] = tp(h->vsolq[a]);
     A.virrq[a] = tp(h->virrq[a]);

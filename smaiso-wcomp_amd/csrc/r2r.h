@@ -34,7 +34,15 @@ enum FuseOp : int {
   OP_MID_AMP = 8,      // mode *= scale exp(-1.57*1.57*|q|^2*2)
   OP_MID_NEGLAP = 9,   // mode *= -scale |q|^2
   OP_LD_PSI = 10,      // x = new psiq from psiq, psiq_new (= old), Nl^ (src[0]), Nl^_old; stored to psiq_new
-  OP_COUNT = 11,
+  // One-axis operators: forward 1-D transform along the pass axis, an operator
+  // on the modes, inverse 1-D transform, in one pass; c = c0/(2N).  They are
+  // the 3-D spectral operators of the reference restricted to the one axis
+  // they act on (iDCT.DCT = 2N along every other axis).
+  OP_AX_GRAD = 11,       // out = iDST(y), y[n] = -c Vs[n] DCT(in)[n+1] (0 at n = N-1)
+  OP_AX_DIV = 12,        // out (+)= iDCT(y), y[n] = c Vq[n] DST(in)[n-1] (0 at n = 0)
+  OP_AX_GAUSS = 13,      // out = iDCT(c exp(-1.57^2 2 Vq[n]^2) DCT(in))
+  OP_AX_GAUSS_AMP = 14,  // OP_AX_GAUSS (z only) of the amplitude computed on load (OP_LD_AMP)
+  OP_COUNT = 15,
 };
 
 struct Fuse {
@@ -50,6 +58,7 @@ struct Fuse {
   double* psiq_new = nullptr;     // holds psiq_old on entry, overwritten in place
   const void* nlq_old = nullptr;
   int first = 0;                  // first step: Nl_old = Nl
+  int accumulate = 0;             // OP_AX_DIV: add the result to out
 };
 
 struct PassIO {

@@ -55,24 +55,6 @@ hipError_t R2R::tables(DType dt, int n, Tables** out) {
   return hipSuccess;
 }
 
-// Tile and block size per (type, length, axis).
-template <typename T, int N>
-struct TileChoice {
-  static constexpr int cap(int p) { return p < 1 ? 1 : p; }
-  static constexpr int budget(int b) {
-    int p = b / N, q = 1;
-    while (q * 2 <= p) q *= 2;
-    return q;
-  }
-  // measured at 256^3 on MI355X: one wavefront per 2 line pairs along z;
-  // along y and x 8 pairs x 256 threads (FP64) and 16 pairs x 512 threads (FP32)
-  static constexpr int PZ = cap(budget(512) < 2 ? budget(512) : 2);
-  static constexpr int PS0 = sizeof(T) == 8 ? 8 : 16;
-  static constexpr int PS = cap(budget(2048 * (8 / (int)sizeof(T))) < PS0 ? budget(2048 * (8 / (int)sizeof(T))) : PS0);
-  static constexpr int NTZ = 64;
-  static constexpr int NTS = sizeof(T) == 8 ? 256 : 512;
-};
-
 template <typename T, int N>
 static hipError_t launch_fft(const PassIO& io, void* tw, void* mk, int kind, int axis,
                              const Geom& g, hipStream_t s) {
@@ -107,7 +89,7 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
   d.p = make_args<T>(io, nullptr, nullptr, kind, axis, g);
   d.costab = static_cast<const T*>(costab);
   d.n = n;
-  const bool mid = io.f.op == OP_MID_AMP || io.f.op == OP_MID_NEGLAP;
+  const bool mid = io.f.op == OP_MID_AMP || io.f.op == OP_MID_NEGLAP || is_axis_op(io.f.op);
   int tl = 16;
   while (tl > 1 && (size_t)tl * n * sizeof(T) * (mid ? 2 : 1) > 48 * 1024) tl /= 2;
   d.tl = tl;
@@ -122,6 +104,7 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
 template <typename T>
 static hipError_t dispatch(int n, const PassIO& io, void* tw, void* mk, void* ct, int kind,
                            int axis, const Geom& g, hipStream_t s) {
+  if (is_axis_op(io.f.op) && R2R::fft_supported(n)) return launch_axis_op<T>(n, io, tw, mk, axis, g, s);
   switch (n) {
     case 16: return launch_fft<T, 16>(io, tw, mk, kind, axis, g, s);
     case 32: return launch_fft<T, 32>(io, tw, mk, kind, axis, g, s);
@@ -141,6 +124,8 @@ static hipError_t dispatch(int n, const PassIO& io, void* tw, void* mk, void* ct
 static bool op_valid(int op, int axis) {
   if (op < 0 || op >= OP_COUNT) return false;
   if (op == OP_PLAIN) return true;
+  if (op == OP_AX_GAUSS_AMP) return axis == 2;
+  if (is_axis_op(op)) return true;
   if (op == OP_MID_AMP || op == OP_MID_NEGLAP) return axis == 0;
   return axis == 2;  // load ops act in the first (z) pass
 }

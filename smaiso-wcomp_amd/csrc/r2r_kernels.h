@@ -398,6 +398,7 @@ __device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo
     case OP_LD_PROD: return load_elem<T, OP_LD_PROD>(a, i, j, m, idx);
     case OP_LD_DIV: return load_elem<T, OP_LD_DIV>(a, i, j, m, idx);
     case OP_LD_PSI: return load_elem<T, OP_LD_PSI>(a, i, j, m, idx);
+    case OP_AX_GAUSS_AMP: return load_elem<T, OP_LD_AMP>(a, i, j, m, idx);
     default: return load_elem<T, OP_PLAIN>(a, i, j, m, idx);
   }
 }
@@ -432,14 +433,40 @@ __device__ __forceinline__ cplx<T> load_pair(const PassArgs<T>& a, const TileGeo
 
 template <typename T>
 __device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m, T y0,
-                                           T y1) {
+                                           T y1, bool acc = false) {
   T* o = a.out + g.base + l0 + (long long)m * g.stride_m;
   if (a.vec && l0 + 1 < g.nvalid) {
+    if (acc) {
+      const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
+      y0 += old.x;
+      y1 += old.y;
+    }
     *reinterpret_cast<cplx<T>*>(o) = cplx<T>{y0, y1};
   } else {
-    if (l0 < g.nvalid) o[0] = y0;
-    if (l0 + 1 < g.nvalid) o[1] = y1;
+    if (l0 < g.nvalid) o[0] = acc ? o[0] + y0 : y0;
+    if (l0 + 1 < g.nvalid) o[1] = acc ? o[1] + y1 : y1;
   }
+}
+
+// Mode m of the one-axis operators' output from the forward outputs y of one
+// line (a stride-1 view of the sequence): see FuseOp.  For both lines of a
+// pair, y is the complex packing (line 0, line 1).
+template <typename T, int OP>
+__device__ __forceinline__ double ax_weight(const Fuse& f, int m, int n) {
+  const double c = f.c0 / (2.0 * n);
+  if constexpr (OP == OP_AX_GRAD) return -c * f.vs[f.axis][m];
+  else if constexpr (OP == OP_AX_DIV) return c * f.vq[f.axis][m];
+  else {
+    const double q = f.vq[f.axis][m];
+    return c * exp(-1.57 * 1.57 * q * q * 2);
+  }
+}
+// source mode of output mode m, or -1 for a zero output
+template <int OP>
+__device__ __forceinline__ int ax_source(int m, int n) {
+  if constexpr (OP == OP_AX_GRAD) return m <= n - 2 ? m + 1 : -1;
+  else if constexpr (OP == OP_AX_DIV) return m >= 1 ? m - 1 : -1;
+  else return m;
 }
 
 // Makhoul post-processing of item k of one sequence: Z at k and kb (kb = N-k,
@@ -496,6 +523,10 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   constexpr int SEQ = N + 1;
   constexpr int HN = N / 2;
   constexpr bool MID = (OP == OP_MID_AMP || OP == OP_MID_NEGLAP);
+  constexpr bool AX = (OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GAUSS_AMP);
+  // load op of the first (z) pass
+  constexpr int LD_OP = OP == OP_AX_GAUSS_AMP ? OP_LD_AMP : ((MID || AX) ? OP_PLAIN : OP);
+  constexpr int AXOP = OP == OP_AX_GAUSS_AMP ? OP_AX_GAUSS : OP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   T* rbuf = reinterpret_cast<T*>(smem_raw);
@@ -503,8 +534,11 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
   const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
   g.base += mo;
-  const bool fwd = MID || (a.kind == REDFT10 || a.kind == RODFT10);
-  const bool sine = !MID && (a.kind == RODFT10 || a.kind == RODFT01);
+  const bool fwd = MID || AX || (a.kind == REDFT10 || a.kind == RODFT10);
+  // sine: DST on the load side (forward DST-II, or inverse DST-III input);
+  // sine_out: DST on the store side
+  const bool sine = AX ? (OP == OP_AX_DIV) : (!MID && (a.kind == RODFT10 || a.kind == RODFT01));
+  const bool sine_out = AX ? (OP == OP_AX_GRAD) : sine;
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
@@ -522,7 +556,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           int i, j;
           const long long gl = (long long)g.outer + l;
           line_ij(gl, a.ny, i, j);
-          v[it] = load_elem<T, MID ? OP_PLAIN : OP>(a, i, j, m, mo + gl * N + m);
+          v[it] = load_elem<T, LD_OP>(a, i, j, m, mo + gl * N + m);
         }
       }
 #pragma unroll
@@ -643,6 +677,79 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
         }
       }
       __syncthreads();
+    } else if constexpr (AX) {
+      // forward outputs of both lines -> registers -> LDS at their output
+      // index; the operator reads shifted modes, so each step is fenced
+      cplx<T> ra[ITW], rb[ITW];
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          T y0a, y1a, y0b, y1b;
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          ra[it] = cplx<T>{y0a, y1a};
+          rb[it] = cplx<T>{y0b, y1b};
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          buf[c * SEQ + (sine ? N - 1 - k : k)] = ra[it];
+          buf[c * SEQ + (sine ? N - 1 - kb : kb)] = rb[it];
+        }
+      }
+      __syncthreads();
+      // operator, then the inverse's pre-twiddle (DST-III reads its input
+      // reversed)
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          const int pa = sine_out ? N - 1 - k : k;
+          const int pb = sine_out ? N - 1 - kb : kb;
+          cplx<T> xk = {T(0), T(0)}, xb = {T(0), T(0)};
+          const int sa = ax_source<AXOP>(pa, N), sb = ax_source<AXOP>(pb, N);
+          if (sa >= 0) {
+            const double wgt = ax_weight<T, AXOP>(a.f, pa, N);
+            const cplx<T> y = buf[c * SEQ + sa];
+            xk = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
+          }
+          if (sb >= 0) {
+            const double wgt = ax_weight<T, AXOP>(a.f, pb, N);
+            const cplx<T> y = buf[c * SEQ + sb];
+            xb = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
+          }
+          makhoul_pre<T, N>(a.mk, k, xk, xb, ra[it], rb[it]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < ITW; ++it) {
+        const int w = tid + it * NT;
+        if (W % NT == 0 || w < W) {
+          int c, k;
+          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          else { k = w / P; c = w - k * P; }
+          const int kb = (k == 0) ? HN : N - k;
+          buf[c * SEQ + k] = ra[it];
+          buf[c * SEQ + kb] = rb[it];
+        }
+      }
+      __syncthreads();
     } else {
       // ---- post: item k < N/2 writes outputs k and N-k (item 0: 0 and N/2)
 #pragma unroll
@@ -683,7 +790,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   fft_stage<T, N, P, NT, MAXR, +1, 0>(buf, a.tw);
   constexpr int WO = P * HN;
   constexpr int ITO = (WO + NT - 1) / NT;
-  const T s = sine ? T(-1) : T(1);
+  const T s = sine_out ? T(-1) : T(1);
+  const bool acc = AX && a.f.accumulate;
 #pragma unroll
   for (int it = 0; it < ITO; ++it) {
     const int w = tid + it * NT;
@@ -695,19 +803,29 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
       const cplx<T> ub = buf[c * SEQ + N - 1 - n];
       const int l0 = 2 * c;
       if (a.axis == 2) {
-        if (l0 < g.nvalid) {
-          T* o = a.out + g.base + (long long)l0 * N + 2 * n;
-          if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.x, s * ub.x};
-          else { o[0] = ua.x; o[1] = s * ub.x; }
-        }
-        if (l0 + 1 < g.nvalid) {
-          T* o = a.out + g.base + (long long)(l0 + 1) * N + 2 * n;
-          if (a.vec) *reinterpret_cast<cplx<T>*>(o) = cplx<T>{ua.y, s * ub.y};
-          else { o[0] = ua.y; o[1] = s * ub.y; }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (l0 + q < g.nvalid) {
+            T* o = a.out + g.base + (long long)(l0 + q) * N + 2 * n;
+            T v0 = q == 0 ? ua.x : ua.y;
+            T v1 = s * (q == 0 ? ub.x : ub.y);
+            if (a.vec) {
+              if (acc) {
+                const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
+                v0 += old.x;
+                v1 += old.y;
+              }
+              *reinterpret_cast<cplx<T>*>(o) = cplx<T>{v0, v1};
+            } else {
+              if (acc) { v0 += o[0]; v1 += o[1]; }
+              o[0] = v0;
+              o[1] = v1;
+            }
+          }
         }
       } else {
-        store_pair(a, g, l0, 2 * n, ua.x, ua.y);
-        store_pair(a, g, l0, 2 * n + 1, s * ub.x, s * ub.y);
+        store_pair(a, g, l0, 2 * n, ua.x, ua.y, acc);
+        store_pair(a, g, l0, 2 * n + 1, s * ub.x, s * ub.y, acc);
       }
     }
   }
@@ -761,6 +879,8 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
   T* x = reinterpret_cast<T*>(smem_raw);
   T* y = x + tl * n;  // second buffer of the mid ops
   const bool mid = a.f.op == OP_MID_AMP || a.f.op == OP_MID_NEGLAP;
+  const int op = a.f.op;
+  const bool ax = op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP;
   TileGeo g;
   if (a.axis == 2) {
     long long first = (long long)blockIdx.x * tl;
@@ -802,7 +922,29 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     x[l * n + m] = v;
   }
   __syncthreads();
-  if (mid) {
+  if (ax) {
+    // forward 1-D transform into y, the operator back into x
+    const int fk = op == OP_AX_DIV ? RODFT10 : REDFT10;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+      const int k = e / tl, l = e - (e / tl) * tl;
+      if (l < g.nvalid) y[l * n + k] = direct_sum<T>(fk, x + l * n, n, k, d.costab);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+      const int m = e / tl, l = e - (e / tl) * tl;
+      if (l >= g.nvalid) continue;
+      T v = T(0);
+      if (op == OP_AX_GRAD) {
+        if (ax_source<OP_AX_GRAD>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_GRAD>(a.f, m, n) * (double)y[l * n + m + 1]);
+      } else if (op == OP_AX_DIV) {
+        if (ax_source<OP_AX_DIV>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_DIV>(a.f, m, n) * (double)y[l * n + m - 1]);
+      } else {
+        v = (T)(ax_weight<T, OP_AX_GAUSS>(a.f, m, n) * (double)y[l * n + m]);
+      }
+      x[l * n + m] = v;
+    }
+    __syncthreads();
+  } else if (mid) {
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
       const int k = e / tl, l = e - (e / tl) * tl;
       if (l >= g.nvalid) continue;
@@ -818,10 +960,12 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     if (a.axis == 2) { l = e / n; k = e - l * n; }
     else { k = e / tl; l = e - k * tl; }
     if (l >= g.nvalid) continue;
-    const T r = mid ? direct_sum<T>(REDFT01, y + l * n, n, k, d.costab)
-                    : direct_sum<T>(a.kind, x + l * n, n, k, d.costab);
+    T r;
+    if (ax) r = direct_sum<T>(op == OP_AX_GRAD ? RODFT01 : REDFT01, x + l * n, n, k, d.costab);
+    else if (mid) r = direct_sum<T>(REDFT01, y + l * n, n, k, d.costab);
+    else r = direct_sum<T>(a.kind, x + l * n, n, k, d.costab);
     const long long oi = g.base + l * g.line_step + k * g.stride_m;
-    a.out[oi] = r;
+    a.out[oi] = (ax && a.f.accumulate) ? a.out[oi] + r : r;
   }
 }
 
@@ -857,6 +1001,34 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
   return hipGetLastError();
 }
 
+
+// Tile and block size per (type, length, axis).
+template <typename T, int N>
+struct TileChoice {
+  static constexpr int cap(int p) { return p < 1 ? 1 : p; }
+  static constexpr int budget(int b) {
+    int p = b / N, q = 1;
+    while (q * 2 <= p) q *= 2;
+    return q;
+  }
+  // measured at 256^3 on MI355X: one wavefront per 2 line pairs along z;
+  // along y and x 8 pairs x 256 threads (FP64) and 16 pairs x 512 threads (FP32)
+  static constexpr int PZ = cap(budget(512) < 2 ? budget(512) : 2);
+  static constexpr int PS0 = sizeof(T) == 8 ? 8 : 16;
+  static constexpr int PS = cap(budget(2048 * (8 / (int)sizeof(T))) < PS0 ? budget(2048 * (8 / (int)sizeof(T))) : PS0);
+  static constexpr int NTZ = 64;
+  static constexpr int NTS = sizeof(T) == 8 ? 256 : 512;
+};
+
+inline bool is_axis_op(int op) {
+  return op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP;
+}
+
+// One-axis operator pass (OP_AX_*) of FFT length n with the default tiles;
+// defined in r2r_axis_ops.hip.
+template <typename T>
+hipError_t launch_axis_op(int n, const PassIO& io, void* tw, void* mk, int axis, const Geom& g,
+                          hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // tile variants that R2R::autotune times for plain passes (variant 0 is the

@@ -42,7 +42,7 @@ struct StepKernels {
   // a10 (FP64 arithmetic)
   static void forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                      const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
-                     const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu, T* sx,
+                     const double* rdx, const double* rdy, const double* rdz, T* fgrad, T* erho, T* mu, T* sx,
                      T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s);
   // a11 (fgrad += G erho) + a12
   static void velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
@@ -50,7 +50,7 @@ struct StepKernels {
                        T* virry, T* virrz, hipStream_t s);
   // a15 + a16 (FP64 arithmetic)
   static void density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
-                         const T* rdx, const T* rdy, const T* rdz, const T* divv, double* rho,
+                         const double* rdx, const double* rdy, const double* rdz, const T* divv, double* rho,
                          double* f_rho, const T* mu, const T* amp, const T* psi,
                          const T* divf, const T* gpx, const T* gpy, const T* gpz, T* nl,
                          hipStream_t s);
