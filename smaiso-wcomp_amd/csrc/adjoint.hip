@@ -145,7 +145,6 @@ __global__ __launch_bounds__(kB) void k_adj_vel_a(StepConsts c, AdjArgs<T> A) {
     const MemberConsts m = c.mc[0];
     const double nu = m.nu, ilam = m.lam + 2 * m.nu;
     const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
-    const double gfac = -1.57 * 1.57 * 2 * c.scale;
     const double qv[3] = {c.vq[0][p.i], c.vq[1][p.j], c.vq[2][p.k]};
     const double wv[3] = {c.vs[0][p.i], c.vs[1][p.j], c.vs[2][p.k]};
     const int nn[3] = {c.nx, c.ny, c.nz};
@@ -172,12 +171,15 @@ __global__ __launch_bounds__(kB) void k_adj_vel_a(StepConsts c, AdjArgs<T> A) {
 #pragma unroll
       for (int b = 0; b < 3; ++b) mq2 += sqd(b == a ? wv[b] : qv[b]);
       const double cm1 = c.scale / mq2, cm2 = wv[a] / mq2;
-      double cme = cm1;
-      if (a == 2) {  // quirk Q3 (S:1932)
+      // exp(gfac/CM1_a) as a product of the one-axis Gaussian tables
+      const double gx = a == 0 ? c.gs[0][p.i] : c.gq[0][p.i];
+      double gy = a == 1 ? c.gs[1][p.j] : c.gq[1][p.j];
+      double gz = a == 2 ? c.gs[2][p.k] : c.gq[2][p.k];
+      if (a == 2) {  // quirk Q3 (S:1932): CM1y in the exponent
         const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
-        if (p.k == 0 && p.j > 0 && (p.i % slab) != 0) cme = c.scale / (sqd(qv[0]) + sqd(wv[1]) + sqd(qv[2]));
+        if (p.k == 0 && p.j > 0 && (p.i % slab) != 0) { gy = c.gs[1][p.j]; gz = c.gq[2][p.k]; }
       }
-      const double dot = -(cm1 / nu) * cm2 * vs + (cm1 / ilam) * cm2 * exp(gfac / cme) * vi;
+      const double dot = -(cm1 / nu) * cm2 * vs + (cm1 / ilam) * cm2 * (gx * gy * gz) * vi;
       A.dotb[a][idx] = dot;
       A.cfgb[a][idx] = (cm1 / ilam) * wv[a] * vi;
       A.dirb[a][idx] = (cm1 / nu) * vs + wv[a] * dot;
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(kB) void k_adj_vel_b(StepConsts c, AdjArgs<T> A) {
   // fgq^ = CT(fgrad) + G CT(erho), S:1479-1488
   const double w = wc_at(i, j, k) * bfg;
   A.wfg[idx] = w;
-  A.wGfg[idx] = w * exp(-1.57 * 1.57 * mq2c_at(c, i, j, k) * 2);
+  A.wGfg[idx] = w * (c.gq[0][i] * c.gq[1][j] * c.gq[2][k]);
 }
 
 // ---------------------------------------------------------------------------

@@ -80,6 +80,8 @@ struct smaiso_handle {
   std::vector<void*> allocs;
   double* vq[3] = {nullptr, nullptr, nullptr};
   double* vs[3] = {nullptr, nullptr, nullptr};
+  double* gq[3] = {nullptr, nullptr, nullptr};  // exp(-1.57^2 2 vq^2) per axis
+  double* gs[3] = {nullptr, nullptr, nullptr};  // exp(-1.57^2 2 vs^2) per axis
   // FP64 state
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
   double* rd[3] = {nullptr, nullptr, nullptr};  // grad rho, FP64 in both precisions (rho is
@@ -133,7 +135,10 @@ struct smaiso_handle {
     c.scale = scale;
     c.dt = p.dt; c.alpha = p.alpha; c.beta = p.beta; c.gamma = p.gamma; c.eps = p.epsilon;
     c.q0 = p.q0; c.q02 = p.q0 * p.q0; c.rho_m = rho_m;
-    for (int a = 0; a < 3; ++a) { c.vq[a] = vq[a]; c.vs[a] = vs[a]; }
+    for (int a = 0; a < 3; ++a) {
+      c.vq[a] = vq[a]; c.vs[a] = vs[a];
+      c.gq[a] = gq[a]; c.gs[a] = gs[a];
+    }
     c.mc = mc;
     return c;
   }
@@ -421,10 +426,12 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   const int nn[3] = {p.nx, p.ny, p.nz};
   const double dd[3] = {p.dx, p.dy, p.dz};
   for (int a = 0; a < 3; ++a) {
-    std::vector<double> q(nn[a]), w(nn[a]);
+    std::vector<double> q(nn[a]), w(nn[a]), eq(nn[a]), ew(nn[a]);
     for (int i = 0; i < nn[a]; ++i) {
       q[i] = M_PI * i / (dd[a] * nn[a]);
       w[i] = M_PI * (i + 1) / (dd[a] * nn[a]);
+      eq[i] = std::exp(-1.57 * 1.57 * 2 * (q[i] * q[i]));
+      ew[i] = std::exp(-1.57 * 1.57 * 2 * (w[i] * w[i]));
     }
     if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
     h->vq[a] = static_cast<double*>(tmpp);
@@ -433,6 +440,14 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
     if ((e = hipMemcpy(h->vq[a], q.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
       return hipfail(e, "hipMemcpy");
     if ((e = hipMemcpy(h->vs[a], w.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return hipfail(e, "hipMemcpy");
+    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+    h->gq[a] = static_cast<double*>(tmpp);
+    if ((e = alloc(nn[a] * 8, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");
+    h->gs[a] = static_cast<double*>(tmpp);
+    if ((e = hipMemcpy(h->gq[a], eq.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return hipfail(e, "hipMemcpy");
+    if ((e = hipMemcpy(h->gs[a], ew.data(), nn[a] * 8, hipMemcpyHostToDevice)) != hipSuccess)
       return hipfail(e, "hipMemcpy");
   }
   if ((e = alloc(sizeof(MemberConsts) * n, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc");

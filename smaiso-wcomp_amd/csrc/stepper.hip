@@ -140,14 +140,23 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
   const double wx = c.vs[0][p.i], wy = c.vs[1][p.j], wz = c.vs[2][p.k];
   const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
-  const double inu = mm.nu, ilam = mm.lam + 2 * mm.nu;
-  const double gfac = -1.57 * 1.57 * 2 * c.scale;
+  const double rnu = 1.0 / mm.nu, rlam = 1.0 / (mm.lam + 2 * mm.nu);
   const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
-
-  // fgrad^ + erho^ G at a (shifted) mode index, S:1479-1488
-  auto fg_at = [&](unsigned q, int ii, int jj, int kk) -> double {
-    const double m = sq(c.vq[0][ii]) + sq(c.vq[1][jj]) + sq(c.vq[2][kk]);
-    return (double)fgq[q] + (double)erq[q] * exp(-1.57 * 1.57 * m * 2);
+  // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q_a|^2) on component
+  // a's lattice, and G at the shifted fgrad^ index.  FP32-mixed fields take
+  // them as products of the one-axis tables (the kernel is ALU-bound there);
+  // FP64 fields evaluate exp (the table loads measured slower there)
+  constexpr bool TAB = sizeof(T) == 4;
+  auto g3 = [&](int a, int i, int j, int k, bool si, bool sj, bool sk) -> double {
+    (void)a;
+    if constexpr (TAB) {
+      return (si ? c.gs[0][i] : c.gq[0][i]) * (sj ? c.gs[1][j] : c.gq[1][j]) * (sk ? c.gs[2][k] : c.gq[2][k]);
+    } else {
+      const double x = si ? c.vs[0][i] : c.vq[0][i];
+      const double y = sj ? c.vs[1][j] : c.vq[1][j];
+      const double z = sk ? c.vs[2][k] : c.vq[2][k];
+      return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
+    }
   };
 
   // x component, i <= Nx-2 (quirk Q2), S:1583-1709
@@ -156,10 +165,11 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     const double Syp = (p.j > 0) ? (double)syq[idx + sI - sJ] : 0.0;
     const double Szp = (p.k > 0) ? (double)szq[idx + sI - 1] : 0.0;
     const double dot = wx * Sx + qy * Syp + qz * Szp;
-    const double mq2 = wx * wx + qy2 + qz2;
-    const double cm1 = c.scale / mq2, cm2 = wx / mq2;
-    ox = (cm1 / inu) * (Sx - cm2 * dot);
-    rx = (cm1 / ilam) * (wx * fg_at(idx + sI, p.i + 1, p.j, p.k) + cm2 * dot * exp(gfac / cm1));
+    const double inv = 1.0 / (wx * wx + qy2 + qz2);
+    const double cm1 = c.scale * inv, cm2 = wx * inv;
+    const double fg = (double)fgq[idx + sI] + (double)erq[idx + sI] * g3(0, p.i + 1, p.j, p.k, false, false, false);
+    ox = (cm1 * rnu) * (Sx - cm2 * dot);
+    rx = (cm1 * rlam) * (wx * fg + cm2 * dot * g3(0, p.i, p.j, p.k, true, false, false));
   }
   double oy = 0, ry = 0, oz = 0, rz = 0;
   if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
@@ -168,25 +178,27 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
       const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + sJ] : 0.0;
       const double Szp = (p.k > 0) ? (double)szq[idx + sJ - 1] : 0.0;
       const double dot = qx * Sxp + wy * Sy + qz * Szp;
-      const double mq2 = qx2 + wy * wy + qz2;
-      const double cm1 = c.scale / mq2, cm2 = wy / mq2;
-      oy = (cm1 / inu) * (Sy - cm2 * dot);
-      ry = (cm1 / ilam) * (wy * fg_at(idx + sJ, p.i, p.j + 1, p.k) + cm2 * dot * exp(gfac / cm1));
+      const double inv = 1.0 / (qx2 + wy * wy + qz2);
+      const double cm1 = c.scale * inv, cm2 = wy * inv;
+      const double fg = (double)fgq[idx + sJ] + (double)erq[idx + sJ] * g3(1, p.i, p.j + 1, p.k, false, false, false);
+      oy = (cm1 * rnu) * (Sy - cm2 * dot);
+      ry = (cm1 * rlam) * (wy * fg + cm2 * dot * g3(1, p.i, p.j, p.k, false, true, false));
     }
     // z component
     {
       const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + 1] : 0.0;
       const double Syp = (p.j > 0) ? (double)syq[idx - sJ + 1] : 0.0;
       const double dot = qx * Sxp + qy * Syp + wz * Sz;
-      const double mq2 = qx2 + qy2 + wz * wz;
-      const double cm1 = c.scale / mq2, cm2 = wz / mq2;
-      oz = (cm1 / inu) * (Sz - cm2 * dot);
+      const double inv = 1.0 / (qx2 + qy2 + wz * wz);
+      const double cm1 = c.scale * inv, cm2 = wz * inv;
+      const double fg = (double)fgq[idx + 1] + (double)erq[idx + 1] * g3(2, p.i, p.j, p.k + 1, false, false, false);
+      oz = (cm1 * rnu) * (Sz - cm2 * dot);
       // quirk Q3 (S:1932): CM1y in the exponent at k=0, j>=1, except the
       // first plane of each reference slab
-      double cme = cm1;
       const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
-      if (p.k == 0 && p.j > 0 && (p.i % slab) != 0) cme = c.scale / (qx2 + wy * wy + qz2);
-      rz = (cm1 / ilam) * (wz * fg_at(idx + 1, p.i, p.j, p.k + 1) + cm2 * dot * exp(gfac / cme));
+      const bool q3 = p.k == 0 && p.j > 0 && (p.i % slab) != 0;
+      const double e = q3 ? g3(2, p.i, p.j, p.k, false, true, false) : g3(2, p.i, p.j, p.k, false, false, true);
+      rz = (cm1 * rlam) * (wz * fg + cm2 * dot * e);
     }
   }
   const long long o = mo + idx;
