@@ -53,9 +53,14 @@ def make_params(S, n, precision):
                     precision=precision)
 
 
-def time_stepper(S, params, psi, rho, steps, warmup, cadence=50, graph=True):
+def time_stepper(S, params, psi, rho, steps, warmup, cadence=50, graph=True, tune=True):
     st = S.Stepper(params)
     st.set_graph(graph)
+    tuned = None
+    if tune:
+        ms, chosen = st.autotune()
+        tuned = dict(chosen=chosen.tolist(), ms=[[[round(float(v), 4) for v in ax] for ax in dt] for dt in ms])
+        log(f"[bench] autotuned tile variants (fp64, fp32) x (x, y, z): {chosen.tolist()}")
     st.set_state(psi, rho)
     st.step(warmup)
     S.device_synchronize()
@@ -70,6 +75,7 @@ def time_stepper(S, params, psi, rho, steps, warmup, cadence=50, graph=True):
             diag = st.diagnostics(allow_nonfinite=True)
     S.device_synchronize()
     t1 = time.perf_counter()
+    st.tuned = tuned
     return st, (t1 - t0), diag
 
 
@@ -250,6 +256,7 @@ def main():
     ap.add_argument("--accuracy-steps", type=str, default="100,500,2000",
                     help="checkpoints of the FP32-mixed vs FP64 comparison ('' to skip)")
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default r2r tiles")
     ap.add_argument("--ensemble", type=int, default=48, help="members of the ensemble leg (0 to skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=500, help="0 to skip the adjoint leg")
@@ -270,7 +277,9 @@ def main():
     psi, rho = S.make_interface_ic(p64, ic)
 
     log(f"[bench] FP64: warmup {args.warmup}, timed {args.steps}")
-    st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup, graph=not args.no_graph)
+    st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup, graph=not args.no_graph,
+                                       tune=not args.no_autotune)
+    tuned64 = st64.tuned
     sps64 = args.steps / sec64
     log(f"[bench] FP64 {sps64:.2f} steps/s, diag {diag64}")
     kt64 = kernel_timing(st64, args.profile_steps)
@@ -280,7 +289,8 @@ def main():
     if not args.no_mixed:
         pm = make_params(S, n, S.FP32_MIXED)
         log(f"[bench] FP32-mixed: warmup {args.warmup}, timed {args.steps}")
-        stm, secm, diagm = time_stepper(S, pm, psi, rho, args.steps, args.warmup, graph=not args.no_graph)
+        stm, secm, diagm = time_stepper(S, pm, psi, rho, args.steps, args.warmup, graph=not args.no_graph,
+                                        tune=not args.no_autotune)
         spsm = args.steps / secm
         ktm = kernel_timing(stm, args.profile_steps)
         del stm
@@ -355,7 +365,7 @@ def main():
         "config": {"workload": f"plain FP64 stepper, {n}^3 grid, interface h=Lz/2+0.5cos(2pi x/Lx)cos(2pi y/Ly), "
                                f"rho 1.0/0.8, dt=0.01, rho_m=1, diagnostics every 50 steps",
                    "grid": [n, n, n], "precision": "fp64", "dt": 0.01, "rho_m": 1.0,
-                   "hipgraph": not args.no_graph},
+                   "hipgraph": not args.no_graph, "autotuned_tiles": tuned64},
         "roofline": roofline,
         "step_roofline": step_roof,
         "kernels_fp64": {k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
