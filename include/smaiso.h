@@ -271,6 +271,30 @@ int smaiso_autotune(smaiso_handle* h, float* ms, int32_t* chosen);
 int smaiso_r2r3d_tiles(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz, const void* in_dev,
                        void* out_dev, int32_t dtype, int32_t variant, void* stream);
 
+/* Data of the reference's output writers (S:2273-2446).  smaiso_get_slice:
+ * the x-z slice at y index j of a field (saveMid, S:94-129, with j = Ny/2):
+ * out[i*Nz + k] = field[(i*Ny + j)*Nz + k], Nx*Nz values.  smaiso_surface_height:
+ * the surface tracker (S:2278-2318): surf[i*Ny + j] = the largest k with
+ * rho > threshold (the reference uses rho_s - 0.15 with rho_s = kp Amp + rho0
+ * = 1); a column with no such k keeps the value surf held on entry (the
+ * reference's stale entry).  Both synchronous. */
+int smaiso_get_slice(smaiso_handle* h, int32_t field, int32_t j, double* out);
+int smaiso_surface_height(smaiso_handle* h, double threshold, double* surf);
+
+/* Full-state checkpoint of a single handle (fixes the reference's psi-only
+ * restart, S:727-796 and S:2334-2345): psiq, psiq_old, rho, f_rho, psi, the
+ * last step's N^l and nLoop, binary.  Loading needs the same grid and
+ * precision; stepping on after a load is bit-identical to never stopping. */
+int smaiso_save_checkpoint(smaiso_handle* h, const char* path);
+int smaiso_load_checkpoint(smaiso_handle* h, const char* path);
+
+/* Reader of the reference's restart files psi<rank>.dat (S:727-748: one value
+ * per line, each rank's x-slab in rank order): reads whitespace-separated
+ * values from paths[0], paths[1], ... until `count` (= Nx*Ny*Nz) values are in
+ * psi.  SMAISO_E_INVALID if a file is missing or the files hold fewer values.
+ * Host only. */
+int smaiso_read_psi_text(const char* const* paths, int32_t npaths, int64_t count, double* psi);
+
 /*
  * Ensembles: `count` independent simulations (each one the reference's whole
  * program, S:1081-2263, with its own parameters and grid) stepped together on

@@ -564,3 +564,28 @@ def velocity_solve(p: Params, t: Tables, Sxq, Syq, Szq, fgq):
     vsolz[:, -1, :] = 0.0; vsolz[:, :, -1] = 0.0
     virrz[:, -1, :] = 0.0; virrz[:, :, -1] = 0.0
     return vsolx, virrx, vsoly, virry, vsolz, virrz
+
+
+# ---------------------------------------------------------------------------
+# data of the output writers (S:94-129, S:2273-2446)
+# ---------------------------------------------------------------------------
+def mid_slice(field: np.ndarray, j: int) -> np.ndarray:
+    """saveMid (S:94-129): the x-z slice at y index j, i-major then k."""
+    return np.ascontiguousarray(np.asarray(field)[:, j, :]).reshape(-1)
+
+
+def surface_track(rho: np.ndarray, threshold: float, prev: np.ndarray) -> np.ndarray:
+    """Surface tracker (S:2278-2318): for each column (i, j) the largest k with
+    rho > threshold, scanning down from k = Nz-1; a column without one keeps
+    its previous value (the reference leaves surfZ_local untouched, quirk Q9).
+    Returns the Nx*Ny values in (i, j) order."""
+    rho = np.asarray(rho)
+    nx, ny, nz = rho.shape
+    out = np.array(prev, dtype=np.float64).reshape(nx * ny).copy()
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz - 1, -1, -1):
+                if rho[i, j, k] > threshold:
+                    out[i * ny + j] = k
+                    break
+    return out

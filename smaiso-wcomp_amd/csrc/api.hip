@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -102,6 +103,8 @@ struct smaiso_handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_parity = -1;
+
+  int* surf_dev = nullptr;  // surface tracker output (nx*ny), allocated on first use
 
   // adjoint work fields (FP64, allocated by the first smaiso_adjoint call)
   std::vector<double*> adj;
@@ -1575,6 +1578,141 @@ int smaiso_interface_ic_vjp(const smaiso_params* p, const smaiso_ic* ic, const d
       dheight[(size_t)i * ny + j] = acc;
     }
   }
+  return SMAISO_OK;
+}
+
+// --- output writers' data, checkpoints, the reference's psi files -------------
+int smaiso_get_slice(smaiso_handle* h, int32_t field, int32_t j, double* out) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (!out) return fail(h, SMAISO_E_INVALID, "out is NULL");
+  if (j < 0 || j >= h->g.ny) return fail(h, SMAISO_E_INVALID, fmt("slice j=%d outside 0..%d", j, h->g.ny - 1));
+  if (!h->all_set()) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
+  HIPTRY(h, hipSetDevice(h->device));
+  const double* src = nullptr;
+  int rc = field_device(h, field, &src);
+  if (rc != SMAISO_OK) return rc;
+  const size_t nz = (size_t)h->g.nz, ny = (size_t)h->g.ny;
+  // saveMid (S:94-129): out[i*Nz + k] = field[(i*Ny + j)*Nz + k]
+  HIPTRY(h, hipMemcpy2DAsync(out, nz * 8, src + (size_t)j * nz, ny * nz * 8, nz * 8, (size_t)h->g.nx,
+                             hipMemcpyDeviceToHost, h->stream));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  h->collect_timing();
+  return SMAISO_OK;
+}
+
+int smaiso_surface_height(smaiso_handle* h, double threshold, double* surf) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (!surf) return fail(h, SMAISO_E_INVALID, "surf is NULL");
+  if (!h->all_set()) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
+  HIPTRY(h, hipSetDevice(h->device));
+  const size_t cols = (size_t)h->g.nx * h->g.ny;
+  if (!h->surf_dev) {
+    void* m = nullptr;
+    HIPTRY(h, hipMalloc(&m, cols * sizeof(int)));
+    h->allocs.push_back(m);
+    h->surf_dev = static_cast<int*>(m);
+  }
+  surface_height(h->rho, h->g.nx, h->g.ny, h->g.nz, threshold, h->surf_dev, h->stream);
+  HIPTRY(h, hipGetLastError());
+  std::vector<int> k(cols);
+  HIPTRY(h, hipMemcpyAsync(k.data(), h->surf_dev, cols * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  // a column without a qualifying k keeps its previous value (S:2284-2293)
+  for (size_t c = 0; c < cols; ++c)
+    if (k[c] >= 0) surf[c] = (double)k[c];
+  return SMAISO_OK;
+}
+
+namespace {
+// Checkpoint layout: this header, then psiq, psiq_old, rho, f_rho (FP64) and
+// psi, N^l of the last step (work precision), each Nx*Ny*Nz values.
+struct CkptHeader {
+  char magic[8];
+  int32_t version, nx, ny, nz, precision, elem;
+  int64_t nloop;
+};
+const char kCkptMagic[8] = {'S', 'M', 'A', 'I', 'S', 'O', 'C', 'K'};
+}  // namespace
+
+int smaiso_save_checkpoint(smaiso_handle* h, const char* path) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (!path) return fail(h, SMAISO_E_INVALID, "path is NULL");
+  if (h->nmem != 1) return fail(h, SMAISO_E_INVALID, "checkpoints are per single handle");
+  if (!h->all_set()) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
+  HIPTRY(h, hipSetDevice(h->device));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  CkptHeader hd{};
+  std::memcpy(hd.magic, kCkptMagic, 8);
+  hd.version = 1;
+  hd.nx = h->g.nx; hd.ny = h->g.ny; hd.nz = h->g.nz;
+  hd.precision = h->p.precision;
+  hd.elem = (int32_t)h->tsz;
+  hd.nloop = h->nloop;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(h, SMAISO_E_INVALID, fmt("cannot open %s for writing", path));
+  const size_t n3 = (size_t)h->n3();
+  std::vector<char> buf(n3 * 8);
+  bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1;
+  const void* arrays[6] = {h->psiq, h->psiq_old, h->rho, h->f_rho, h->psi, h->nlq};
+  for (int a = 0; a < 6 && ok; ++a) {
+    const size_t bytes = n3 * (a < 4 ? 8 : h->tsz);
+    if (hipMemcpy(buf.data(), arrays[a], bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    else ok = std::fwrite(buf.data(), 1, bytes, f) == bytes;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(h, SMAISO_E_HIP, fmt("writing checkpoint %s failed", path));
+  return SMAISO_OK;
+}
+
+int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (!path) return fail(h, SMAISO_E_INVALID, "path is NULL");
+  if (h->nmem != 1) return fail(h, SMAISO_E_INVALID, "checkpoints are per single handle");
+  HIPTRY(h, hipSetDevice(h->device));
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(h, SMAISO_E_INVALID, fmt("cannot open %s", path));
+  CkptHeader hd{};
+  if (std::fread(&hd, sizeof(hd), 1, f) != 1 || std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1) {
+    std::fclose(f);
+    return fail(h, SMAISO_E_INVALID, fmt("%s is not a smaiso checkpoint", path));
+  }
+  if (hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz || hd.precision != h->p.precision ||
+      hd.elem != (int32_t)h->tsz) {
+    std::fclose(f);
+    return fail(h, SMAISO_E_INVALID,
+                fmt("checkpoint %dx%dx%d precision %d does not match the handle (%dx%dx%d precision %d)",
+                    hd.nx, hd.ny, hd.nz, hd.precision, h->g.nx, h->g.ny, h->g.nz, h->p.precision));
+  }
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  const size_t n3 = (size_t)h->n3();
+  std::vector<char> buf(n3 * 8);
+  void* arrays[6] = {h->psiq, h->psiq_old, h->rho, h->f_rho, h->psi, h->nlq};
+  bool ok = true;
+  for (int a = 0; a < 6 && ok; ++a) {
+    const size_t bytes = n3 * (a < 4 ? 8 : h->tsz);
+    ok = std::fread(buf.data(), 1, bytes, f) == bytes &&
+         hipMemcpy(arrays[a], buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  std::fclose(f);
+  if (!ok) return fail(h, SMAISO_E_INVALID, fmt("%s is truncated", path));
+  h->nloop = hd.nloop;
+  h->member_set[0] = 1;
+  h->diag_epoch = -1;
+  return SMAISO_OK;
+}
+
+int smaiso_read_psi_text(const char* const* paths, int32_t npaths, int64_t count, double* psi) {
+  if (!paths || npaths < 1 || count < 0 || !psi) return fail(nullptr, SMAISO_E_INVALID, "bad argument");
+  int64_t got = 0;
+  for (int32_t r = 0; r < npaths && got < count; ++r) {
+    if (!paths[r]) return fail(nullptr, SMAISO_E_INVALID, "path is NULL");
+    std::ifstream in(paths[r]);
+    if (!in.is_open()) return fail(nullptr, SMAISO_E_INVALID, fmt("cannot open %s", paths[r]));
+    double v;
+    while (got < count && (in >> v)) psi[got++] = v;  // S:739-748 reads with operator>>
+  }
+  if (got < count)
+    return fail(nullptr, SMAISO_E_INVALID, fmt("the files hold %lld values, %lld needed", (long long)got, (long long)count));
   return SMAISO_OK;
 }
 

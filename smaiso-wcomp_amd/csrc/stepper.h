@@ -74,6 +74,11 @@ void diag_final(const double* partials, int nb, int nmem, double* out, hipStream
 void convert_d2f(const double* in, float* out, long long n, hipStream_t s);
 void convert_f2d(const float* in, double* out, long long n, hipStream_t s);
 void fill_zero_d(double* out, long long n, hipStream_t s);
+// Surface tracker of S:2278-2318 for one nx x ny x nz field rho: for each
+// column (i, j), the largest k with rho > threshold, or -1 when there is none
+// (the reference keeps the column's previous value then; the caller merges).
+void surface_height(const double* rho, int nx, int ny, int nz, double threshold, int* out,
+                    hipStream_t s);
 
 // Largest number of diagnostic blocks per member.
 constexpr int kDiagBlocks = 1024;

@@ -423,6 +423,35 @@ void fill_zero_d(double* out, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, out, n);
 }
 
+// One wavefront per column: chunks of 64 k values from the top, the first
+// chunk with a qualifying value gives the highest such k (its lowest lane).
+__global__ __launch_bounds__(kBlock) void k_surface(const double* __restrict__ rho, int nx, int ny,
+                                                    int nz, double thr, int* __restrict__ out) {
+  constexpr int kWave = 64;
+  const int lane = threadIdx.x % kWave;
+  const long long col = (long long)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (col >= (long long)nx * ny) return;  // whole wavefronts leave together
+  const double* line = rho + col * nz;
+  int found = -1;
+  for (int top = nz - 1; top >= 0; top -= kWave) {
+    const int k = top - lane;
+    const bool hit = k >= 0 && line[k] > thr;
+    const unsigned long long mask = __ballot(hit);
+    if (mask) {
+      found = top - (__ffsll(mask) - 1);
+      break;
+    }
+  }
+  if (lane == 0) out[col] = found;
+}
+
+void surface_height(const double* rho, int nx, int ny, int nz, double threshold, int* out,
+                    hipStream_t s) {
+  const long long cols = (long long)nx * ny;
+  const unsigned blocks = (unsigned)((cols + (kBlock / 64) - 1) / (kBlock / 64));
+  hipLaunchKernelGGL(k_surface, dim3(blocks), dim3(kBlock), 0, s, rho, nx, ny, nz, threshold, out);
+}
+
 template struct StepKernels<double>;
 template struct StepKernels<float>;
 

@@ -18,6 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)
 REPO_ROOT = os.path.dirname(PKG_ROOT)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsmaiso.so")
+RUN_PATH = os.path.join(PKG_ROOT, "bin", "smaiso_run")  # the command-line driver
 HEADER_PATH = os.path.join(REPO_ROOT, "include", "smaiso.h")
 
 OK, E_INVALID, E_HIP, E_NOMEM, E_STATE, E_NONFINITE = 0, -1, -2, -3, -4, -5
@@ -99,6 +100,11 @@ def lib():
         L.smaiso_timing_enable.argtypes = [vp, C.c_int32]
         L.smaiso_timing_get.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
         L.smaiso_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_get_slice.argtypes = [vp, C.c_int32, C.c_int32, dp]
+        L.smaiso_surface_height.argtypes = [vp, C.c_double, dp]
+        L.smaiso_save_checkpoint.argtypes = [vp, C.c_char_p]
+        L.smaiso_load_checkpoint.argtypes = [vp, C.c_char_p]
+        L.smaiso_read_psi_text.argtypes = [C.POINTER(C.c_char_p), C.c_int32, C.c_int64, dp]
         L.smaiso_autotune.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_int32)]
         L.smaiso_ensemble_autotune.argtypes = [vp]
         L.smaiso_r2r3d_tiles.argtypes = [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, vp, vp,
@@ -125,7 +131,9 @@ def lib():
                      "smaiso_ensemble_destroy", "smaiso_ensemble_groups", "smaiso_ensemble_set_state",
                      "smaiso_ensemble_set_graph", "smaiso_ensemble_step", "smaiso_ensemble_get_field",
                      "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_interface_ic_vjp",
-                     "smaiso_autotune", "smaiso_ensemble_autotune", "smaiso_r2r3d_tiles"):
+                     "smaiso_autotune", "smaiso_ensemble_autotune", "smaiso_r2r3d_tiles",
+                     "smaiso_get_slice", "smaiso_surface_height", "smaiso_save_checkpoint",
+                     "smaiso_load_checkpoint", "smaiso_read_psi_text"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -278,6 +286,26 @@ class Stepper:
         if not (allow_nonfinite and rc == E_NONFINITE):
             _check(rc, self.h)
         return a.value, b.value, c.value
+
+    def slice(self, name: str, j: int | None = None) -> np.ndarray:
+        """x-z slice of a field at y index j (default Ny/2), the saveMid layout."""
+        j = self.params.ny // 2 if j is None else j
+        out = np.empty(self.params.nx * self.params.nz, dtype=np.float64)
+        _check(lib().smaiso_get_slice(self.h, FIELDS[name], int(j), _dp(out)), self.h)
+        return out
+
+    def surface_height(self, threshold: float, prev: np.ndarray | None = None) -> np.ndarray:
+        """Surface tracker (smaiso_surface_height); prev holds the stale values."""
+        surf = np.zeros(self.params.nx * self.params.ny) if prev is None else \
+            np.array(prev, dtype=np.float64).reshape(-1).copy()
+        _check(lib().smaiso_surface_height(self.h, float(threshold), _dp(surf)), self.h)
+        return surf
+
+    def save_checkpoint(self, path: str):
+        _check(lib().smaiso_save_checkpoint(self.h, str(path).encode()), self.h)
+
+    def load_checkpoint(self, path: str):
+        _check(lib().smaiso_load_checkpoint(self.h, str(path).encode()), self.h)
 
     def set_graph(self, enable: bool):
         """Replay pairs of steady steps from one captured hipGraph (smaiso_set_graph)."""
@@ -469,6 +497,14 @@ def interface_ic_vjp(params: Params, ic: InterfaceIC, dpsi: np.ndarray, drho: np
         hp = _dp(height)
     out = np.empty((params.nx, params.ny))
     _check(lib().smaiso_interface_ic_vjp(C.byref(cp), C.byref(ci), hp, _dp(dpsi), _dp(drho), _dp(out)))
+    return out
+
+
+def read_psi_text(paths, count: int) -> np.ndarray:
+    """The reference's psi<rank>.dat restart files, in order (smaiso_read_psi_text)."""
+    arr = (C.c_char_p * len(paths))(*[str(q).encode() for q in paths])
+    out = np.empty(int(count), dtype=np.float64)
+    _check(lib().smaiso_read_psi_text(arr, len(paths), int(count), _dp(out)))
     return out
 
 

@@ -127,3 +127,25 @@ def test_interface_ic_vjp_matches_oracle(explicit):
     oic = O.InterfaceIC(amplitude=0.8, width=1.5, rho_iso=0.85, rho_sm=1.0, noise=0.0, height=h)
     want = A.interface_ic_vjp(op, oic, h, dpsi, drho)
     assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+def test_read_psi_text_rank_files(tmp_path):
+    """The reference's restart files: each rank's slab, one value per line
+    (written here with C++ ostream-like %g formatting), read back in rank order."""
+    rng = np.random.default_rng(4)
+    psi = rng.standard_normal((4, 3, 5))
+    slabs = [psi[:2], psi[2:]]
+    paths = []
+    for r, sl in enumerate(slabs):
+        p = tmp_path / f"psi{r}.dat"
+        p.write_text("".join("%g\n" % v for v in sl.reshape(-1)))
+        paths.append(p)
+    got = S.read_psi_text(paths, psi.size)
+    want = np.array([float("%g" % v) for v in psi.reshape(-1)])
+    assert np.array_equal(got, want)
+    with pytest.raises(S.SmaisoError) as e:
+        S.read_psi_text(paths[:1], psi.size)
+    assert e.value.code == S.E_INVALID and "values" in str(e.value)
+    with pytest.raises(S.SmaisoError) as e:
+        S.read_psi_text([tmp_path / "missing.dat"], 1)
+    assert e.value.code == S.E_INVALID

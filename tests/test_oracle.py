@@ -319,3 +319,37 @@ def test_reference_constants():
     assert math.isclose(p.rho_s, 1.0, rel_tol=1e-15)
     assert math.isclose(p.rho_m_eff, 0.5, rel_tol=1e-15)
     assert p.scale == 0.125 / 64
+
+
+def test_mid_slice_layout():
+    f = np.arange(3 * 4 * 5, dtype=float).reshape(3, 4, 5)
+    s = O.mid_slice(f, 2)
+    # element (i, k) of the slice is field index (i*Ny + j)*Nz + k
+    for i in range(3):
+        for k in range(5):
+            assert s[i * 5 + k] == (i * 4 + 2) * 5 + k
+
+
+def test_surface_track_closed_form_and_stale_columns():
+    """For the tanh interface rho = rho_iso + (1 - rho_iso)/2 (1 - tanh((z - h)/w))
+    the tracked height is the largest k with k dz < h + w atanh(1 - 2 c),
+    c = (thr - rho_iso)/(1 - rho_iso); columns never above thr keep prev."""
+    p = O.Params(nx=6, ny=5, nz=40, dx=0.5, dy=0.5, dz=0.5)
+    x = np.arange(p.nx) * p.dx
+    y = np.arange(p.ny) * p.dy
+    h = 10.0 + 2.3 * np.cos(2 * np.pi * x / (p.nx * p.dx))[:, None] * np.sin(2 * np.pi * y / (p.ny * p.dy))[None, :]
+    ic = O.InterfaceIC(width=1.7, rho_iso=0.8, rho_sm=1.0, noise=0.0, height=h)
+    _, rho = O.make_interface_ic(p, ic)
+    thr = 0.85
+    c = (thr - 0.8) / 0.2
+    zstar = h + ic.width * np.arctanh(1 - 2 * c)
+    want = np.floor(zstar / p.dz - 1e-12).astype(float).reshape(-1)  # k dz < z*
+    prev = np.full(p.nx * p.ny, -7.0)
+    got = O.surface_track(rho, thr, prev)
+    assert np.array_equal(got, want)
+    # a column with no value above the threshold keeps its previous entry
+    rho2 = rho.copy()
+    rho2[2, 3, :] = 0.5
+    got2 = O.surface_track(rho2, thr, prev)
+    assert got2[2 * p.ny + 3] == -7.0
+    assert np.array_equal(np.delete(got2, 2 * p.ny + 3), np.delete(want, 2 * p.ny + 3))
