@@ -220,6 +220,41 @@ def time_adjoint(S, n=128, nsteps=500, every=50):
     return out
 
 
+def time_outputs(S, st, reps=5):
+    """The output writers' device work on the 256^3 state: the surface tracker
+    (one wavefront per column), one x-z slice, and a full-state checkpoint
+    written to and read back from a temporary file."""
+    import tempfile
+    import numpy as np
+    p = st.params
+    prev = np.zeros(p.nx * p.ny)
+    st.surface_height(0.85, prev)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st.surface_height(0.85, prev)
+    surf_ms = (time.perf_counter() - t0) / reps * 1e3
+    st.slice("psi")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st.slice("psi")
+    slice_ms = (time.perf_counter() - t0) / reps * 1e3
+    nbytes = p.nx * p.ny * p.nz * (4 * 8 + 2 * (8 if p.precision == S.FP64 else 4))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "state.ck")
+        t0 = time.perf_counter()
+        st.save_checkpoint(path)
+        save_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st.load_checkpoint(path)
+        load_s = time.perf_counter() - t0
+    res = dict(surface_ms=surf_ms, slice_ms=slice_ms, checkpoint_bytes=nbytes,
+               checkpoint_save_s=save_s, checkpoint_load_s=load_s,
+               note="host-visible times of the synchronous calls; the checkpoint includes disk I/O")
+    log(f"[bench] outputs: surface {surf_ms:.2f} ms, slice {slice_ms:.2f} ms, checkpoint save {save_s:.2f} s "
+        f"load {load_s:.2f} s ({nbytes / 2**20:.0f} MiB)")
+    return res
+
+
 def cpu_baseline(n, psi, rho, seconds_budget=20.0):
     """The FP64 oracle (numpy + scipy.fft) on the host on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -283,6 +318,7 @@ def main():
     sps64 = args.steps / sec64
     log(f"[bench] FP64 {sps64:.2f} steps/s, diag {diag64}")
     kt64 = kernel_timing(st64, args.profile_steps)
+    outputs = time_outputs(S, st64)
     del st64
 
     mixed = None
@@ -375,6 +411,7 @@ def main():
         "fp32_mixed": mixed,
         "ensemble": ens,
         "adjoint": adj,
+        "outputs": outputs,
         "cpu_baseline": cpu,
         "diagnostics_fp64": diag64,
     }
