@@ -11,7 +11,9 @@
 // fused into these kernels: CT^T y = ICT(wc y), ST_a^T y = IST_a(ws_a y),
 // ICT^T x = CT(x)/wc, IST_a^T x = ST_a(x)/ws_a, where wc doubles every index-0
 // plane and ws_a doubles the index-0 planes of the two DCT axes and the
-// index N_a-1 plane of the DST axis a.
+// index N_a-1 plane of the DST axis a.  The one-axis operators of the forward
+// step (gradients, div f, the Gaussian filter) transpose into one-axis
+// passes (OP_AX_GRAD_T, OP_AX_DIV_T, OP_AX_GAUSS).
 #pragma once
 
 #include "stepper.h"
@@ -51,24 +53,22 @@ struct AdjKernels {
   static void update(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
   // a16 + a15 (real space), input bnl = CT^T(bnlq)
   static void nl_density(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
-  // a14 + a13 + a12 part 1 (spectral): inputs ST_a(bv_a), CT(bdivv), CT(bdivf)
+  // a14 + a13 + a12 part 1 (spectral): inputs ST_a(bv_a), CT(bdivv)
   static void velocity_a(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
   // a12 part 2 (spectral gathers) -> ws_a Sq^_a, wc fgq^, wc G fgq^
   static void velocity_b(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
-  // a10 (real space), inputs IST'd wSq, wfq and ICT'd wfg, wGfg
+  // a10 (real space), inputs IST'd wSq, ICT'd wfg, wGfg and f bar (wfq, from the
+  // one-axis transposes of div f)
   static void forces(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
   // a9 + a7 pointwise tails, inputs ybar = op_lap(blap), a0bar = op_gauss(bamp)
   static void amp_lap(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
-  // a8 + a6 + psi = ICT(psiq) (spectral), inputs CT(bpsi), CT(blpsi), ST_a(bgp_a), ST_a(brd_a);
-  // writes the spectral rho adjoint to tmp (for ICT)
+  // a8 + psi = ICT(psiq) (spectral), inputs CT(bpsi), CT(blpsi)
   static void grad(const StepConsts& c, const AdjArgs<T>& A, hipStream_t s);
   // loss sum (psi - target)^2 into slot 3 partials, tmp = 2 (psi - target)
   static void loss(const StepConsts& c, const T* psi, const double* target, double* tmp,
                    double* partials, hipStream_t s);
 };
 
-// bR += tmp
-void adj_add(const StepConsts& c, double* bR, const double* tmp, hipStream_t s);
 // out = in / wc (ICT^T after a CT)
 void adj_div_wc(const StepConsts& c, const double* in, double* out, hipStream_t s);
 // out = scale * wc * (a + b)  (CT^T of set_state's psiq = scale CT(psi0), psiq_old = psiq)

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kB) void k_adj_nl(StepConsts c, AdjArgs<T> A) {
 // (S:1583-1974): per component a, from vsol^_a = ST_a(bv_a)/ws_a and
 // virr^_a = vsol^_a + vq_a shift(CT(bdivv)/wc), the quantities that the
 // gathers of velocity_b read: dot_a, cfg_a (the fgrad^ term) and dir_a (the
-// S^_a diagonal); plus f^_a = scale vq_a shift(CT(bdivf)/wc) and dL/dnu.
+// S^_a diagonal); and dL/dnu.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kB) void k_adj_vel_a(StepConsts c, AdjArgs<T> A) {
@@ -154,15 +154,13 @@ __global__ __launch_bounds__(kB) void k_adj_vel_a(StepConsts c, AdjArgs<T> A) {
     for (int a = 0; a < 3; ++a) {
       const double wsa = ws_at(c, a, p.i, p.j, p.k);
       double vs = A.bv[a][idx] / wsa;
-      double vi = vs, fq = 0.0;
+      double vi = vs;
       if (ii[a] <= nn[a] - 2) {
         const unsigned nb = idx + st[a];
         const double q = c.vq[a][ii[a] + 1];
         const double wn = wc_at(p.i + (a == 0), p.j + (a == 1), p.k + (a == 2));
         vi += q * A.bdivv[nb] / wn;
-        fq = c.scale * q * A.bdivf[nb] / wn;
       }
-      A.wfq[a][idx] = wsa * fq;  // -> IST_a: f_a bar = ST_a^T(fq)
       // output masks, quirks Q1 and Q2
       const bool zero = a == 0 ? (p.i == c.nx - 1) : (p.j == c.ny - 1 || p.k == c.nz - 1);
       if (zero) vs = vi = 0.0;
@@ -328,11 +326,10 @@ __global__ __launch_bounds__(kB) void k_adj_amp_lap(StepConsts c, AdjArgs<T> A) 
 }
 
 // ---------------------------------------------------------------------------
-// a8 (L psi = ICT(alpha (q0^2 - |q|^2) psiq)), a6 (grad_a psi = IST_a(Sh_a
-// psiq), grad_a rho = IST_a(scale Sh_a CT(rho)), with Sh_a x[n] = -vs_a[n]
-// x[n+e_a]) and psi = ICT(psiq), spectral.  The ws_a of IST_a^T equals wc at
-// the shifted index, so every term of psiq bar is divided by wc; the rho
-// term needs no weight (CT^T cancels it).
+// a8 (L psi = ICT(alpha (q0^2 - |q|^2) psiq)) and psi = ICT(psiq), spectral:
+// psiq bar += (alpha (q0^2 - |q|^2) CT(Lpsi bar) + CT(psi bar)) / wc.  The
+// gradients' transposes (a6) were added to psi bar and rho bar in real space
+// by one-axis passes before.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kB) void k_adj_grad(StepConsts c, AdjArgs<T> A) {
@@ -340,27 +337,8 @@ __global__ __launch_bounds__(kB) void k_adj_grad(StepConsts c, AdjArgs<T> A) {
   const unsigned idx = blockIdx.x * kB + threadIdx.x;
   if (idx >= n3) return;
   const Ix p = decomp(idx, c.ny, c.nz);
-  const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
   const double lq = c.alpha * (c.q02 - mq2c_at(c, p.i, p.j, p.k));
-  double acc = lq * A.blpsi[idx] + A.bpsi[idx];
-  double r = 0.0;
-  if (p.i >= 1) {
-    const double w = -c.vs[0][p.i - 1];
-    acc += w * A.bgp[0][idx - sI];
-    r += w * A.brd[0][idx - sI];
-  }
-  if (p.j >= 1) {
-    const double w = -c.vs[1][p.j - 1];
-    acc += w * A.bgp[1][idx - sJ];
-    r += w * A.brd[1][idx - sJ];
-  }
-  if (p.k >= 1) {
-    const double w = -c.vs[2][p.k - 1];
-    acc += w * A.bgp[2][idx - 1];
-    r += w * A.brd[2][idx - 1];
-  }
-  A.bP[idx] += acc / wc_at(p.i, p.j, p.k);
-  A.tmp[idx] = c.scale * r;  // -> ICT: rho bar
+  A.bP[idx] += (lq * A.blpsi[idx] + A.bpsi[idx]) / wc_at(p.i, p.j, p.k);
 }
 
 template <typename T>
@@ -376,13 +354,6 @@ __global__ __launch_bounds__(kB) void k_adj_loss(StepConsts c, const T* __restri
     l = r * r;
   }
   block_partial(l, partials + 3 * (size_t)nblocks(c) + blockIdx.x);
-}
-
-__global__ __launch_bounds__(kB) void k_adj_add(StepConsts c, double* __restrict__ bR,
-                                                const double* __restrict__ tmp) {
-  const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  const unsigned idx = blockIdx.x * kB + threadIdx.x;
-  if (idx < n3) bR[idx] += tmp[idx];
 }
 
 __global__ __launch_bounds__(kB) void k_adj_div_wc(StepConsts c, const double* __restrict__ in,
@@ -462,9 +433,6 @@ void AdjKernels<T>::loss(const StepConsts& c, const T* psi, const double* target
   hipLaunchKernelGGL((k_adj_loss<T>), dim3(nblocks(c)), dim3(kB), 0, s, c, psi, target, tmp, partials);
 }
 
-void adj_add(const StepConsts& c, double* bR, const double* tmp, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_add, dim3(nblocks(c)), dim3(kB), 0, s, c, bR, tmp);
-}
 void adj_div_wc(const StepConsts& c, const double* in, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_div_wc, dim3(nblocks(c)), dim3(kB), 0, s, c, in, out);
 }

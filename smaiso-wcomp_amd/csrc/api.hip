@@ -1428,28 +1428,47 @@ int adjoint_step(smaiso_handle* h, const double* rho_n, bool first) {
   K::nl_density(c, A, s);
   for (int a = 0; a < 3; ++a) HIPTRY(h, h->xform(F64, kST[a], A.bv[a], nullptr, A.bv[a]));
   HIPTRY(h, h->xform(F64, kCT, A.bdivv, nullptr, A.bdivv));
-  HIPTRY(h, h->xform(F64, kCT, A.bdivf, nullptr, A.bdivf));
+  // div f = sum_a D_a(f_a) with one-axis D_a, so f_a bar = D_a^T(div f bar)
+  {
+    Fuse t = fb;
+    t.op = OP_AX_DIV_T;
+    for (int a = 0; a < 3; ++a) {
+      t.axis = a;
+      HIPTRY(h, h->axpass(F64, a, A.bdivf, A.wfq[a], t));
+    }
+  }
   K::velocity_a(c, A, s);
   K::velocity_b(c, A, s);
-  for (int a = 0; a < 3; ++a) {
-    HIPTRY(h, h->xform(F64, kIST[a], A.wSq[a], nullptr, A.wSq[a]));
-    HIPTRY(h, h->xform(F64, kIST[a], A.wfq[a], nullptr, A.wfq[a]));
-  }
+  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xform(F64, kIST[a], A.wSq[a], nullptr, A.wSq[a]));
   HIPTRY(h, h->xform(F64, kICT, A.wfg, nullptr, A.wfg));
   HIPTRY(h, h->xform(F64, kICT, A.wGfg, nullptr, A.wGfg));
   K::forces(c, A, s);
   HIPTRY(h, h->xmid_dt(F64, A.blap, fb, OP_MID_NEGLAP, A.ybar));
-  HIPTRY(h, h->xmid_dt(F64, A.bamp, fb, OP_MID_AMP, A.a0bar));
+  // the amplitude filter is symmetric and a product over the axes
+  {
+    Fuse t = fb;
+    t.op = OP_AX_GAUSS;
+    for (int a = 2; a >= 0; --a) {
+      t.axis = a;
+      HIPTRY(h, h->axpass(F64, a, a == 2 ? A.bamp : A.a0bar, A.a0bar, t));
+    }
+  }
   K::amp_lap(c, A, s);
+  // grad_a = one-axis M_a of psi (= ICT(psiq)) and of rho: psi bar and rho bar
+  // gain M_a^T(grad_a bar), accumulated by one-axis passes
+  {
+    Fuse t = fb;
+    t.op = OP_AX_GRAD_T;
+    t.accumulate = 1;
+    for (int a = 0; a < 3; ++a) {
+      t.axis = a;
+      HIPTRY(h, h->axpass(F64, a, A.bgp[a], A.bpsi, t));
+      HIPTRY(h, h->axpass(F64, a, A.brd[a], A.bR, t));
+    }
+  }
   HIPTRY(h, h->xform(F64, kCT, A.bpsi, nullptr, A.bpsi));
   HIPTRY(h, h->xform(F64, kCT, A.blpsi, nullptr, A.blpsi));
-  for (int a = 0; a < 3; ++a) {
-    HIPTRY(h, h->xform(F64, kST[a], A.bgp[a], nullptr, A.bgp[a]));
-    HIPTRY(h, h->xform(F64, kST[a], A.brd[a], nullptr, A.brd[a]));
-  }
   K::grad(c, A, s);
-  HIPTRY(h, h->xform(F64, kICT, A.tmp, nullptr, A.tmp));
-  adj_add(c, A.bR, A.tmp, s);
   adj_reduce(h->adj_partials, 0, nb, h->adj_acc, 0, s);  // dzeta
   adj_reduce(h->adj_partials, 2, nb, h->adj_acc, 0, s);
   adj_reduce(h->adj_partials, 1, nb, h->adj_acc, 1, s);  // dnu

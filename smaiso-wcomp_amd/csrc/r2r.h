@@ -42,7 +42,11 @@ enum FuseOp : int {
   OP_AX_DIV = 12,        // out (+)= iDCT(y), y[n] = c Vq[n] DST(in)[n-1] (0 at n = 0)
   OP_AX_GAUSS = 13,      // out = iDCT(c exp(-1.57^2 2 Vq[n]^2) DCT(in))
   OP_AX_GAUSS_AMP = 14,  // OP_AX_GAUSS (z only) of the amplitude computed on load (OP_LD_AMP)
-  OP_COUNT = 15,
+  // transposes (for the adjoint): REDFT10^T = REDFT01 diag(2,1,..,1) and
+  // RODFT10^T = RODFT01 diag(1,..,1,2) make them one-axis operators too
+  OP_AX_GRAD_T = 15,     // out (+)= iDCT(y), y[n] = -c Vs[n-1] DST(in)[n-1] (0 at n = 0)
+  OP_AX_DIV_T = 16,      // out (+)= iDST(y), y[n] = c Vq[n+1] DCT(in)[n+1] (0 at n = N-1)
+  OP_COUNT = 17,
 };
 
 struct Fuse {

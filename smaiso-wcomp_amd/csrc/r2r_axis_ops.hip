@@ -17,6 +17,8 @@ hipError_t launch_ax(const PassIO& io, void* tw, void* mk, int axis, const Geom&
       case OP_AX_DIV: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV>(io, tw, mk, kind, axis, g, s);
       case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS>(io, tw, mk, kind, axis, g, s);
       case OP_AX_GAUSS_AMP: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS_AMP>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD_T>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV_T>(io, tw, mk, kind, axis, g, s);
       default: return hipErrorInvalidValue;
     }
   }
@@ -24,6 +26,8 @@ hipError_t launch_ax(const PassIO& io, void* tw, void* mk, int axis, const Geom&
     case OP_AX_GRAD: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD>(io, tw, mk, kind, axis, g, s);
     case OP_AX_DIV: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV>(io, tw, mk, kind, axis, g, s);
     case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GAUSS>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD_T>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV_T>(io, tw, mk, kind, axis, g, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -456,6 +456,8 @@ __device__ __forceinline__ double ax_weight(const Fuse& f, int m, int n) {
   const double c = f.c0 / (2.0 * n);
   if constexpr (OP == OP_AX_GRAD) return -c * f.vs[f.axis][m];
   else if constexpr (OP == OP_AX_DIV) return c * f.vq[f.axis][m];
+  else if constexpr (OP == OP_AX_GRAD_T) return -c * f.vs[f.axis][m - 1];
+  else if constexpr (OP == OP_AX_DIV_T) return c * f.vq[f.axis][m + 1];
   else {
     const double q = f.vq[f.axis][m];
     return c * exp(-1.57 * 1.57 * q * q * 2);
@@ -464,8 +466,8 @@ __device__ __forceinline__ double ax_weight(const Fuse& f, int m, int n) {
 // source mode of output mode m, or -1 for a zero output
 template <int OP>
 __device__ __forceinline__ int ax_source(int m, int n) {
-  if constexpr (OP == OP_AX_GRAD) return m <= n - 2 ? m + 1 : -1;
-  else if constexpr (OP == OP_AX_DIV) return m >= 1 ? m - 1 : -1;
+  if constexpr (OP == OP_AX_GRAD || OP == OP_AX_DIV_T) return m <= n - 2 ? m + 1 : -1;
+  else if constexpr (OP == OP_AX_DIV || OP == OP_AX_GRAD_T) return m >= 1 ? m - 1 : -1;
   else return m;
 }
 
@@ -523,7 +525,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   constexpr int SEQ = N + 1;
   constexpr int HN = N / 2;
   constexpr bool MID = (OP == OP_MID_AMP || OP == OP_MID_NEGLAP);
-  constexpr bool AX = (OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GAUSS_AMP);
+  constexpr bool AX = (OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GAUSS_AMP ||
+                       OP == OP_AX_GRAD_T || OP == OP_AX_DIV_T);
   // load op of the first (z) pass
   constexpr int LD_OP = OP == OP_AX_GAUSS_AMP ? OP_LD_AMP : ((MID || AX) ? OP_PLAIN : OP);
   constexpr int AXOP = OP == OP_AX_GAUSS_AMP ? OP_AX_GAUSS : OP;
@@ -537,8 +540,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   const bool fwd = MID || AX || (a.kind == REDFT10 || a.kind == RODFT10);
   // sine: DST on the load side (forward DST-II, or inverse DST-III input);
   // sine_out: DST on the store side
-  const bool sine = AX ? (OP == OP_AX_DIV) : (!MID && (a.kind == RODFT10 || a.kind == RODFT01));
-  const bool sine_out = AX ? (OP == OP_AX_GRAD) : sine;
+  const bool sine = AX ? (OP == OP_AX_DIV || OP == OP_AX_GRAD_T) : (!MID && (a.kind == RODFT10 || a.kind == RODFT01));
+  const bool sine_out = AX ? (OP == OP_AX_GRAD || OP == OP_AX_DIV_T) : sine;
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
@@ -880,7 +883,10 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
   T* y = x + tl * n;  // second buffer of the mid ops
   const bool mid = a.f.op == OP_MID_AMP || a.f.op == OP_MID_NEGLAP;
   const int op = a.f.op;
-  const bool ax = op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP;
+  const bool ax = op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP ||
+                  op == OP_AX_GRAD_T || op == OP_AX_DIV_T;
+  const bool ax_fsine = op == OP_AX_DIV || op == OP_AX_GRAD_T;   // forward DST-II
+  const bool ax_isine = op == OP_AX_GRAD || op == OP_AX_DIV_T;   // inverse DST-III
   TileGeo g;
   if (a.axis == 2) {
     long long first = (long long)blockIdx.x * tl;
@@ -924,7 +930,7 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
   __syncthreads();
   if (ax) {
     // forward 1-D transform into y, the operator back into x
-    const int fk = op == OP_AX_DIV ? RODFT10 : REDFT10;
+    const int fk = ax_fsine ? RODFT10 : REDFT10;
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
       const int k = e / tl, l = e - (e / tl) * tl;
       if (l < g.nvalid) y[l * n + k] = direct_sum<T>(fk, x + l * n, n, k, d.costab);
@@ -938,6 +944,10 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
         if (ax_source<OP_AX_GRAD>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_GRAD>(a.f, m, n) * (double)y[l * n + m + 1]);
       } else if (op == OP_AX_DIV) {
         if (ax_source<OP_AX_DIV>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_DIV>(a.f, m, n) * (double)y[l * n + m - 1]);
+      } else if (op == OP_AX_GRAD_T) {
+        if (ax_source<OP_AX_GRAD_T>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_GRAD_T>(a.f, m, n) * (double)y[l * n + m - 1]);
+      } else if (op == OP_AX_DIV_T) {
+        if (ax_source<OP_AX_DIV_T>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_DIV_T>(a.f, m, n) * (double)y[l * n + m + 1]);
       } else {
         v = (T)(ax_weight<T, OP_AX_GAUSS>(a.f, m, n) * (double)y[l * n + m]);
       }
@@ -961,7 +971,7 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     else { k = e / tl; l = e - k * tl; }
     if (l >= g.nvalid) continue;
     T r;
-    if (ax) r = direct_sum<T>(op == OP_AX_GRAD ? RODFT01 : REDFT01, x + l * n, n, k, d.costab);
+    if (ax) r = direct_sum<T>(ax_isine ? RODFT01 : REDFT01, x + l * n, n, k, d.costab);
     else if (mid) r = direct_sum<T>(REDFT01, y + l * n, n, k, d.costab);
     else r = direct_sum<T>(a.kind, x + l * n, n, k, d.costab);
     const long long oi = g.base + l * g.line_step + k * g.stride_m;
@@ -1021,7 +1031,8 @@ struct TileChoice {
 };
 
 inline bool is_axis_op(int op) {
-  return op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP;
+  return op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP ||
+         op == OP_AX_GRAD_T || op == OP_AX_DIV_T;
 }
 
 // One-axis operator pass (OP_AX_*) of FFT length n with the default tiles;
