@@ -84,7 +84,7 @@ def kernel_timing(st, nsteps):
     st.timing(True)
     st.step(nsteps)
     res = {}
-    for kc in ("r2r_z", "r2r_y", "r2r_x", "spectral", "pointwise"):
+    for kc in ("r2r_z", "r2r_y", "r2r_x", "r2r_axis", "spectral", "pointwise"):
         ms, n, by = st.timing_get(kc)
         res[kc] = dict(total_ms=ms, launches=n, bytes=by)
     st.timing(False)

@@ -209,13 +209,14 @@ int smaiso_make_interface_ic(const smaiso_params* p, const smaiso_ic* ic, const 
 int smaiso_timing_enable(smaiso_handle* h, int32_t enable);
 
 /* Kernel classes for smaiso_timing_get. */
-#define SMAISO_KC_R2R_Z 0       /* axis pass along z */
-#define SMAISO_KC_R2R_Y 1       /* axis pass along y */
-#define SMAISO_KC_R2R_X 2       /* axis pass along x */
+#define SMAISO_KC_R2R_Z 0       /* transform pass along z (plain or with a fused load) */
+#define SMAISO_KC_R2R_Y 1       /* transform pass along y */
+#define SMAISO_KC_R2R_X 2       /* transform pass along x (also the fused mid operators) */
 #define SMAISO_KC_SPECTRAL 3    /* per-mode kernels */
-#define SMAISO_KC_POINTWISE 4   /* real-space kernels */
+#define SMAISO_KC_POINTWISE 4   /* real-space kernels (also the forces fused into a z pass) */
 #define SMAISO_KC_REDUCE 5      /* diagnostics */
-#define SMAISO_KC_COUNT 6
+#define SMAISO_KC_R2R_AXIS 6    /* one-axis operator passes (forward, operator, inverse along one axis) */
+#define SMAISO_KC_COUNT 7
 
 /* Total milliseconds, launch count and algorithmic bytes moved by kernel
  * class kc since timing was enabled. */

@@ -8,6 +8,7 @@
 #include "../../include/smaiso.h"
 #include "common.h"
 #include "adjoint.h"
+#include "fused.h"
 #include "r2r.h"
 #include "stepper.h"
 
@@ -231,7 +232,8 @@ struct smaiso_handle {
     return n * (in + out);
   }
   hipError_t run_pass(DType dt, int kind, int axis, const PassIO& io, const Geom& gg) {
-    Scope sc(this, axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X,
+    Scope sc(this, is_axis_op(io.f.op) ? SMAISO_KC_R2R_AXIS
+                   : axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X,
              pass_bytes(dt, io, gg));
     return r2r.pass(dt, kind, axis, io, gg, stream);
   }
@@ -254,6 +256,16 @@ struct smaiso_handle {
   }
   hipError_t xwf(const int kinds[3], const Fuse& f, void* out) {
     return xform(mixed ? F32 : F64, kinds, nullptr, nullptr, out, &f);
+  }
+  // the y and x passes of a transform whose z pass has run, in place
+  hipError_t xw_yx(const int kinds[3], void* buf) {
+    const DType dt = mixed ? F32 : F64;
+    PassIO io;
+    io.in = buf;
+    io.out = buf;
+    hipError_t e = run_pass(dt, kinds[1], 1, io, g);
+    if (e != hipSuccess) return e;
+    return run_pass(dt, kinds[0], 0, io, g);
   }
   // DCT-II (z pass with f's load op), DCT-II along y, then one x pass doing
   // DCT-II, the diagonal operator mid_op and DCT-III, then DCT-III along y, z.
@@ -514,16 +526,22 @@ int run_step(smaiso_handle* h) {
   // The FP32-mixed path keeps the 3-D form for grad psi: the shift is
   // applied on load to the FP64 coefficients psiq, each rounded relative to
   // its own size, where a pass over the FP32 copy of psi would amplify that
-  // copy's white rounding noise by |q|
+  // copy's white rounding noise by |q|.  The passes that read one input run
+  // back to back, so its later reads find it in the memory-side cache.
   for (int a = 0; a < 3; ++a) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
     HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
+  }
+  for (int a = 0; a < 3; ++a) {
+    Fuse g = fb;
+    g.axis = a;
     if (h->mixed) {
       g.op = OP_LD_GRAD_D; g.src[0] = h->psiq; g.c0 = 1.0;
       HIPTRY(h, h->xwf(kIST[a], g, gp[a]));
     } else {
+      g.op = OP_AX_GRAD;
       HIPTRY(h, h->axpass(F64, a, h->psi, gp[a], g));
     }
   }
@@ -560,18 +578,36 @@ int run_step(smaiso_handle* h) {
     smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 3 + db1);
     K::relax(c, static_cast<T*>(h->amp), static_cast<T*>(h->laprl), h->rho, s);
   }
-  // a10 (S:1385-1425)
-  {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
-    K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), h->rho,
-              static_cast<T*>(h->amp), static_cast<T*>(h->psi), gp[0], gp[1], gp[2], rd[0],
-              rd[1], rd[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
-              static_cast<T*>(h->mu), S[0], S[1], S[2], f[0], f[1], f[2], s);
+  // a10 (S:1385-1425) and a11 (S:1427-1457): the z pass of the five
+  // transforms computes the forces on load where the length allows
+  if (forces_zpass_supported(h->mixed ? F32 : F64, h->g.nz)) {
+    ForcesIO<T> io;
+    io.lpsi = static_cast<const T*>(h->lpsi); io.laprl = static_cast<const T*>(h->laprl);
+    io.amp = static_cast<const T*>(h->amp); io.psi = static_cast<const T*>(h->psi);
+    io.rho = h->rho;
+    for (int a = 0; a < 3; ++a) { io.gp[a] = gp[a]; io.rd[a] = rd[a]; io.f[a] = f[a]; io.zout[2 + a] = S[a]; }
+    io.mu = static_cast<T*>(h->mu);
+    io.zout[0] = static_cast<T*>(h->fgrad);
+    io.zout[1] = static_cast<T*>(h->erho);
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
+      HIPTRY(h, forces_zpass<T>(c, io, h->r2r, s));
+    }
+    HIPTRY(h, h->xw_yx(kCT, h->fgrad));
+    HIPTRY(h, h->xw_yx(kCT, h->erho));
+    for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw_yx(kST[a], S[a]));
+  } else {
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
+      K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), h->rho,
+                static_cast<T*>(h->amp), static_cast<T*>(h->psi), gp[0], gp[1], gp[2], rd[0],
+                rd[1], rd[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
+                static_cast<T*>(h->mu), S[0], S[1], S[2], f[0], f[1], f[2], s);
+    }
+    HIPTRY(h, h->xw(kCT, h->fgrad, h->fgrad));
+    HIPTRY(h, h->xw(kCT, h->erho, h->erho));
+    for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kST[a], S[a], S[a]));
   }
-  // a11 (S:1427-1457)
-  HIPTRY(h, h->xw(kCT, h->fgrad, h->fgrad));
-  HIPTRY(h, h->xw(kCT, h->erho, h->erho));
-  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kST[a], S[a], S[a]));
   // a11 tail + a12 (S:1479-1974)
   {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 11);

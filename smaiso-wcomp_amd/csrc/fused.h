@@ -1,0 +1,34 @@
+// Time-step stages fused into the z pass of the transforms that consume
+// them (kernels in fused.hip).  A pointwise stage that writes fields which
+// are transformed next costs one write and one read of each such field; when
+// the z pass computes the stage on load, those fields go from registers to
+// LDS instead, and the pass writes their z-transformed lines in place of the
+// real-space values.  The y and x passes of the transforms follow as usual.
+#pragma once
+
+#include "r2r.h"
+#include "stepper.h"
+
+namespace smaiso {
+
+// a10 (smaiso-wcomp.cpp:1385-1425) fused with the z passes of a11
+// (S:1427-1457): fgrad and erho (REDFT10 along z) and S_x, S_y (REDFT10),
+// S_z (RODFT10).
+template <typename T>
+struct ForcesIO {
+  const T *lpsi, *laprl, *amp, *psi;
+  const T* gp[3];      // grad psi
+  const double* rho;
+  const double* rd[3];  // grad rho
+  T* mu;                // chemical potential (real space)
+  T* f[3];              // forces f_a (real space)
+  T* zout[5];           // z pass of fgrad, erho, S_x, S_y, S_z
+};
+
+// hipErrorNotSupported when nz has no fused kernel (a length off the FFT
+// path, or an LDS tile above 64 KiB); the caller then runs the unfused stages.
+template <typename T>
+hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hipStream_t s);
+bool forces_zpass_supported(DType dt, int nz);
+
+}  // namespace smaiso

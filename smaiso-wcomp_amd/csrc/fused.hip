@@ -1,0 +1,179 @@
+// Stages of the time step fused into transform passes (see fused.h).
+#include "fused.h"
+
+#include "r2r_kernels.h"
+
+namespace smaiso {
+
+namespace {
+
+constexpr int kForceOut = 5;  // fgrad, erho, S_x, S_y, S_z
+
+template <typename T, int N>
+struct ForcesTile {
+  static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
+  static constexpr int NT = 256;
+  static constexpr size_t lds = (size_t)kForceOut * P * (N + 1) * sizeof(cplx<T>);
+  static constexpr bool fits = lds <= 65536;
+};
+
+// One workgroup: 2P consecutive z lines.  Load phase: every point of the
+// tile evaluates a10 in FP64 registers (as k_forces), stores mu and f_a and
+// puts the five transformed fields at their Makhoul positions of 5P complex
+// sequences in LDS (field r, line pair c -> sequence r P + c).  Then one
+// FFT over the 5P sequences and Makhoul's post-processing store the z pass
+// outputs (S_z is a DST-II along z: odd inputs negated, outputs reversed).
+template <typename T, int N, int P, int NT>
+__global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> io,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      const cplx<T>* __restrict__ mk) {
+  constexpr int NF = kForceOut, TL = 2 * P, SEQ = N + 1, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, NF * P, NT);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
+  T* rbuf = reinterpret_cast<T*>(smem_raw);
+  using A = double;
+  const int tid = threadIdx.x;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
+  const long long base = (long long)blockIdx.y * nl * N + first * N;
+  const MemberConsts mc = c.mc[blockIdx.y];
+
+  constexpr int E = TL * N;
+  constexpr int IT = (E + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (E % NT == 0 || e < E) {
+      const int l = e / N, m = e - l * N;
+      A v[NF] = {A(0), A(0), A(0), A(0), A(0)};
+      if (l < nvalid) {
+        const long long idx = base + e;
+        const A L = io.lpsi[idx], lr = io.laprl[idx], r = io.rho[idx], am = io.amp[idx], ps = io.psi[idx];
+        const A kp = mc.kp, zeta = mc.zeta, rho0 = mc.rho0;
+        const A en = A(0.5) * (L * L);
+        const A ra = rho0 + kp * am;
+        v[0] = A(0.5) * zeta * (r * r - ra * ra);
+        const A mu1 = lr + r * L;
+        const A gx = io.gp[0][idx], gy = io.gp[1][idx], gz = io.gp[2][idx];
+        v[2] = mu1 * gx + en * io.rd[0][idx];
+        v[3] = mu1 * gy + en * io.rd[1][idx];
+        v[4] = mu1 * gz + en * io.rd[2][idx];
+        const A p2 = ps * ps, p3 = p2 * ps, p5 = p3 * p2;
+        io.mu[idx] = (T)(lr + r * (L - (A)c.eps * ps - (A)c.beta * p3 + (A)c.gamma * p5));
+        v[1] = r * en;
+        const A fc = kp * zeta * (r - rho0 - kp * am) / (am + A(0.001));
+        io.f[0][idx] = (T)(fc * gx);
+        io.f[1][idx] = (T)(fc * gy);
+        io.f[2][idx] = (T)(fc * gz);
+      }
+      const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+#pragma unroll
+      for (int r = 0; r < NF; ++r) {
+        T x = (T)v[r];
+        if (r == NF - 1 && (m & 1)) x = -x;
+        rbuf[2 * ((r * P + (l >> 1)) * SEQ + pos) + (l & 1)] = x;
+      }
+    }
+  }
+  __syncthreads();
+  fft_stage<T, N, NF * P, NT, MAXR, -1, 0>(buf, tw);
+
+  constexpr int W = NF * P * HN;
+  constexpr int ITW = (W + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int sq = w / HN, k = w - sq * HN;
+      const int r = sq / P, cc = sq - r * P;
+      const int kb = (k == 0) ? HN : N - k;
+      T y0a, y1a, y0b, y1b;
+      makhoul_post<T, N>(mk, k, kb, buf[sq * SEQ + k], buf[sq * SEQ + kb], y0a, y1a, y0b, y1b);
+      const bool sine = r == NF - 1;
+      const int oa = sine ? N - 1 - k : k;
+      const int ob = sine ? N - 1 - kb : kb;
+      T* o = r == 0 ? io.zout[0] : r == 1 ? io.zout[1] : r == 2 ? io.zout[2] : r == 3 ? io.zout[3] : io.zout[4];
+      const int l0 = 2 * cc;
+      if (l0 < nvalid) {
+        T* q = o + base + (long long)l0 * N;
+        q[oa] = y0a;
+        q[ob] = y0b;
+      }
+      if (l0 + 1 < nvalid) {
+        T* q = o + base + (long long)(l0 + 1) * N;
+        q[oa] = y1a;
+        q[ob] = y1b;
+      }
+    }
+  }
+}
+
+template <typename T, int N>
+hipError_t launch_forces(const StepConsts& c, const ForcesIO<T>& io, const void* tw, const void* mk,
+                         hipStream_t s) {
+  using F = ForcesTile<T, N>;
+  if constexpr (!F::fits) {
+    return hipErrorNotSupported;
+  } else {
+    constexpr int TL = 2 * F::P;
+    const long long blocks = ((long long)c.nx * c.ny + TL - 1) / TL;
+    hipLaunchKernelGGL((forces_z_kernel<T, N, F::P, F::NT>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                       dim3(F::NT), F::lds, s, c, io, static_cast<const cplx<T>*>(tw),
+                       static_cast<const cplx<T>*>(mk));
+    return hipGetLastError();
+  }
+}
+
+template <typename T>
+bool fits_for(int n) {
+  switch (n) {
+    case 16: return ForcesTile<T, 16>::fits;
+    case 32: return ForcesTile<T, 32>::fits;
+    case 48: return ForcesTile<T, 48>::fits;
+    case 64: return ForcesTile<T, 64>::fits;
+    case 96: return ForcesTile<T, 96>::fits;
+    case 128: return ForcesTile<T, 128>::fits;
+    case 192: return ForcesTile<T, 192>::fits;
+    case 256: return ForcesTile<T, 256>::fits;
+    case 384: return ForcesTile<T, 384>::fits;
+    case 512: return ForcesTile<T, 512>::fits;
+    case 1024: return ForcesTile<T, 1024>::fits;
+    default: return false;
+  }
+}
+
+}  // namespace
+
+bool forces_zpass_supported(DType dt, int nz) {
+  return dt == F64 ? fits_for<double>(nz) : fits_for<float>(nz);
+}
+
+template <typename T>
+hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hipStream_t s) {
+  const DType dt = sizeof(T) == 8 ? F64 : F32;
+  if (!forces_zpass_supported(dt, c.nz)) return hipErrorNotSupported;
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(dt, c.nz, &tw, &mk);
+  if (e != hipSuccess) return e;
+  switch (c.nz) {
+    case 16: return launch_forces<T, 16>(c, io, tw, mk, s);
+    case 32: return launch_forces<T, 32>(c, io, tw, mk, s);
+    case 48: return launch_forces<T, 48>(c, io, tw, mk, s);
+    case 64: return launch_forces<T, 64>(c, io, tw, mk, s);
+    case 96: return launch_forces<T, 96>(c, io, tw, mk, s);
+    case 128: return launch_forces<T, 128>(c, io, tw, mk, s);
+    case 192: return launch_forces<T, 192>(c, io, tw, mk, s);
+    case 256: return launch_forces<T, 256>(c, io, tw, mk, s);
+    case 384: return launch_forces<T, 384>(c, io, tw, mk, s);
+    case 512: return launch_forces<T, 512>(c, io, tw, mk, s);
+    case 1024: return launch_forces<T, 1024>(c, io, tw, mk, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+template hipError_t forces_zpass<double>(const StepConsts&, const ForcesIO<double>&, R2R&, hipStream_t);
+template hipError_t forces_zpass<float>(const StepConsts&, const ForcesIO<float>&, R2R&, hipStream_t);
+
+}  // namespace smaiso

@@ -49,6 +49,11 @@ enum FuseOp : int {
   OP_COUNT = 17,
 };
 
+inline bool is_axis_op(int op) {
+  return op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP ||
+         op == OP_AX_GRAD_T || op == OP_AX_DIV_T;
+}
+
 struct Fuse {
   int op = OP_PLAIN;
   const void* src[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -90,6 +95,17 @@ class R2R {
   hipError_t prepare(DType dt, int n) {
     Tables* t = nullptr;
     return tables(dt, n, &t);
+  }
+  // Device twiddle tables of length n (cplx of the type of dt): tw[m] =
+  // exp(-2 pi i m/n), mk[m] = exp(-i pi m/(2n)); for kernels outside R2R
+  // that run an FFT pass of their own.
+  hipError_t table_ptrs(DType dt, int n, const void** tw, const void** mk) {
+    Tables* t = nullptr;
+    hipError_t e = tables(dt, n, &t);
+    if (e != hipSuccess) return e;
+    *tw = t->tw;
+    *mk = t->mk;
+    return hipSuccess;
   }
 
   // Autotuner: for each axis of geometry g whose length runs on the FFT

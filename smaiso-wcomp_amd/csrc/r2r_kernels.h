@@ -1030,11 +1030,6 @@ struct TileChoice {
   static constexpr int NTS = sizeof(T) == 8 ? 256 : 512;
 };
 
-inline bool is_axis_op(int op) {
-  return op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP ||
-         op == OP_AX_GRAD_T || op == OP_AX_DIV_T;
-}
-
 // One-axis operator pass (OP_AX_*) of FFT length n with the default tiles;
 // defined in r2r_axis_ops.hip.
 template <typename T>

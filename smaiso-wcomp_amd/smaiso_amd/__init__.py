@@ -33,7 +33,7 @@ FIELDS = {
     "rdx": 24, "rdy": 25, "rdz": 26, "lap_rl": 27, "psi_old": 28,
 }
 TILE_VARIANTS = 4
-KERNEL_CLASSES = {"r2r_z": 0, "r2r_y": 1, "r2r_x": 2, "spectral": 3, "pointwise": 4, "reduce": 5}
+KERNEL_CLASSES = {"r2r_z": 0, "r2r_y": 1, "r2r_x": 2, "spectral": 3, "pointwise": 4, "reduce": 5, "r2r_axis": 6}
 
 
 class SmaisoError(RuntimeError):
