@@ -13,7 +13,7 @@ template <typename T, int N>
 struct ForcesTile {
   static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
   static constexpr int NT = 256;
-  static constexpr size_t lds = (size_t)kForceOut * P * (N + 1) * sizeof(cplx<T>);
+  static constexpr size_t lds = (size_t)kForceOut * P * seq_len(N, pick_maxr(N, kForceOut * P, NT)) * sizeof(cplx<T>);
   static constexpr bool fits = lds <= 65536;
 };
 
@@ -27,8 +27,9 @@ template <typename T, int N, int P, int NT>
 __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> io,
                                                       const cplx<T>* __restrict__ tw,
                                                       const cplx<T>* __restrict__ mk) {
-  constexpr int NF = kForceOut, TL = 2 * P, SEQ = N + 1, HN = N / 2;
+  constexpr int NF = kForceOut, TL = 2 * P, HN = N / 2;
   constexpr int MAXR = pick_maxr(N, NF * P, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   T* rbuf = reinterpret_cast<T*>(smem_raw);
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       for (int r = 0; r < NF; ++r) {
         T x = (T)v[r];
         if (r == NF - 1 && (m & 1)) x = -x;
-        rbuf[2 * ((r * P + (l >> 1)) * SEQ + pos) + (l & 1)] = x;
+        rbuf[2 * ((r * P + (l >> 1)) * SEQ + lp<SH>(pos)) + (l & 1)] = x;
       }
     }
   }
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       const int r = sq / P, cc = sq - r * P;
       const int kb = (k == 0) ? HN : N - k;
       T y0a, y1a, y0b, y1b;
-      makhoul_post<T, N>(mk, k, kb, buf[sq * SEQ + k], buf[sq * SEQ + kb], y0a, y1a, y0b, y1b);
+      makhoul_post<T, N>(mk, k, kb, buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
       const bool sine = r == NF - 1;
       const int oa = sine ? N - 1 - k : k;
       const int ob = sine ? N - 1 - kb : kb;

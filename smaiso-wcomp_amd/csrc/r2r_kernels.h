@@ -193,6 +193,20 @@ constexpr int pick_maxr(int n, int p, int nt) {
   return (p * n / 16 >= nt) ? 16 : (p * n / 8 >= nt) ? 8 : 4;
 }
 
+// LDS layout of a sequence: item x sits at x + (x >> SH), one spare slot
+// after every 2^SH items, with SH = log2 of the plan's first radix R0.  The
+// first stage writes with a stride of R0 items across lanes and the later
+// stages in runs of NS; with the spare slots both map the lanes that LDS
+// serves together onto distinct banks (a plain layout puts every lane of a
+// stride-R0 write on the same few banks).  A sequence takes seq_len items,
+// the last one spare so that neighbouring sequences start on other banks.
+constexpr int pad_shift(int n, int maxr) {
+  return fft_radix(n, maxr, 0) >= 16 ? 4 : fft_radix(n, maxr, 0) >= 8 ? 3 : 2;
+}
+constexpr int seq_len(int n, int maxr) { return n + (n >> pad_shift(n, maxr)) + 1; }
+template <int SH>
+__device__ __forceinline__ int lp(int x) { return x + (x >> SH); }
+
 template <typename T>
 struct PassArgs {
   const T* in;
@@ -247,9 +261,10 @@ __device__ __forceinline__ TileGeo tile_geometry(int axis, int nx, int ny, int n
   return t;
 }
 
-// Sequence p of the tile lives at buf[p * SEQ], SEQ = N + 1 complex values:
-// the pad keeps lanes that walk across sequences off a single LDS bank.
-template <typename T, int N, int P, int NT, int MAXR, int SIGN, int S, int SEQ = N + 1>
+// Sequence p of the tile lives at buf[p * SEQ], item x of it at
+// buf[p * SEQ + lp<SH>(x)] (SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR)).
+template <typename T, int N, int P, int NT, int MAXR, int SIGN, int S, int SEQ = seq_len(N, MAXR),
+          int SH = pad_shift(N, MAXR)>
 __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
   constexpr int R = fft_radix(N, MAXR, S);
   if constexpr (R != 0) {
@@ -267,7 +282,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
         const int p = w / J;
         const int j = w - p * J;
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + j + q * J];
+        for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + lp<SH>(j + q * J)];
         if constexpr (NS > 1) {
           const int k = j % NS;
 #pragma unroll
@@ -290,11 +305,11 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
         const int k = j % NS;
         const int idx = (j / NS) * NS * R + k;
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[p * SEQ + idx + q * NS] = v[it][q];
+        for (int q = 0; q < R; ++q) buf[p * SEQ + lp<SH>(idx + q * NS)] = v[it][q];
       }
     }
     __syncthreads();
-    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1, SEQ>(buf, tw);
+    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1, SEQ, SH>(buf, tw);
   }
 }
 
@@ -522,7 +537,8 @@ template <typename T, int N, int P, int NT, int OP>
 __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   constexpr int TL = 2 * P;
   constexpr int MAXR = pick_maxr(N, P, NT);
-  constexpr int SEQ = N + 1;
+  constexpr int SEQ = seq_len(N, MAXR);
+  constexpr int SH = pad_shift(N, MAXR);
   constexpr int HN = N / 2;
   constexpr bool MID = (OP == OP_MID_AMP || OP == OP_MID_NEGLAP);
   constexpr bool AX = (OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GAUSS_AMP ||
@@ -532,7 +548,6 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   constexpr int AXOP = OP == OP_AX_GAUSS_AMP ? OP_AX_GAUSS : OP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
-  T* rbuf = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x;
   TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
   const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
@@ -547,30 +562,38 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
     // ---- load: every global load of the tile is issued before the first LDS
     // write, so each thread keeps all of its loads in flight ---------------
     if (a.axis == 2) {
-      constexpr int E = TL * N;
+      // item e = (line pair c, element m): lanes walk m along the two lines
+      // of a pair, and each item stores one complex value to LDS
+      constexpr int E = P * N;
       constexpr int IT = (E + NT - 1) / NT;
-      T v[IT];
+      cplx<T> v[IT];
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
-        const int l = e / N, m = e - (e / N) * N;
-        v[it] = T(0);
-        if ((E % NT == 0 || e < E) && l < g.nvalid) {
+        const int c = e / N, m = e - (e / N) * N;
+        v[it] = cplx<T>{T(0), T(0)};
+        if (E % NT == 0 || e < E) {
+          const int l0 = 2 * c;
+          const long long gl = (long long)g.outer + l0;
           int i, j;
-          const long long gl = (long long)g.outer + l;
           line_ij(gl, a.ny, i, j);
-          v[it] = load_elem<T, LD_OP>(a, i, j, m, mo + gl * N + m);
+          if (l0 < g.nvalid) v[it].x = load_elem<T, LD_OP>(a, i, j, m, mo + gl * N + m);
+          if (l0 + 1 < g.nvalid) {
+            int i1 = i, j1 = j + 1;
+            if (j1 == a.ny) { j1 = 0; ++i1; }
+            v[it].y = load_elem<T, LD_OP>(a, i1, j1, m, mo + (gl + 1) * N + m);
+          }
         }
       }
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
         if (E % NT == 0 || e < E) {
-          const int l = e / N, m = e - (e / N) * N;
-          T x = v[it];
-          if (sine && (m & 1)) x = -x;
+          const int c = e / N, m = e - (e / N) * N;
+          cplx<T> x = v[it];
+          if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
           const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-          rbuf[2 * ((l >> 1) * SEQ + pos) + (l & 1)] = x;
+          buf[c * SEQ + lp<SH>(pos)] = x;
         }
       }
     } else {
@@ -590,7 +613,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           cplx<T> x = v[it];
           if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
           const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-          buf[c * SEQ + pos] = x;
+          buf[c * SEQ + lp<SH>(pos)] = x;
         }
       }
     }
@@ -643,8 +666,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
         const int kb = (k == 0) ? HN : N - k;
         cplx<T> zk, zb;
         makhoul_pre<T, N>(a.mk, k, rk[it], rn[it], zk, zb);
-        buf[c * SEQ + k] = zk;
-        buf[c * SEQ + kb] = zb;
+        buf[c * SEQ + lp<SH>(k)] = zk;
+        buf[c * SEQ + lp<SH>(kb)] = zb;
       }
     }
   }
@@ -666,7 +689,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
           const int jj = g.outer;
           const int kc0 = g.k0 + 2 * c < a.nz ? g.k0 + 2 * c : a.nz - 1;
@@ -675,8 +698,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           const cplx<T> rn = {mid_apply<T, OP>(a.f, kb, jj, kc0, y0b), mid_apply<T, OP>(a.f, kb, jj, kc1, y1b)};
           cplx<T> zk, zb;
           makhoul_pre<T, N>(a.mk, k, rk, rn, zk, zb);
-          buf[c * SEQ + k] = zk;
-          buf[c * SEQ + kb] = zb;
+          buf[c * SEQ + lp<SH>(k)] = zk;
+          buf[c * SEQ + lp<SH>(kb)] = zb;
         }
       }
       __syncthreads();
@@ -693,7 +716,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           ra[it] = cplx<T>{y0a, y1a};
           rb[it] = cplx<T>{y0b, y1b};
         }
@@ -707,8 +730,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          buf[c * SEQ + (sine ? N - 1 - k : k)] = ra[it];
-          buf[c * SEQ + (sine ? N - 1 - kb : kb)] = rb[it];
+          buf[c * SEQ + lp<SH>((sine ? N - 1 - k : k))] = ra[it];
+          buf[c * SEQ + lp<SH>((sine ? N - 1 - kb : kb))] = rb[it];
         }
       }
       __syncthreads();
@@ -728,12 +751,12 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           const int sa = ax_source<AXOP>(pa, N), sb = ax_source<AXOP>(pb, N);
           if (sa >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pa, N);
-            const cplx<T> y = buf[c * SEQ + sa];
+            const cplx<T> y = buf[c * SEQ + lp<SH>(sa)];
             xk = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
           }
           if (sb >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pb, N);
-            const cplx<T> y = buf[c * SEQ + sb];
+            const cplx<T> y = buf[c * SEQ + lp<SH>(sb)];
             xb = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
           }
           makhoul_pre<T, N>(a.mk, k, xk, xb, ra[it], rb[it]);
@@ -748,8 +771,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          buf[c * SEQ + k] = ra[it];
-          buf[c * SEQ + kb] = rb[it];
+          buf[c * SEQ + lp<SH>(k)] = ra[it];
+          buf[c * SEQ + lp<SH>(kb)] = rb[it];
         }
       }
       __syncthreads();
@@ -764,7 +787,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + k], buf[c * SEQ + kb], y0a, y1a, y0b, y1b);
+          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           const int oa = sine ? (N - 1 - k) : k;
           const int ob = sine ? (N - 1 - kb) : kb;
           const int l0 = 2 * c;
@@ -802,8 +825,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
       int c, n;
       if (a.axis == 2) { c = w / HN; n = w - c * HN; }
       else { n = w / P; c = w - n * P; }
-      const cplx<T> ua = buf[c * SEQ + n];
-      const cplx<T> ub = buf[c * SEQ + N - 1 - n];
+      const cplx<T> ua = buf[c * SEQ + lp<SH>(n)];
+      const cplx<T> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
       const int l0 = 2 * c;
       if (a.axis == 2) {
 #pragma unroll
@@ -1006,7 +1029,7 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
-  const size_t lds = (size_t)P * (N + 1) * sizeof(cplx<T>);
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<T>);
   hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
@@ -1059,7 +1082,7 @@ template <typename T> struct TileVariant<T, false, 3> {
 // thread holds at most 256 bytes of the tile in registers.
 template <typename T, int N, int P, int NT>
 constexpr bool tile_fits() {
-  return (long long)P * (N + 1) * (long long)sizeof(cplx<T>) <= 65536 &&
+  return (long long)P * seq_len(N, pick_maxr(N, P, NT)) * (long long)sizeof(cplx<T>) <= 65536 &&
          (2LL * P * N / NT) * (long long)sizeof(T) <= 256 && NT % 64 == 0 && NT <= 1024;
 }
 
