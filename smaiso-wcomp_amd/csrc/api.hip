@@ -493,8 +493,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
     h->out64 = static_cast<double*>(tmpp);
   }
   for (int a = 0; a < 3; ++a) {
-    if ((e = h->r2r.prepare(F64, nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
-    if (h->mixed && (e = h->r2r.prepare(F32, nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
+    if ((e = h->r2r.prepare(nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
   }
   *out = h.release();
   return SMAISO_OK;

@@ -13,7 +13,7 @@ template <typename T, int N>
 struct ForcesTile {
   static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
   static constexpr int NT = 256;
-  static constexpr size_t lds = (size_t)kForceOut * P * seq_len(N, pick_maxr(N, kForceOut * P, NT)) * sizeof(cplx<T>);
+  static constexpr size_t lds = (size_t)kForceOut * P * seq_len(N, pick_maxr(N, kForceOut * P, NT)) * sizeof(cplx<double>);
   static constexpr bool fits = lds <= 65536;
 };
 
@@ -23,17 +23,18 @@ struct ForcesTile {
 // sequences in LDS (field r, line pair c -> sequence r P + c).  Then one
 // FFT over the 5P sequences and Makhoul's post-processing store the z pass
 // outputs (S_z is a DST-II along z: odd inputs negated, outputs reversed).
+// The tile and the FFT are FP64 for both storage types, as in the r2r passes.
 template <typename T, int N, int P, int NT>
 __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> io,
-                                                      const cplx<T>* __restrict__ tw,
-                                                      const cplx<T>* __restrict__ mk) {
+                                                      const cplx<double>* __restrict__ tw,
+                                                      const cplx<double>* __restrict__ mk) {
   constexpr int NF = kForceOut, TL = 2 * P, HN = N / 2;
   constexpr int MAXR = pick_maxr(N, NF * P, NT);
   constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
-  T* rbuf = reinterpret_cast<T*>(smem_raw);
   using A = double;
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
+  A* rbuf = reinterpret_cast<A*>(smem_raw);
   const int tid = threadIdx.x;
   const long long nl = (long long)c.nx * c.ny;
   const long long first = (long long)blockIdx.x * TL;
@@ -72,14 +73,14 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
 #pragma unroll
       for (int r = 0; r < NF; ++r) {
-        T x = (T)v[r];
+        A x = v[r];
         if (r == NF - 1 && (m & 1)) x = -x;
         rbuf[2 * ((r * P + (l >> 1)) * SEQ + lp<SH>(pos)) + (l & 1)] = x;
       }
     }
   }
   __syncthreads();
-  fft_stage<T, N, NF * P, NT, MAXR, -1, 0>(buf, tw);
+  fft_stage<A, N, NF * P, NT, MAXR, -1, 0>(buf, tw);
 
   constexpr int W = NF * P * HN;
   constexpr int ITW = (W + NT - 1) / NT;
@@ -90,8 +91,8 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       const int sq = w / HN, k = w - sq * HN;
       const int r = sq / P, cc = sq - r * P;
       const int kb = (k == 0) ? HN : N - k;
-      T y0a, y1a, y0b, y1b;
-      makhoul_post<T, N>(mk, k, kb, buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+      A y0a, y1a, y0b, y1b;
+      makhoul_post<A, N>(mk, k, kb, buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
       const bool sine = r == NF - 1;
       const int oa = sine ? N - 1 - k : k;
       const int ob = sine ? N - 1 - kb : kb;
@@ -99,13 +100,13 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       const int l0 = 2 * cc;
       if (l0 < nvalid) {
         T* q = o + base + (long long)l0 * N;
-        q[oa] = y0a;
-        q[ob] = y0b;
+        q[oa] = (T)y0a;
+        q[ob] = (T)y0b;
       }
       if (l0 + 1 < nvalid) {
         T* q = o + base + (long long)(l0 + 1) * N;
-        q[oa] = y1a;
-        q[ob] = y1b;
+        q[oa] = (T)y1a;
+        q[ob] = (T)y1b;
       }
     }
   }
@@ -121,8 +122,8 @@ hipError_t launch_forces(const StepConsts& c, const ForcesIO<T>& io, const void*
     constexpr int TL = 2 * F::P;
     const long long blocks = ((long long)c.nx * c.ny + TL - 1) / TL;
     hipLaunchKernelGGL((forces_z_kernel<T, N, F::P, F::NT>), dim3((unsigned)blocks, (unsigned)c.nmem),
-                       dim3(F::NT), F::lds, s, c, io, static_cast<const cplx<T>*>(tw),
-                       static_cast<const cplx<T>*>(mk));
+                       dim3(F::NT), F::lds, s, c, io, static_cast<const cplx<double>*>(tw),
+                       static_cast<const cplx<double>*>(mk));
     return hipGetLastError();
   }
 }
@@ -156,7 +157,7 @@ hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hi
   const DType dt = sizeof(T) == 8 ? F64 : F32;
   if (!forces_zpass_supported(dt, c.nz)) return hipErrorNotSupported;
   const void *tw = nullptr, *mk = nullptr;
-  hipError_t e = r2r.table_ptrs(dt, c.nz, &tw, &mk);
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
   if (e != hipSuccess) return e;
   switch (c.nz) {
     case 16: return launch_forces<T, 16>(c, io, tw, mk, s);

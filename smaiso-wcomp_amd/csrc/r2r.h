@@ -90,18 +90,20 @@ class R2R {
                          hipStream_t s);
   // True when length n runs on the LDS FFT path (else the direct O(n^2) path).
   static bool fft_supported(int n);
-  // Build the twiddle tables of length n now (they are otherwise built on
-  // first use, with synchronous copies that a stream capture cannot hold).
-  hipError_t prepare(DType dt, int n) {
+  // Build the (FP64) twiddle tables of length n now (they are otherwise
+  // built on first use, with synchronous copies that a stream capture cannot
+  // hold).
+  hipError_t prepare(int n) {
     Tables* t = nullptr;
-    return tables(dt, n, &t);
+    return tables(F64, n, &t);
   }
-  // Device twiddle tables of length n (cplx of the type of dt): tw[m] =
-  // exp(-2 pi i m/n), mk[m] = exp(-i pi m/(2n)); for kernels outside R2R
-  // that run an FFT pass of their own.
-  hipError_t table_ptrs(DType dt, int n, const void** tw, const void** mk) {
+  // Device twiddle tables of length n (FP64 complex; the passes compute in
+  // FP64 for both storage types): tw[m] = exp(-2 pi i m/n), mk[m] =
+  // exp(-i pi m/(2n)); for kernels outside R2R that run an FFT pass of their
+  // own.
+  hipError_t table_ptrs(int n, const void** tw, const void** mk) {
     Tables* t = nullptr;
-    hipError_t e = tables(dt, n, &t);
+    hipError_t e = tables(F64, n, &t);
     if (e != hipSuccess) return e;
     *tw = t->tw;
     *mk = t->mk;

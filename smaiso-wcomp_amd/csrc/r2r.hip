@@ -87,16 +87,16 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
                                 const Geom& g, hipStream_t s) {
   DirectArgs<T> d;
   d.p = make_args<T>(io, nullptr, nullptr, kind, axis, g);
-  d.costab = static_cast<const T*>(costab);
+  d.costab = static_cast<const double*>(costab);
   d.n = n;
   const bool mid = io.f.op == OP_MID_AMP || io.f.op == OP_MID_NEGLAP || is_axis_op(io.f.op);
-  int tl = 16;
-  while (tl > 1 && (size_t)tl * n * sizeof(T) * (mid ? 2 : 1) > 48 * 1024) tl /= 2;
+  int tl = 16;  // lines per tile; the LDS tile is FP64
+  while (tl > 1 && (size_t)tl * n * sizeof(double) * (mid ? 2 : 1) > 48 * 1024) tl /= 2;
   d.tl = tl;
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + tl - 1) / tl;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + tl - 1) / tl);
-  const size_t lds = (size_t)tl * n * sizeof(T) * (mid ? 2 : 1);
+  const size_t lds = (size_t)tl * n * sizeof(double) * (mid ? 2 : 1);
   hipLaunchKernelGGL((direct_pass_kernel<T>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(kThreads), lds, s, d);
   return hipGetLastError();
 }
@@ -135,7 +135,7 @@ hipError_t R2R::pass(DType dt, int kind, int axis, const PassIO& io, const Geom&
   if (!op_valid(io.f.op, axis)) return hipErrorInvalidValue;
   const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
   Tables* tb = nullptr;
-  hipError_t e = tables(dt, n, &tb);
+  hipError_t e = tables(F64, n, &tb);  // FP64 tables for both storage types
   if (e != hipSuccess) return e;
   if (io.f.op == OP_PLAIN && !tuned_.empty()) {
     const auto it = tuned_.find(tune_key(dt, axis, g));

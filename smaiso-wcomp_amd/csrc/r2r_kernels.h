@@ -21,6 +21,12 @@
 //    and store index maps.
 //  * The complex DFT_N is a self-sorting Stockham FFT in LDS with radix 2, 3,
 //    4, 8 and 16 butterflies in registers and a per-N twiddle table.
+//  * The tile, the twiddles and every operation on them are FP64 for both
+//    storage types; a pass rounds to the storage type only when it stores.
+//    An FP32 FFT's rounding is relative to the norm of each line and swamps
+//    the small high modes that the spectral operators then amplify, while
+//    rounding a stored value is relative to that value.  (Without packed
+//    FP32 instructions, FP64 VALU arithmetic costs what FP32 costs.)
 //  * Lengths outside the FFT list use a direct O(N^2) LDS kernel that
 //    evaluates FFTW's defining sums with a cosine table.
 #pragma once
@@ -212,8 +218,8 @@ struct PassArgs {
   const T* in;
   const T* in2;
   T* out;
-  const cplx<T>* tw;
-  const cplx<T>* mk;
+  const cplx<double>* tw;  // FP64 tables for both storage types
+  const cplx<double>* mk;
   int nx, ny, nz;
   int axis;
   int kind;
@@ -345,22 +351,23 @@ __device__ __forceinline__ void line_ij(long long g, int ny, int& i, int& j) {
 }
 
 // Element m of z-line g (= i*ny + j) of the first pass under load op OP;
-// idx = g*nz + m.
+// idx = g*nz + m.  The value is formed in FP64 and goes to the FP64 tile as
+// it is: a pass rounds to the storage type T only when it stores.
 template <typename T, int OP>
-__device__ __forceinline__ T load_elem(const PassArgs<T>& a, int i, int j, int m, long long idx) {
+__device__ __forceinline__ double load_elem(const PassArgs<T>& a, int i, int j, int m, long long idx) {
   if constexpr (OP == OP_PLAIN) {
-    T v = a.in[idx];
-    if (a.in2) v += a.in2[idx];
+    double v = (double)a.in[idx];
+    if (a.in2) v += (double)a.in2[idx];
     return v;
   } else if constexpr (OP == OP_LD_PROD) {
     // rho (FP64 state) times L psi
-    return (T)(static_cast<const double*>(a.f.src[0])[idx] * (double)static_cast<const T*>(a.f.src[1])[idx]);
+    return static_cast<const double*>(a.f.src[0])[idx] * (double)static_cast<const T*>(a.f.src[1])[idx];
   } else if constexpr (OP == OP_LD_AMP) {
-    const T ps = static_cast<const T*>(a.f.src[0])[idx];
-    const T x = static_cast<const T*>(a.f.src[1])[idx];
-    const T y = static_cast<const T*>(a.f.src[2])[idx];
-    const T z = static_cast<const T*>(a.f.src[3])[idx];
-    return sqrt((T)a.f.q02 * (ps * ps) + x * x + y * y + z * z);
+    const double ps = (double)static_cast<const T*>(a.f.src[0])[idx];
+    const double x = (double)static_cast<const T*>(a.f.src[1])[idx];
+    const double y = (double)static_cast<const T*>(a.f.src[2])[idx];
+    const double z = (double)static_cast<const T*>(a.f.src[3])[idx];
+    return sqrt(a.f.q02 * (ps * ps) + x * x + y * y + z * z);
   } else if constexpr (OP == OP_LD_GRAD_D || OP == OP_LD_GRAD_T) {
     // branch-free: the load always happens (at a clamped address) so that the
     // compiler can keep every load of the tile in flight
@@ -374,12 +381,12 @@ __device__ __forceinline__ T load_elem(const PassArgs<T>& a, int i, int j, int m
     double src;
     if constexpr (OP == OP_LD_GRAD_D) src = static_cast<const double*>(a.f.src[0])[src_idx];
     else src = (double)static_cast<const T*>(a.f.src[0])[src_idx];
-    return ok ? (T)((-a.f.c0 * w) * src) : T(0);
+    return ok ? (-a.f.c0 * w) * src : 0.0;
   } else if constexpr (OP == OP_LD_SH || OP == OP_LD_RELAXLAP) {
     const double m2 = mq2c(a.f, i, j, m);
     const double ps = static_cast<const double*>(a.f.src[0])[idx];
-    if constexpr (OP == OP_LD_SH) return (T)(a.f.alpha * (a.f.q02 - m2) * ps);
-    else return (T)(-m2 * a.f.alpha * (a.f.q02 - m2) * ps);
+    if constexpr (OP == OP_LD_SH) return a.f.alpha * (a.f.q02 - m2) * ps;
+    else return -m2 * a.f.alpha * (a.f.q02 - m2) * ps;
   } else if constexpr (OP == OP_LD_DIV) {
     const T* sx = static_cast<const T*>(a.f.src[0]);
     const T* sy = static_cast<const T*>(a.f.src[1]);
@@ -387,20 +394,20 @@ __device__ __forceinline__ T load_elem(const PassArgs<T>& a, int i, int j, int m
     const double vx = (double)sx[i > 0 ? idx - (long long)a.ny * a.nz : idx];
     const double vy = (double)sy[j > 0 ? idx - a.nz : idx];
     const double vz = (double)sz[m > 0 ? idx - 1 : idx];
-    return (T)(a.f.c0 * (a.f.vq[0][i] * (i > 0 ? vx : 0.0) + a.f.vq[1][j] * (j > 0 ? vy : 0.0) +
-                         a.f.vq[2][m] * (m > 0 ? vz : 0.0)));
+    return a.f.c0 * (a.f.vq[0][i] * (i > 0 ? vx : 0.0) + a.f.vq[1][j] * (j > 0 ? vy : 0.0) +
+                     a.f.vq[2][m] * (m > 0 ? vz : 0.0));
   } else if constexpr (OP == OP_LD_PSI) {
     // the new psiq is the transform's input and is also stored (over psiq_old)
     const double r = psi_update_value<T>(a.f, idx, i, j, m);
     a.f.psiq_new[idx] = r;
-    return (T)r;
+    return r;
   } else {
-    return T(0);
+    return 0.0;
   }
 }
 
 template <typename T>
-__device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo) {
+__device__ double load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo) {
   int i, j;
   line_ij(g, a.ny, i, j);
   const long long idx = mo + g * a.nz + m;
@@ -420,46 +427,50 @@ __device__ T load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo
 
 // Diagonal multiplier of the mid ops at mode (i, j, k); y is the forward
 // output, evaluated in the reference's order: (scale y) G, (-scale |q|^2) y.
-template <typename T, int OP>
-__device__ __forceinline__ T mid_apply(const Fuse& f, int i, int j, int k, T y) {
+template <int OP>
+__device__ __forceinline__ double mid_apply(const Fuse& f, int i, int j, int k, double y) {
   const double m2 = mq2c(f, i, j, k);
-  if constexpr (OP == OP_MID_AMP) return (T)(f.scale * (double)y * exp(-1.57 * 1.57 * m2 * 2));
-  else return (T)((-f.scale * m2) * (double)y);
+  if constexpr (OP == OP_MID_AMP) return f.scale * y * exp(-1.57 * 1.57 * m2 * 2);
+  else return (-f.scale * m2) * y;
 }
 
-// Adjacent columns l0, l0+1 of a strided tile at row m (zero outside the grid).
+// Adjacent columns l0, l0+1 of a strided tile at row m (zero outside the
+// grid), widened to FP64.
 template <typename T>
-__device__ __forceinline__ cplx<T> load_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m) {
+__device__ __forceinline__ cplx<double> load_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m) {
   const long long addr = g.base + l0 + (long long)m * g.stride_m;
-  cplx<T> v = {T(0), T(0)};
+  cplx<double> v = {0.0, 0.0};
   if (a.vec && l0 + 1 < g.nvalid) {
-    v = *reinterpret_cast<const cplx<T>*>(a.in + addr);
+    const cplx<T> u = *reinterpret_cast<const cplx<T>*>(a.in + addr);
+    v = cplx<double>{(double)u.x, (double)u.y};
     if (a.in2) {
       const cplx<T> w = *reinterpret_cast<const cplx<T>*>(a.in2 + addr);
-      v.x += w.x;
-      v.y += w.y;
+      v.x += (double)w.x;
+      v.y += (double)w.y;
     }
   } else {
-    if (l0 < g.nvalid) { v.x = a.in[addr]; if (a.in2) v.x += a.in2[addr]; }
-    if (l0 + 1 < g.nvalid) { v.y = a.in[addr + 1]; if (a.in2) v.y += a.in2[addr + 1]; }
+    if (l0 < g.nvalid) { v.x = (double)a.in[addr]; if (a.in2) v.x += (double)a.in2[addr]; }
+    if (l0 + 1 < g.nvalid) { v.y = (double)a.in[addr + 1]; if (a.in2) v.y += (double)a.in2[addr + 1]; }
   }
   return v;
 }
 
+// Store (y0, y1) at columns l0, l0+1 of row m, rounded to the storage type
+// (added to what is there when acc).
 template <typename T>
-__device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m, T y0,
-                                           T y1, bool acc = false) {
+__device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m, double y0,
+                                           double y1, bool acc = false) {
   T* o = a.out + g.base + l0 + (long long)m * g.stride_m;
   if (a.vec && l0 + 1 < g.nvalid) {
     if (acc) {
       const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
-      y0 += old.x;
-      y1 += old.y;
+      y0 += (double)old.x;
+      y1 += (double)old.y;
     }
-    *reinterpret_cast<cplx<T>*>(o) = cplx<T>{y0, y1};
+    *reinterpret_cast<cplx<T>*>(o) = cplx<T>{(T)y0, (T)y1};
   } else {
-    if (l0 < g.nvalid) o[0] = acc ? o[0] + y0 : y0;
-    if (l0 + 1 < g.nvalid) o[1] = acc ? o[1] + y1 : y1;
+    if (l0 < g.nvalid) o[0] = (T)(acc ? (double)o[0] + y0 : y0);
+    if (l0 + 1 < g.nvalid) o[1] = (T)(acc ? (double)o[1] + y1 : y1);
   }
 }
 
@@ -546,8 +557,10 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   // load op of the first (z) pass
   constexpr int LD_OP = OP == OP_AX_GAUSS_AMP ? OP_LD_AMP : ((MID || AX) ? OP_PLAIN : OP);
   constexpr int AXOP = OP == OP_AX_GAUSS_AMP ? OP_AX_GAUSS : OP;
+  // the tile and every operation on it are FP64 for both storage types
+  using A = double;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
   const int tid = threadIdx.x;
   TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
   const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
@@ -566,12 +579,12 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
       // of a pair, and each item stores one complex value to LDS
       constexpr int E = P * N;
       constexpr int IT = (E + NT - 1) / NT;
-      cplx<T> v[IT];
+      cplx<A> v[IT];
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
         const int c = e / N, m = e - (e / N) * N;
-        v[it] = cplx<T>{T(0), T(0)};
+        v[it] = cplx<A>{A(0), A(0)};
         if (E % NT == 0 || e < E) {
           const int l0 = 2 * c;
           const long long gl = (long long)g.outer + l0;
@@ -590,7 +603,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
         const int e = tid + it * NT;
         if (E % NT == 0 || e < E) {
           const int c = e / N, m = e - (e / N) * N;
-          cplx<T> x = v[it];
+          cplx<A> x = v[it];
           if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
           const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
           buf[c * SEQ + lp<SH>(pos)] = x;
@@ -599,18 +612,18 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
     } else {
       constexpr int E = P * N;  // pairs of adjacent columns
       constexpr int IT = (E + NT - 1) / NT;
-      cplx<T> v[IT];
+      cplx<A> v[IT];
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
-        v[it] = (E % NT == 0 || e < E) ? load_pair(a, g, 2 * (e % P), e / P) : cplx<T>{T(0), T(0)};
+        v[it] = (E % NT == 0 || e < E) ? load_pair(a, g, 2 * (e % P), e / P) : cplx<A>{A(0), A(0)};
       }
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
         if (E % NT == 0 || e < E) {
           const int c = e % P, m = e / P;
-          cplx<T> x = v[it];
+          cplx<A> x = v[it];
           if (sine && (m & 1)) { x.x = -x.x; x.y = -x.y; }
           const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
           buf[c * SEQ + lp<SH>(pos)] = x;
@@ -623,11 +636,11 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
     // lines of a pair.  DST-III reads the input reversed, Y_k = x_{N-1-k}.
     constexpr int W = P * HN;
     constexpr int ITW = (W + NT - 1) / NT;
-    cplx<T> rk[ITW], rn[ITW];
+    cplx<A> rk[ITW], rn[ITW];
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
       const int w = tid + it * NT;
-      rk[it] = rn[it] = cplx<T>{T(0), T(0)};
+      rk[it] = rn[it] = cplx<A>{A(0), A(0)};
       if (W % NT == 0 || w < W) {
         int c, k;
         if (a.axis == 2) { c = w / HN; k = w - c * HN; }
@@ -664,8 +677,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
         if (a.axis == 2) { c = w / HN; k = w - c * HN; }
         else { k = w / P; c = w - k * P; }
         const int kb = (k == 0) ? HN : N - k;
-        cplx<T> zk, zb;
-        makhoul_pre<T, N>(a.mk, k, rk[it], rn[it], zk, zb);
+        cplx<A> zk, zb;
+        makhoul_pre<A, N>(a.mk, k, rk[it], rn[it], zk, zb);
         buf[c * SEQ + lp<SH>(k)] = zk;
         buf[c * SEQ + lp<SH>(kb)] = zb;
       }
@@ -674,7 +687,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   __syncthreads();
 
   if (fwd) {
-    fft_stage<T, N, P, NT, MAXR, -1, 0>(buf, a.tw);
+    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, a.tw);
     constexpr int W = P * HN;
     constexpr int ITW = (W + NT - 1) / NT;
     if constexpr (MID) {
@@ -688,16 +701,16 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          A y0a, y1a, y0b, y1b;
+          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
           const int jj = g.outer;
           const int kc0 = g.k0 + 2 * c < a.nz ? g.k0 + 2 * c : a.nz - 1;
           const int kc1 = g.k0 + 2 * c + 1 < a.nz ? g.k0 + 2 * c + 1 : a.nz - 1;
-          const cplx<T> rk = {mid_apply<T, OP>(a.f, k, jj, kc0, y0a), mid_apply<T, OP>(a.f, k, jj, kc1, y1a)};
-          const cplx<T> rn = {mid_apply<T, OP>(a.f, kb, jj, kc0, y0b), mid_apply<T, OP>(a.f, kb, jj, kc1, y1b)};
-          cplx<T> zk, zb;
-          makhoul_pre<T, N>(a.mk, k, rk, rn, zk, zb);
+          const cplx<A> rk = {mid_apply<OP>(a.f, k, jj, kc0, y0a), mid_apply<OP>(a.f, k, jj, kc1, y1a)};
+          const cplx<A> rn = {mid_apply<OP>(a.f, kb, jj, kc0, y0b), mid_apply<OP>(a.f, kb, jj, kc1, y1b)};
+          cplx<A> zk, zb;
+          makhoul_pre<A, N>(a.mk, k, rk, rn, zk, zb);
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
@@ -706,7 +719,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
     } else if constexpr (AX) {
       // forward outputs of both lines -> registers -> LDS at their output
       // index; the operator reads shifted modes, so each step is fenced
-      cplx<T> ra[ITW], rb[ITW];
+      cplx<A> ra[ITW], rb[ITW];
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -715,10 +728,10 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
-          ra[it] = cplx<T>{y0a, y1a};
-          rb[it] = cplx<T>{y0b, y1b};
+          A y0a, y1a, y0b, y1b;
+          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          ra[it] = cplx<A>{y0a, y1a};
+          rb[it] = cplx<A>{y0b, y1b};
         }
       }
       __syncthreads();
@@ -747,19 +760,19 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           const int kb = (k == 0) ? HN : N - k;
           const int pa = sine_out ? N - 1 - k : k;
           const int pb = sine_out ? N - 1 - kb : kb;
-          cplx<T> xk = {T(0), T(0)}, xb = {T(0), T(0)};
+          cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
           const int sa = ax_source<AXOP>(pa, N), sb = ax_source<AXOP>(pb, N);
           if (sa >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pa, N);
-            const cplx<T> y = buf[c * SEQ + lp<SH>(sa)];
-            xk = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
+            const cplx<A> y = buf[c * SEQ + lp<SH>(sa)];
+            xk = cplx<A>{wgt * y.x, wgt * y.y};
           }
           if (sb >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pb, N);
-            const cplx<T> y = buf[c * SEQ + lp<SH>(sb)];
-            xb = cplx<T>{(T)(wgt * (double)y.x), (T)(wgt * (double)y.y)};
+            const cplx<A> y = buf[c * SEQ + lp<SH>(sb)];
+            xb = cplx<A>{wgt * y.x, wgt * y.y};
           }
-          makhoul_pre<T, N>(a.mk, k, xk, xb, ra[it], rb[it]);
+          makhoul_pre<A, N>(a.mk, k, xk, xb, ra[it], rb[it]);
         }
       }
       __syncthreads();
@@ -786,21 +799,21 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          T y0a, y1a, y0b, y1b;
-          makhoul_post<T, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          A y0a, y1a, y0b, y1b;
+          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           const int oa = sine ? (N - 1 - k) : k;
           const int ob = sine ? (N - 1 - kb) : kb;
           const int l0 = 2 * c;
           if (a.axis == 2) {
             if (l0 < g.nvalid) {
               T* o = a.out + g.base + (long long)l0 * N;
-              o[oa] = y0a;
-              o[ob] = y0b;
+              o[oa] = (T)y0a;
+              o[ob] = (T)y0b;
             }
             if (l0 + 1 < g.nvalid) {
               T* o = a.out + g.base + (long long)(l0 + 1) * N;
-              o[oa] = y1a;
-              o[ob] = y1b;
+              o[oa] = (T)y1a;
+              o[ob] = (T)y1b;
             }
           } else {
             store_pair(a, g, l0, oa, y0a, y1a);
@@ -813,10 +826,10 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
   }
 
   // ---- inverse FFT and store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n}
-  fft_stage<T, N, P, NT, MAXR, +1, 0>(buf, a.tw);
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, a.tw);
   constexpr int WO = P * HN;
   constexpr int ITO = (WO + NT - 1) / NT;
-  const T s = sine_out ? T(-1) : T(1);
+  const A s = sine_out ? A(-1) : A(1);
   const bool acc = AX && a.f.accumulate;
 #pragma unroll
   for (int it = 0; it < ITO; ++it) {
@@ -825,27 +838,27 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
       int c, n;
       if (a.axis == 2) { c = w / HN; n = w - c * HN; }
       else { n = w / P; c = w - n * P; }
-      const cplx<T> ua = buf[c * SEQ + lp<SH>(n)];
-      const cplx<T> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
+      const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
       const int l0 = 2 * c;
       if (a.axis == 2) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (l0 + q < g.nvalid) {
             T* o = a.out + g.base + (long long)(l0 + q) * N + 2 * n;
-            T v0 = q == 0 ? ua.x : ua.y;
-            T v1 = s * (q == 0 ? ub.x : ub.y);
+            A v0 = q == 0 ? ua.x : ua.y;
+            A v1 = s * (q == 0 ? ub.x : ub.y);
             if (a.vec) {
               if (acc) {
                 const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
-                v0 += old.x;
-                v1 += old.y;
+                v0 += (A)old.x;
+                v1 += (A)old.y;
               }
-              *reinterpret_cast<cplx<T>*>(o) = cplx<T>{v0, v1};
+              *reinterpret_cast<cplx<T>*>(o) = cplx<T>{(T)v0, (T)v1};
             } else {
-              if (acc) { v0 += o[0]; v1 += o[1]; }
-              o[0] = v0;
-              o[1] = v1;
+              if (acc) { v0 += (A)o[0]; v1 += (A)o[1]; }
+              o[0] = (T)v0;
+              o[1] = (T)v1;
             }
           }
         }
@@ -865,15 +878,15 @@ constexpr int kThreads = 256;
 template <typename T>
 struct DirectArgs {
   PassArgs<T> p;
-  const T* costab;  // 4n entries: cos(pi t/(2n))
+  const double* costab;  // 4n entries: cos(pi t/(2n))
   int n;
   int tl;
 };
 
 // FFTW's defining sum of `kind` at output k for one line xl of length n; the
 // angle index t (units of pi/(2n)) is advanced incrementally mod 4n.
-template <typename T>
-__device__ T direct_sum(int kind, const T* xl, int n, int k, const T* __restrict__ ct) {
+__device__ inline double direct_sum(int kind, const double* xl, int n, int k, const double* __restrict__ ct) {
+  using T = double;
   const int n4 = 4 * n;
   T acc = T(0);
   if (kind == REDFT10) {            // 2 sum x_m cos(pi (2m+1) k / 2n)
@@ -902,8 +915,9 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const PassArgs<T>& a = d.p;
   const int n = d.n, tl = d.tl;
-  T* x = reinterpret_cast<T*>(smem_raw);
-  T* y = x + tl * n;  // second buffer of the mid ops
+  // FP64 tile for both storage types
+  double* x = reinterpret_cast<double*>(smem_raw);
+  double* y = x + tl * n;  // second buffer of the mid ops
   const bool mid = a.f.op == OP_MID_AMP || a.f.op == OP_MID_NEGLAP;
   const int op = a.f.op;
   const bool ax = op == OP_AX_GRAD || op == OP_AX_DIV || op == OP_AX_GAUSS || op == OP_AX_GAUSS_AMP ||
@@ -938,14 +952,14 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     int l, m;
     if (a.axis == 2) { l = e / n; m = e - l * n; }
     else { m = e / tl; l = e - m * tl; }
-    T v = T(0);
+    double v = 0.0;
     if (l < g.nvalid) {
       if (a.axis == 2) {
         v = load_elem_rt(a, (long long)g.outer + l, m, mo);
       } else {
         const long long addr = g.base + l * g.line_step + m * g.stride_m;
-        v = a.in[addr];
-        if (a.in2) v += a.in2[addr];
+        v = (double)a.in[addr];
+        if (a.in2) v += (double)a.in2[addr];
       }
     }
     x[l * n + m] = v;
@@ -956,23 +970,23 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     const int fk = ax_fsine ? RODFT10 : REDFT10;
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
       const int k = e / tl, l = e - (e / tl) * tl;
-      if (l < g.nvalid) y[l * n + k] = direct_sum<T>(fk, x + l * n, n, k, d.costab);
+      if (l < g.nvalid) y[l * n + k] = direct_sum(fk, x + l * n, n, k, d.costab);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
       const int m = e / tl, l = e - (e / tl) * tl;
       if (l >= g.nvalid) continue;
-      T v = T(0);
+      double v = 0.0;
       if (op == OP_AX_GRAD) {
-        if (ax_source<OP_AX_GRAD>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_GRAD>(a.f, m, n) * (double)y[l * n + m + 1]);
+        if (ax_source<OP_AX_GRAD>(m, n) >= 0) v = (ax_weight<T, OP_AX_GRAD>(a.f, m, n) * y[l * n + m + 1]);
       } else if (op == OP_AX_DIV) {
-        if (ax_source<OP_AX_DIV>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_DIV>(a.f, m, n) * (double)y[l * n + m - 1]);
+        if (ax_source<OP_AX_DIV>(m, n) >= 0) v = (ax_weight<T, OP_AX_DIV>(a.f, m, n) * y[l * n + m - 1]);
       } else if (op == OP_AX_GRAD_T) {
-        if (ax_source<OP_AX_GRAD_T>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_GRAD_T>(a.f, m, n) * (double)y[l * n + m - 1]);
+        if (ax_source<OP_AX_GRAD_T>(m, n) >= 0) v = (ax_weight<T, OP_AX_GRAD_T>(a.f, m, n) * y[l * n + m - 1]);
       } else if (op == OP_AX_DIV_T) {
-        if (ax_source<OP_AX_DIV_T>(m, n) >= 0) v = (T)(ax_weight<T, OP_AX_DIV_T>(a.f, m, n) * (double)y[l * n + m + 1]);
+        if (ax_source<OP_AX_DIV_T>(m, n) >= 0) v = (ax_weight<T, OP_AX_DIV_T>(a.f, m, n) * y[l * n + m + 1]);
       } else {
-        v = (T)(ax_weight<T, OP_AX_GAUSS>(a.f, m, n) * (double)y[l * n + m]);
+        v = (ax_weight<T, OP_AX_GAUSS>(a.f, m, n) * y[l * n + m]);
       }
       x[l * n + m] = v;
     }
@@ -981,10 +995,10 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     for (int e = threadIdx.x; e < E; e += blockDim.x) {
       const int k = e / tl, l = e - (e / tl) * tl;
       if (l >= g.nvalid) continue;
-      const T v = direct_sum<T>(REDFT10, x + l * n, n, k, d.costab);
+      const double v = direct_sum(REDFT10, x + l * n, n, k, d.costab);
       const int kc = g.k0 + l;
-      y[l * n + k] = (a.f.op == OP_MID_AMP) ? mid_apply<T, OP_MID_AMP>(a.f, k, g.outer, kc, v)
-                                            : mid_apply<T, OP_MID_NEGLAP>(a.f, k, g.outer, kc, v);
+      y[l * n + k] = (a.f.op == OP_MID_AMP) ? mid_apply<OP_MID_AMP>(a.f, k, g.outer, kc, v)
+                                            : mid_apply<OP_MID_NEGLAP>(a.f, k, g.outer, kc, v);
     }
     __syncthreads();
   }
@@ -993,12 +1007,12 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     if (a.axis == 2) { l = e / n; k = e - l * n; }
     else { k = e / tl; l = e - k * tl; }
     if (l >= g.nvalid) continue;
-    T r;
-    if (ax) r = direct_sum<T>(ax_isine ? RODFT01 : REDFT01, x + l * n, n, k, d.costab);
-    else if (mid) r = direct_sum<T>(REDFT01, y + l * n, n, k, d.costab);
-    else r = direct_sum<T>(a.kind, x + l * n, n, k, d.costab);
+    double r;
+    if (ax) r = direct_sum(ax_isine ? RODFT01 : REDFT01, x + l * n, n, k, d.costab);
+    else if (mid) r = direct_sum(REDFT01, y + l * n, n, k, d.costab);
+    else r = direct_sum(a.kind, x + l * n, n, k, d.costab);
     const long long oi = g.base + l * g.line_step + k * g.stride_m;
-    a.out[oi] = (ax && a.f.accumulate) ? a.out[oi] + r : r;
+    a.out[oi] = (T)((ax && a.f.accumulate) ? (double)a.out[oi] + r : r);
   }
 }
 
@@ -1010,8 +1024,8 @@ PassArgs<T> make_args(const PassIO& io, void* tw, void* mk, int kind, int axis, 
   a.in = static_cast<const T*>(io.in);
   a.in2 = static_cast<const T*>(io.in2);
   a.out = static_cast<T*>(io.out);
-  a.tw = static_cast<const cplx<T>*>(tw);
-  a.mk = static_cast<const cplx<T>*>(mk);
+  a.tw = static_cast<const cplx<double>*>(tw);
+  a.mk = static_cast<const cplx<double>*>(mk);
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
   a.axis = axis;
   a.kind = kind;
@@ -1029,13 +1043,16 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
-  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<T>);
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
   hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 
 
-// Tile and block size per (type, length, axis).
+// Tile and block size per (length, axis); the LDS tile is FP64 for both
+// storage types, so T does not change the choice.  Measured at 256^3 on
+// MI355X: one wavefront per 2 line pairs along z; along y and x 8 pairs x 256
+// threads.
 template <typename T, int N>
 struct TileChoice {
   static constexpr int cap(int p) { return p < 1 ? 1 : p; }
@@ -1044,13 +1061,10 @@ struct TileChoice {
     while (q * 2 <= p) q *= 2;
     return q;
   }
-  // measured at 256^3 on MI355X: one wavefront per 2 line pairs along z;
-  // along y and x 8 pairs x 256 threads (FP64) and 16 pairs x 512 threads (FP32)
   static constexpr int PZ = cap(budget(512) < 2 ? budget(512) : 2);
-  static constexpr int PS0 = sizeof(T) == 8 ? 8 : 16;
-  static constexpr int PS = cap(budget(2048 * (8 / (int)sizeof(T))) < PS0 ? budget(2048 * (8 / (int)sizeof(T))) : PS0);
+  static constexpr int PS = cap(budget(2048) < 8 ? budget(2048) : 8);
   static constexpr int NTZ = 64;
-  static constexpr int NTS = sizeof(T) == 8 ? 256 : 512;
+  static constexpr int NTS = 256;
 };
 
 // One-axis operator pass (OP_AX_*) of FFT length n with the default tiles;
@@ -1068,22 +1082,16 @@ struct TileVariant;
 template <typename T> struct TileVariant<T, true, 1> { static constexpr int P = 1, NT = 64; };
 template <typename T> struct TileVariant<T, true, 2> { static constexpr int P = 2, NT = 128; };
 template <typename T> struct TileVariant<T, true, 3> { static constexpr int P = 4, NT = 128; };
-template <typename T> struct TileVariant<T, false, 1> {
-  static constexpr int P = sizeof(T) == 8 ? 4 : 8, NT = sizeof(T) == 8 ? 128 : 256;
-};
-template <typename T> struct TileVariant<T, false, 2> {
-  static constexpr int P = sizeof(T) == 8 ? 8 : 16, NT = sizeof(T) == 8 ? 512 : 256;
-};
-template <typename T> struct TileVariant<T, false, 3> {
-  static constexpr int P = sizeof(T) == 8 ? 16 : 32, NT = sizeof(T) == 8 ? 256 : 512;
-};
+template <typename T> struct TileVariant<T, false, 1> { static constexpr int P = 4, NT = 128; };
+template <typename T> struct TileVariant<T, false, 2> { static constexpr int P = 8, NT = 512; };
+template <typename T> struct TileVariant<T, false, 3> { static constexpr int P = 8, NT = 128; };
 
 // A variant is built for length N when its LDS tile fits 64 KiB and each
 // thread holds at most 256 bytes of the tile in registers.
 template <typename T, int N, int P, int NT>
 constexpr bool tile_fits() {
-  return (long long)P * seq_len(N, pick_maxr(N, P, NT)) * (long long)sizeof(cplx<T>) <= 65536 &&
-         (2LL * P * N / NT) * (long long)sizeof(T) <= 256 && NT % 64 == 0 && NT <= 1024;
+  return (long long)P * seq_len(N, pick_maxr(N, P, NT)) * (long long)sizeof(cplx<double>) <= 65536 &&
+         (2LL * P * N / NT) * 8LL <= 256 && NT % 64 == 0 && NT <= 1024;
 }
 
 // Plain pass of length n with tile variant v (1..kTileVariants-1);
