@@ -521,12 +521,10 @@ int run_step(smaiso_handle* h) {
 
   // a6 (S:1104-1250): grad_a = IST_a(-Vs_a shift(scale CT(.))) acts on one
   // axis only (iDCT.DCT = 2N on the others), so each component is one pass
-  // along its axis: from rho in FP64, and in FP64 from psi (= ICT(psiq)).
-  // The FP32-mixed path keeps the 3-D form for grad psi: the shift is
-  // applied on load to the FP64 coefficients psiq, each rounded relative to
-  // its own size, where a pass over the FP32 copy of psi would amplify that
-  // copy's white rounding noise by |q|.  The passes that read one input run
-  // back to back, so its later reads find it in the memory-side cache.
+  // along its axis: from rho in FP64, and from psi (= ICT(psiq)) in the work
+  // type (the passes compute in FP64, so the FP32 copy of psi contributes
+  // only its storage rounding).  The passes that read one input run back to
+  // back, so its later reads find it in the memory-side cache.
   for (int a = 0; a < 3; ++a) {
     Fuse g = fb;
     g.axis = a;
@@ -536,13 +534,8 @@ int run_step(smaiso_handle* h) {
   for (int a = 0; a < 3; ++a) {
     Fuse g = fb;
     g.axis = a;
-    if (h->mixed) {
-      g.op = OP_LD_GRAD_D; g.src[0] = h->psiq; g.c0 = 1.0;
-      HIPTRY(h, h->xwf(kIST[a], g, gp[a]));
-    } else {
-      g.op = OP_AX_GRAD;
-      HIPTRY(h, h->axpass(F64, a, h->psi, gp[a], g));
-    }
+    g.op = OP_AX_GRAD;
+    HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, h->psi, gp[a], g));
   }
   // a7 (S:1254-1289): the Gaussian exp(-c |q|^2) is a product over the axes,
   // one pass per axis; |amp| is computed on load by the first

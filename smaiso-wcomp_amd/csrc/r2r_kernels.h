@@ -1086,11 +1086,15 @@ template <typename T> struct TileVariant<T, false, 1> { static constexpr int P =
 template <typename T> struct TileVariant<T, false, 2> { static constexpr int P = 8, NT = 512; };
 template <typename T> struct TileVariant<T, false, 3> { static constexpr int P = 8, NT = 128; };
 
-// A variant is built for length N when its LDS tile fits 64 KiB and each
+// LDS a workgroup may take on gfx950 (160 KiB per CU, all of it available to
+// one workgroup).
+constexpr long long kMaxLds = 160 * 1024;
+
+// A variant is built for length N when its LDS tile fits kMaxLds and each
 // thread holds at most 256 bytes of the tile in registers.
 template <typename T, int N, int P, int NT>
 constexpr bool tile_fits() {
-  return (long long)P * seq_len(N, pick_maxr(N, P, NT)) * (long long)sizeof(cplx<double>) <= 65536 &&
+  return (long long)P * seq_len(N, pick_maxr(N, P, NT)) * (long long)sizeof(cplx<double>) <= kMaxLds &&
          (2LL * P * N / NT) * 8LL <= 256 && NT % 64 == 0 && NT <= 1024;
 }
 
