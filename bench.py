@@ -355,7 +355,7 @@ def main():
         # rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this bench (copy-calibrated)
         with open(tpath) as f:
             tr = json.load(f)
-        cls = tr.get("r2r_z" if dom == "r2r_z" else "r2r_yx")
+        cls = tr.get(dom) or tr.get("r2r_yx" if dom in ("r2r_y", "r2r_x") else dom)
         if cls:
             traffic = cls["read_bytes"] + cls["write_bytes"]
             traffic_src = tr.get("source")
