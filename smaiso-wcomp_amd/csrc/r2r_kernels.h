@@ -480,12 +480,17 @@ __device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& 
 template <typename T, int OP>
 __device__ __forceinline__ double ax_weight(const Fuse& f, int m, int n) {
   const double c = f.c0 / (2.0 * n);
-  if constexpr (OP == OP_AX_GRAD) return -c * f.vs[f.axis][m];
-  else if constexpr (OP == OP_AX_DIV) return c * f.vq[f.axis][m];
-  else if constexpr (OP == OP_AX_GRAD_T) return -c * f.vs[f.axis][m - 1];
-  else if constexpr (OP == OP_AX_DIV_T) return c * f.vq[f.axis][m + 1];
+  // DCT-lattice wavenumber of index i along the pass axis, evaluated with the
+  // expression of the tables (same value) rather than loaded from them: the
+  // load would stall the operator step behind an L2 round trip
+  const double dn = f.dn[f.axis];
+  auto vq = [&](int i) { return dn != 0.0 ? M_PI * (double)i / dn : f.vq[f.axis][i]; };
+  if constexpr (OP == OP_AX_GRAD) return -c * vq(m + 1);          // Vs[m] = Vq[m+1]
+  else if constexpr (OP == OP_AX_DIV) return c * vq(m);
+  else if constexpr (OP == OP_AX_GRAD_T) return -c * vq(m);       // Vs[m-1]
+  else if constexpr (OP == OP_AX_DIV_T) return c * vq(m + 1);
   else {
-    const double q = f.vq[f.axis][m];
+    const double q = vq(m);
     return c * exp(-1.57 * 1.57 * q * q * 2);
   }
 }
