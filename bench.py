@@ -91,14 +91,16 @@ def kernel_timing(st, nsteps):
     return res
 
 
-def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000)):
+def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000), params=None, label=""):
     """FP32-mixed vs FP64 on the same state: relative L2 errors of psi, rho and
     v (all three components against the FP64 velocity's norm) at checkpoints.
     The FP64 GPU stepper is the reference here; its own parity with the FP64
-    oracle is checked by the test suite."""
+    oracle is checked by the test suite.  params(precision) builds the
+    parameters (default: make_params at n^3)."""
     import numpy as np
-    a = S.Stepper(make_params(S, n, S.FP64))
-    b = S.Stepper(make_params(S, n, S.FP32_MIXED))
+    mk = params or (lambda prec: make_params(S, n, prec))
+    a = S.Stepper(mk(S.FP64))
+    b = S.Stepper(mk(S.FP32_MIXED))
     a.set_state(psi, rho)
     b.set_state(psi, rho)
     out, done = {}, 0
@@ -116,7 +118,30 @@ def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000)):
         den = math.sqrt(sum(float(np.linalg.norm(x)) ** 2 for x in va))
         res["v"] = num / den
         out[str(c)] = res
-        log(f"[bench] mixed vs fp64 at step {c}: {res}")
+        log(f"[bench] mixed vs fp64{label} at step {c}: {res}")
+    a.close()
+    b.close()
+    return out
+
+
+def flat_interface_accuracy(S, shapes, checkpoints=(100, 500, 2000)):
+    """FP32-mixed vs FP64 on the flat-interface initial state (layers
+    cos(q0 z) below z = Lz/2 times 1/2 (1 - tanh((z - Lz/2)/2)), density 1.0 in
+    the smectic and 0.9 in the isotropic phase, 1e-3 seeded noise; rho0 = 0.9,
+    dt = 0.01, rho_m = 1) on each grid of `shapes`, errors at the checkpoints."""
+    out = {}
+    for (nx, ny, nz) in shapes:
+        def params(prec, nx=nx, ny=ny, nz=nz):
+            return S.Params(nx=nx, ny=ny, nz=nz, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0,
+                            precision=prec)
+        ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=0.9, rho_sm=1.0, undulation=0.0,
+                           noise=1e-3, seed=1)
+        psi, rho = S.make_interface_ic(params(S.FP64), ic)
+        key = f"{nx}x{ny}x{nz}"
+        out[key] = mixed_accuracy(S, None, psi, rho, checkpoints, params=params, label=f" (flat, {key})")
+        del psi, rho
+    out["workload"] = ("flat interface at Lz/2, rho 1.0/0.9, 1e-3 noise (seed 1), rho0=0.9, dt=0.01, rho_m=1; "
+                       "relative L2 of FP32-mixed vs FP64 (v: all components against the FP64 velocity norm)")
     return out
 
 
@@ -290,6 +315,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--accuracy-steps", type=str, default="100,500,2000",
                     help="checkpoints of the FP32-mixed vs FP64 comparison ('' to skip)")
+    ap.add_argument("--flat-grids", type=str, default="256x256x256,512x512x256",
+                    help="grids of the flat-interface FP32-mixed vs FP64 comparison ('' to skip)")
     ap.add_argument("--no-graph", action="store_true", help="launch every step eagerly")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default r2r tiles")
     ap.add_argument("--ensemble", type=int, default=48, help="members of the ensemble leg (0 to skip)")
@@ -341,6 +368,10 @@ def main():
     if mixed is not None and args.accuracy_steps:
         cps = tuple(int(x) for x in args.accuracy_steps.split(","))
         mixed["rel_l2_vs_fp64"] = mixed_accuracy(S, n, psi, rho, cps)
+        if args.flat_grids:
+            shapes = [tuple(int(v) for v in g.split("x")) for g in args.flat_grids.split(",")]
+            log(f"[bench] flat-interface accuracy on {shapes}")
+            mixed["rel_l2_vs_fp64_flat"] = flat_interface_accuracy(S, shapes, cps)
 
     # dominant kernel class of the FP64 stepper
     dom = max(kt64, key=lambda k: kt64[k]["total_ms"])

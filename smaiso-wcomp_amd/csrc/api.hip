@@ -88,7 +88,11 @@ struct smaiso_handle {
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
   double* rd[3] = {nullptr, nullptr, nullptr};  // grad rho, FP64 in both precisions (rho is
                                                // ~constant: its gradient is a small difference)
-  double *partials = nullptr, *diag_out = nullptr, *out64 = nullptr;
+  double *partials = nullptr, *diag_out = nullptr;
+  // FP64 field of the group's size, FP32 storage only: the field getters'
+  // widened copy, and within a step the intermediates of psi's inverse
+  // transform (nothing is kept in it from one call to the next)
+  double* out64 = nullptr;
   std::vector<double> diag_host;
   long long diag_epoch = -1;  // epoch of the cached diagnostics (-1: none)
   // work fields (T = double or float)
@@ -239,6 +243,13 @@ struct smaiso_handle {
              pass_bytes(dt, io, gg));
     return r2r.pass(dt, kind, axis, io, gg, stream);
   }
+  // one pass reading in_dt and storing out_dt (R2R::pass_cast)
+  hipError_t run_pass_cast(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io) {
+    const double n = (double)g.n3() * g.nb;
+    const double bytes = pass_bytes(in_dt, io, g) + n * ((out_dt == F64 ? 8 : 4) - (in_dt == F64 ? 8 : 4));
+    Scope sc(this, axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X, bytes);
+    return r2r.pass_cast(in_dt, out_dt, kind, axis, io, g, stream);
+  }
   // 3-D transform z, y, x of all members (or of gg's batch); f's load op acts
   // in the z pass
   hipError_t xform(DType dt, const int kinds[3], const void* in, const void* in2, void* out,
@@ -258,6 +269,23 @@ struct smaiso_handle {
   }
   hipError_t xwf(const int kinds[3], const Fuse& f, void* out) {
     return xform(mixed ? F32 : F64, kinds, nullptr, nullptr, out, &f);
+  }
+  // As xwf for FP32 storage, with the outputs of the z and y passes kept in
+  // FP64 (in out64): the z pass reads FP32 sources and stores FP64, the y
+  // pass runs in FP64 and the x pass stores FP32.  Each pass computes in
+  // FP64 either way; this removes the FP32 rounding of partly transformed
+  // values, which each later pass spreads over whole lines of the output.
+  hipError_t xwf_wide(const int kinds[3], const Fuse& f, void* out) {
+    PassIO io;
+    io.f = f;
+    io.out = out64;
+    hipError_t e = run_pass_cast(F32, F64, kinds[2], 2, io);
+    if (e != hipSuccess) return e;
+    io.in = out64;
+    io.f.op = OP_PLAIN;
+    if ((e = run_pass(F64, kinds[1], 1, io, g)) != hipSuccess) return e;
+    io.out = out;
+    return run_pass_cast(F64, F32, kinds[0], 0, io);
   }
   // the y and x passes of a transform whose z pass has run, in place
   hipError_t xw_yx(const int kinds[3], void* buf) {
@@ -647,7 +675,12 @@ int run_step(smaiso_handle* h) {
     g.psiq_new = h->psiq_old;
     g.nlq_old = h->nlq_old;
     g.first = h->nloop == 1 ? 1 : 0;  // quirk Q5
-    HIPTRY(h, h->xwf(kICT, g, h->psi));
+    // FP32 storage: psi's transform keeps FP64 intermediates.  Their FP32
+    // rounding, fed back through the nonlinear terms every step, was the
+    // largest FP32 storage error on near-stationary states (about 20x the
+    // rounding of the stored fields themselves on a flat interface)
+    if (h->mixed) HIPTRY(h, h->xwf_wide(kICT, g, h->psi));
+    else HIPTRY(h, h->xwf(kICT, g, h->psi));
   }
   std::swap(h->psiq, h->psiq_old);
   return SMAISO_OK;

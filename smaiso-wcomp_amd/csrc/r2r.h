@@ -92,6 +92,13 @@ class R2R {
   // op in io.f applies to the z pass.
   hipError_t transform3d(DType dt, const int kinds[3], const PassIO& io, const Geom& g,
                          hipStream_t s);
+  // One axis pass that reads in_dt and stores out_dt: FP32 -> FP64 in a z
+  // pass (OP_PLAIN or OP_LD_PSI, whose work-type sources are then FP32),
+  // FP64 -> FP32 in a y or x pass (OP_PLAIN); hipErrorNotSupported for the
+  // other directions.  With in_dt == out_dt it is pass().  A 3-D transform
+  // between FP32 fields made of these keeps its intermediates in FP64.
+  hipError_t pass_cast(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io,
+                       const Geom& g, hipStream_t s);
   // True when length n runs on the LDS FFT path (else the direct O(n^2) path).
   static bool fft_supported(int n);
   // Build the (FP64) twiddle tables of length n now (they are otherwise

@@ -213,11 +213,13 @@ constexpr int seq_len(int n, int maxr) { return n + (n >> pad_shift(n, maxr)) + 
 template <int SH>
 __device__ __forceinline__ int lp(int x) { return x + (x >> SH); }
 
-template <typename T>
+// T: storage type of the input and of the load ops' work-type sources;
+// TO: storage type of the output (T unless a pass changes the storage type)
+template <typename T, typename TO = T>
 struct PassArgs {
   const T* in;
   const T* in2;
-  T* out;
+  TO* out;
   const cplx<double>* tw;  // FP64 tables for both storage types
   const cplx<double>* mk;
   int nx, ny, nz;
@@ -353,8 +355,8 @@ __device__ __forceinline__ void line_ij(long long g, int ny, int& i, int& j) {
 // Element m of z-line g (= i*ny + j) of the first pass under load op OP;
 // idx = g*nz + m.  The value is formed in FP64 and goes to the FP64 tile as
 // it is: a pass rounds to the storage type T only when it stores.
-template <typename T, int OP>
-__device__ __forceinline__ double load_elem(const PassArgs<T>& a, int i, int j, int m, long long idx) {
+template <typename T, int OP, typename TO>
+__device__ __forceinline__ double load_elem(const PassArgs<T, TO>& a, int i, int j, int m, long long idx) {
   if constexpr (OP == OP_PLAIN) {
     double v = (double)a.in[idx];
     if (a.in2) v += (double)a.in2[idx];
@@ -406,8 +408,8 @@ __device__ __forceinline__ double load_elem(const PassArgs<T>& a, int i, int j, 
   }
 }
 
-template <typename T>
-__device__ double load_elem_rt(const PassArgs<T>& a, long long g, int m, long long mo) {
+template <typename T, typename TO>
+__device__ double load_elem_rt(const PassArgs<T, TO>& a, long long g, int m, long long mo) {
   int i, j;
   line_ij(g, a.ny, i, j);
   const long long idx = mo + g * a.nz + m;
@@ -436,8 +438,8 @@ __device__ __forceinline__ double mid_apply(const Fuse& f, int i, int j, int k, 
 
 // Adjacent columns l0, l0+1 of a strided tile at row m (zero outside the
 // grid), widened to FP64.
-template <typename T>
-__device__ __forceinline__ cplx<double> load_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m) {
+template <typename T, typename TO>
+__device__ __forceinline__ cplx<double> load_pair(const PassArgs<T, TO>& a, const TileGeo& g, int l0, int m) {
   const long long addr = g.base + l0 + (long long)m * g.stride_m;
   cplx<double> v = {0.0, 0.0};
   if (a.vec && l0 + 1 < g.nvalid) {
@@ -457,20 +459,20 @@ __device__ __forceinline__ cplx<double> load_pair(const PassArgs<T>& a, const Ti
 
 // Store (y0, y1) at columns l0, l0+1 of row m, rounded to the storage type
 // (added to what is there when acc).
-template <typename T>
-__device__ __forceinline__ void store_pair(const PassArgs<T>& a, const TileGeo& g, int l0, int m, double y0,
+template <typename T, typename TO>
+__device__ __forceinline__ void store_pair(const PassArgs<T, TO>& a, const TileGeo& g, int l0, int m, double y0,
                                            double y1, bool acc = false) {
-  T* o = a.out + g.base + l0 + (long long)m * g.stride_m;
+  TO* o = a.out + g.base + l0 + (long long)m * g.stride_m;
   if (a.vec && l0 + 1 < g.nvalid) {
     if (acc) {
-      const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
+      const cplx<TO> old = *reinterpret_cast<const cplx<TO>*>(o);
       y0 += (double)old.x;
       y1 += (double)old.y;
     }
-    *reinterpret_cast<cplx<T>*>(o) = cplx<T>{(T)y0, (T)y1};
+    *reinterpret_cast<cplx<TO>*>(o) = cplx<TO>{(TO)y0, (TO)y1};
   } else {
-    if (l0 < g.nvalid) o[0] = (T)(acc ? (double)o[0] + y0 : y0);
-    if (l0 + 1 < g.nvalid) o[1] = (T)(acc ? (double)o[1] + y1 : y1);
+    if (l0 < g.nvalid) o[0] = (TO)(acc ? (double)o[0] + y0 : y0);
+    if (l0 + 1 < g.nvalid) o[1] = (TO)(acc ? (double)o[1] + y1 : y1);
   }
 }
 
@@ -548,9 +550,10 @@ __device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int 
 }
 
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
-// form complex sequence c.  OP selects a fused operation (FuseOp).
-template <typename T, int N, int P, int NT, int OP>
-__global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
+// form complex sequence c.  OP selects a fused operation (FuseOp); the
+// output is stored as TO.
+template <typename T, int N, int P, int NT, int OP, typename TO = T>
+__global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
   constexpr int TL = 2 * P;
   constexpr int MAXR = pick_maxr(N, P, NT);
   constexpr int SEQ = seq_len(N, MAXR);
@@ -811,14 +814,14 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
           const int l0 = 2 * c;
           if (a.axis == 2) {
             if (l0 < g.nvalid) {
-              T* o = a.out + g.base + (long long)l0 * N;
-              o[oa] = (T)y0a;
-              o[ob] = (T)y0b;
+              TO* o = a.out + g.base + (long long)l0 * N;
+              o[oa] = (TO)y0a;
+              o[ob] = (TO)y0b;
             }
             if (l0 + 1 < g.nvalid) {
-              T* o = a.out + g.base + (long long)(l0 + 1) * N;
-              o[oa] = (T)y1a;
-              o[ob] = (T)y1b;
+              TO* o = a.out + g.base + (long long)(l0 + 1) * N;
+              o[oa] = (TO)y1a;
+              o[ob] = (TO)y1b;
             }
           } else {
             store_pair(a, g, l0, oa, y0a, y1a);
@@ -850,20 +853,20 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (l0 + q < g.nvalid) {
-            T* o = a.out + g.base + (long long)(l0 + q) * N + 2 * n;
+            TO* o = a.out + g.base + (long long)(l0 + q) * N + 2 * n;
             A v0 = q == 0 ? ua.x : ua.y;
             A v1 = s * (q == 0 ? ub.x : ub.y);
             if (a.vec) {
               if (acc) {
-                const cplx<T> old = *reinterpret_cast<const cplx<T>*>(o);
+                const cplx<TO> old = *reinterpret_cast<const cplx<TO>*>(o);
                 v0 += (A)old.x;
                 v1 += (A)old.y;
               }
-              *reinterpret_cast<cplx<T>*>(o) = cplx<T>{(T)v0, (T)v1};
+              *reinterpret_cast<cplx<TO>*>(o) = cplx<TO>{(TO)v0, (TO)v1};
             } else {
               if (acc) { v0 += (A)o[0]; v1 += (A)o[1]; }
-              o[0] = (T)v0;
-              o[1] = (T)v1;
+              o[0] = (TO)v0;
+              o[1] = (TO)v1;
             }
           }
         }
@@ -880,9 +883,9 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T> a) {
 // ---------------------------------------------------------------------------
 constexpr int kThreads = 256;
 
-template <typename T>
+template <typename T, typename TO = T>
 struct DirectArgs {
-  PassArgs<T> p;
+  PassArgs<T, TO> p;
   const double* costab;  // 4n entries: cos(pi t/(2n))
   int n;
   int tl;
@@ -915,10 +918,10 @@ __device__ inline double direct_sum(int kind, const double* xl, int n, int k, co
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) {
+template <typename T, typename TO = T>
+__global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T, TO> d) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const PassArgs<T>& a = d.p;
+  const PassArgs<T, TO>& a = d.p;
   const int n = d.n, tl = d.tl;
   // FP64 tile for both storage types
   double* x = reinterpret_cast<double*>(smem_raw);
@@ -1017,18 +1020,18 @@ __global__ __launch_bounds__(kThreads) void direct_pass_kernel(DirectArgs<T> d) 
     else if (mid) r = direct_sum(REDFT01, y + l * n, n, k, d.costab);
     else r = direct_sum(a.kind, x + l * n, n, k, d.costab);
     const long long oi = g.base + l * g.line_step + k * g.stride_m;
-    a.out[oi] = (T)((ax && a.f.accumulate) ? (double)a.out[oi] + r : r);
+    a.out[oi] = (TO)((ax && a.f.accumulate) ? (double)a.out[oi] + r : r);
   }
 }
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-template <typename T>
-PassArgs<T> make_args(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g) {
-  PassArgs<T> a;
+template <typename T, typename TO = T>
+PassArgs<T, TO> make_args(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g) {
+  PassArgs<T, TO> a;
   a.in = static_cast<const T*>(io.in);
   a.in2 = static_cast<const T*>(io.in2);
-  a.out = static_cast<T*>(io.out);
+  a.out = static_cast<TO*>(io.out);
   a.tw = static_cast<const cplx<double>*>(tw);
   a.mk = static_cast<const cplx<double>*>(mk);
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
@@ -1040,16 +1043,16 @@ PassArgs<T> make_args(const PassIO& io, void* tw, void* mk, int kind, int axis, 
   return a;
 }
 
-template <typename T, int N, int P, int NT, int OP>
+template <typename T, int N, int P, int NT, int OP, typename TO = T>
 hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
                          hipStream_t s) {
   constexpr int TL = 2 * P;
-  PassArgs<T> a = make_args<T>(io, tw, mk, kind, axis, g);
+  PassArgs<T, TO> a = make_args<T, TO>(io, tw, mk, kind, axis, g);
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
   const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
-  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP, TO>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 

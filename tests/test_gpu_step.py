@@ -154,6 +154,51 @@ def test_mixed_precision_psi_within_1e6_of_fp64():
         assert err < tol, (name, err)
 
 
+def test_mixed_precision_flat_interface_256():
+    """FP32-mixed on the flat-interface state (rho 1.0/0.9, rho0 0.9) at 256^3
+    after 500 steps: psi within 1e-7 of the FP64 stepper.  This
+    near-stationary state is the one where FP32 rounding of partly
+    transformed values between the passes of psi's inverse transform used to
+    dominate (3.6e-7 here); with FP64 intermediates the error is the FP32
+    rounding of psi itself (about 3e-8)."""
+    shape = (256, 256, 256)
+    ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=0.9, rho_sm=1.0, undulation=0.0,
+                       noise=1e-3, seed=1)
+
+    def params(prec):
+        return S.Params(nx=shape[0], ny=shape[1], nz=shape[2], dt=0.01, rho0=0.9, rho_m=1.0,
+                        divv_switch=0, precision=prec)
+
+    psi, rho = S.make_interface_ic(params(S.FP64), ic)
+    a, b = S.Stepper(params(S.FP64)), S.Stepper(params(S.FP32_MIXED))
+    a.set_state(psi, rho)
+    b.set_state(psi, rho)
+    a.step(500)
+    b.step(500)
+    for name, tol in (("psi", 1e-7), ("rho", 1e-8)):
+        fa, fb = a.field(name), b.field(name)
+        err = np.linalg.norm(fb - fa) / np.linalg.norm(fa)
+        assert err < tol, (name, err)
+    a.close()
+    b.close()
+
+
+def test_mixed_odd_lengths_direct_cast_path():
+    """FP32-mixed on a grid whose x (20) and z (40) lengths run on the direct
+    path, so the FP32 -> FP64 z pass and the FP64 -> FP32 x pass of psi's
+    transform run the direct kernel's casting instances: 3 steps against the
+    FP64 oracle at FP32 storage accuracy."""
+    sp, op = params_pair(nx=20, ny=32, nz=40, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0,
+                         precision=S.FP32_MIXED)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(undulation=0.5, seed=5))
+    st, o = run_pair(sp, op, psi, rho, 3)
+    ref = oracle_fields(o)
+    sc = group_scales(ref)
+    for name, tol in (("psi", 1e-6), ("rho", 1e-7), ("psiq", 1e-6)):
+        mx, l2 = rel_err(st.field(name), ref[name], *sc[name])
+        assert l2 < tol, (name, mx, l2)
+
+
 def test_uniform_state_is_stationary():
     sp, op = params_pair(nx=16, ny=16, nz=16, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
     psi = np.zeros(sp.shape)
