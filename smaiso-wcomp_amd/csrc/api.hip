@@ -447,6 +447,8 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
     c.rho0 = ps[m].rho0;
     c.amp_eq = h->amp_eq;
     c.kp = (1 - ps[m].rho0) / h->amp_eq;  // S:252
+    c.rnu = 1.0 / c.nu;
+    c.rlam = 1.0 / (c.lam + 2 * c.nu);
     h->mc_host.push_back(c);
   }
   h->diag_host.assign(4 * n, 0.0);

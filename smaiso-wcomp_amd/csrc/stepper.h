@@ -16,6 +16,7 @@ namespace smaiso {
 // Parameters that may differ between the members of a batch.
 struct MemberConsts {
   double nu, lam, zeta, rho0, kp, amp_eq;
+  double rnu, rlam;  // 1/nu and 1/(lam + 2 nu), correctly rounded on the host
 };
 
 // Scalars and wavenumber tables shared by every kernel of a step.

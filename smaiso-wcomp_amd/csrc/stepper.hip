@@ -120,12 +120,21 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
     T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
-  // XCD-aware order: workgroups are dealt to the 8 XCDs round robin; map them
-  // so that each XCD walks a contiguous range of lines and finds the j+-1 and
-  // i+-1 neighbour lines in its own L2
+  // XCD-aware order: workgroups are dealt to the 8 XCDs round robin.  When
+  // an i plane splits into 8 equal runs of whole workgroups, XCD x takes run
+  // x of every plane (a band of j lines), plane after plane: the i+1 lines it
+  // reads as neighbours are read again as centres one band later (~1/8 plane
+  // of traffic), inside its own 4 MiB L2.  Otherwise each XCD walks a
+  // contiguous range of lines.
   const unsigned nb = gridDim.x;
+  const unsigned plane = (unsigned)c.ny * c.nz;
   unsigned b = blockIdx.x;
-  if ((nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+  if (plane % (8u * kBlock) == 0) {
+    const unsigned bpp = plane / kBlock, bpx = bpp >> 3, s = b >> 3;
+    b = (s / bpx) * bpp + (b & 7u) * bpx + (s % bpx);
+  } else if ((nb & 7u) == 0) {
+    b = (b & 7u) * (nb >> 3) + (b >> 3);
+  }
   const unsigned idx = b * kBlock + threadIdx.x;
   if (idx >= n3) return;
   const long long mo = member_offset(c);
@@ -140,21 +149,21 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
   const double wx = c.vs[0][p.i], wy = c.vs[1][p.j], wz = c.vs[2][p.k];
   const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
-  const double rnu = 1.0 / mm.nu, rlam = 1.0 / (mm.lam + 2 * mm.nu);
+  const double rnu = mm.rnu, rlam = mm.rlam;
   const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
-  // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q_a|^2) on component
-  // a's lattice, and G at the shifted fgrad^ index.  FP32-mixed fields take
-  // them as products of the one-axis tables (the kernel is ALU-bound there);
-  // FP64 fields evaluate exp (the table loads measured slower there)
+  // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q|^2).  A
+  // component's factor on its own (shifted) lattice at n equals the factor on
+  // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same double),
+  // which is also the G of its fgrad^ term: one evaluation serves both, three
+  // per point.  FP32-mixed fields take them as products of the one-axis
+  // tables (the kernel is ALU-bound there); FP64 fields evaluate exp (the
+  // table loads measured slower there)
   constexpr bool TAB = sizeof(T) == 4;
-  auto g3 = [&](int a, int i, int j, int k, bool si, bool sj, bool sk) -> double {
-    (void)a;
+  auto g3 = [&](int i, int j, int k) -> double {
     if constexpr (TAB) {
-      return (si ? c.gs[0][i] : c.gq[0][i]) * (sj ? c.gs[1][j] : c.gq[1][j]) * (sk ? c.gs[2][k] : c.gq[2][k]);
+      return c.gq[0][i] * c.gq[1][j] * c.gq[2][k];
     } else {
-      const double x = si ? c.vs[0][i] : c.vq[0][i];
-      const double y = sj ? c.vs[1][j] : c.vq[1][j];
-      const double z = sk ? c.vs[2][k] : c.vq[2][k];
+      const double x = c.vq[0][i], y = c.vq[1][j], z = c.vq[2][k];
       return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
     }
   };
@@ -167,12 +176,14 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     const double dot = wx * Sx + qy * Syp + qz * Szp;
     const double inv = 1.0 / (wx * wx + qy2 + qz2);
     const double cm1 = c.scale * inv, cm2 = wx * inv;
-    const double fg = (double)fgq[idx + sI] + (double)erq[idx + sI] * g3(0, p.i + 1, p.j, p.k, false, false, false);
+    const double gx = g3(p.i + 1, p.j, p.k);
+    const double fg = (double)fgq[idx + sI] + (double)erq[idx + sI] * gx;
     ox = (cm1 * rnu) * (Sx - cm2 * dot);
-    rx = (cm1 * rlam) * (wx * fg + cm2 * dot * g3(0, p.i, p.j, p.k, true, false, false));
+    rx = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
   }
   double oy = 0, ry = 0, oz = 0, rz = 0;
   if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
+    const double gy = g3(p.i, p.j + 1, p.k);
     // y component
     {
       const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + sJ] : 0.0;
@@ -180,9 +191,9 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
       const double dot = qx * Sxp + wy * Sy + qz * Szp;
       const double inv = 1.0 / (qx2 + wy * wy + qz2);
       const double cm1 = c.scale * inv, cm2 = wy * inv;
-      const double fg = (double)fgq[idx + sJ] + (double)erq[idx + sJ] * g3(1, p.i, p.j + 1, p.k, false, false, false);
+      const double fg = (double)fgq[idx + sJ] + (double)erq[idx + sJ] * gy;
       oy = (cm1 * rnu) * (Sy - cm2 * dot);
-      ry = (cm1 * rlam) * (wy * fg + cm2 * dot * g3(1, p.i, p.j, p.k, false, true, false));
+      ry = (cm1 * rlam) * (wy * fg + cm2 * dot * gy);
     }
     // z component
     {
@@ -191,14 +202,14 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
       const double dot = qx * Sxp + qy * Syp + wz * Sz;
       const double inv = 1.0 / (qx2 + qy2 + wz * wz);
       const double cm1 = c.scale * inv, cm2 = wz * inv;
-      const double fg = (double)fgq[idx + 1] + (double)erq[idx + 1] * g3(2, p.i, p.j, p.k + 1, false, false, false);
+      const double gz = g3(p.i, p.j, p.k + 1);
+      const double fg = (double)fgq[idx + 1] + (double)erq[idx + 1] * gz;
       oz = (cm1 * rnu) * (Sz - cm2 * dot);
       // quirk Q3 (S:1932): CM1y in the exponent at k=0, j>=1, except the
-      // first plane of each reference slab
+      // first plane of each reference slab (the y component's factor)
       const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
       const bool q3 = p.k == 0 && p.j > 0 && (p.i % slab) != 0;
-      const double e = q3 ? g3(2, p.i, p.j, p.k, false, true, false) : g3(2, p.i, p.j, p.k, false, false, true);
-      rz = (cm1 * rlam) * (wz * fg + cm2 * dot * e);
+      rz = (cm1 * rlam) * (wz * fg + cm2 * dot * (q3 ? gy : gz));
     }
   }
   const long long o = mo + idx;
