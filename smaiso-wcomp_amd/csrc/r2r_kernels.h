@@ -725,9 +725,14 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
       }
       __syncthreads();
     } else if constexpr (AX) {
-      // forward outputs of both lines -> registers -> LDS at their output
-      // index; the operator reads shifted modes, so each step is fenced
-      cplx<A> ra[ITW], rb[ITW];
+      // forward outputs -> operator -> inverse pre-twiddle, in registers and
+      // in place: item k holds the forward outputs at fa, fb (k and kb, or
+      // N-1-k and N-1-kb for a forward DST), and the modes its pre-twiddle
+      // reads, the operator's sources of output indices pa, pb (k and kb, or
+      // N-1-k and N-1-kb for an inverse DST), are always among them or zero:
+      // the one-mode shifts compose with the DST index reversal.  So no
+      // exchange between items is needed, and item k rewrites only the
+      // positions k, kb it read
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -738,62 +743,25 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           const int kb = (k == 0) ? HN : N - k;
           A y0a, y1a, y0b, y1b;
           makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
-          ra[it] = cplx<A>{y0a, y1a};
-          rb[it] = cplx<A>{y0b, y1b};
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < ITW; ++it) {
-        const int w = tid + it * NT;
-        if (W % NT == 0 || w < W) {
-          int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
-          else { k = w / P; c = w - k * P; }
-          const int kb = (k == 0) ? HN : N - k;
-          buf[c * SEQ + lp<SH>((sine ? N - 1 - k : k))] = ra[it];
-          buf[c * SEQ + lp<SH>((sine ? N - 1 - kb : kb))] = rb[it];
-        }
-      }
-      __syncthreads();
-      // operator, then the inverse's pre-twiddle (DST-III reads its input
-      // reversed)
-#pragma unroll
-      for (int it = 0; it < ITW; ++it) {
-        const int w = tid + it * NT;
-        if (W % NT == 0 || w < W) {
-          int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
-          else { k = w / P; c = w - k * P; }
-          const int kb = (k == 0) ? HN : N - k;
+          const int fa = sine ? N - 1 - k : k;
           const int pa = sine_out ? N - 1 - k : k;
           const int pb = sine_out ? N - 1 - kb : kb;
-          cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
           const int sa = ax_source<AXOP>(pa, N), sb = ax_source<AXOP>(pb, N);
+          cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
           if (sa >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pa, N);
-            const cplx<A> y = buf[c * SEQ + lp<SH>(sa)];
+            const cplx<A> y = sa == fa ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
             xk = cplx<A>{wgt * y.x, wgt * y.y};
           }
           if (sb >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pb, N);
-            const cplx<A> y = buf[c * SEQ + lp<SH>(sb)];
+            const cplx<A> y = sb == fa ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
             xb = cplx<A>{wgt * y.x, wgt * y.y};
           }
-          makhoul_pre<A, N>(a.mk, k, xk, xb, ra[it], rb[it]);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < ITW; ++it) {
-        const int w = tid + it * NT;
-        if (W % NT == 0 || w < W) {
-          int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
-          else { k = w / P; c = w - k * P; }
-          const int kb = (k == 0) ? HN : N - k;
-          buf[c * SEQ + lp<SH>(k)] = ra[it];
-          buf[c * SEQ + lp<SH>(kb)] = rb[it];
+          cplx<A> zk, zb;
+          makhoul_pre<A, N>(a.mk, k, xk, xb, zk, zb);
+          buf[c * SEQ + lp<SH>(k)] = zk;
+          buf[c * SEQ + lp<SH>(kb)] = zb;
         }
       }
       __syncthreads();
