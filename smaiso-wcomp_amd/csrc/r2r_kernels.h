@@ -292,11 +292,16 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
 #pragma unroll
         for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + lp<SH>(j + q * J)];
         if constexpr (NS > 1) {
+          // twiddles w^q, q = 1..R-1, of w = tw[k STRIDE]: one table load and
+          // powers by products (a few ulp from the table's entries)
           const int k = j % NS;
+          cplx<T> w1 = tw[k * STRIDE];
+          if constexpr (SIGN > 0) w1.y = -w1.y;
+          cplx<T> t = w1;
+          v[it][1] = cmul(v[it][1], t);
 #pragma unroll
-          for (int q = 1; q < R; ++q) {
-            cplx<T> t = tw[q * k * STRIDE];
-            if constexpr (SIGN > 0) t.y = -t.y;
+          for (int q = 2; q < R; ++q) {
+            t = cmul(t, w1);
             v[it][q] = cmul(v[it][q], t);
           }
         }
@@ -504,6 +509,14 @@ __device__ __forceinline__ int ax_source(int m, int n) {
   else return m;
 }
 
+// mk[N-k] from mk[k] = exp(-i pi k/2N): exp(-i pi/2) conj(mk[k]) = (-mk.y,
+// -mk.x), exact; item 0's partner N/2 takes the table entry (a uniform
+// address).
+template <typename T, int N>
+__device__ __forceinline__ cplx<T> mk_mirror(const cplx<T>* __restrict__ mk, int k, cplx<T> mkk) {
+  return k == 0 ? mk[N / 2] : cplx<T>{-mkk.y, -mkk.x};
+}
+
 // Makhoul post-processing of item k of one sequence: Z at k and kb (kb = N-k,
 // or N/2 for k = 0) -> the forward outputs of both lines at k and at kb.
 template <typename T, int N>
@@ -512,15 +525,16 @@ __device__ __forceinline__ void makhoul_post(const cplx<T>* __restrict__ mk, int
   // partner of k is N-k: Z_k pairs with conj Z_{N-k}; item 0 pairs 0 with 0 and N/2 with N/2
   const cplx<T> pa = (k == 0) ? za : zb;
   const cplx<T> qa = (k == 0) ? zb : za;
+  const cplx<T> mkk = mk[k];
   {
-    const cplx<T> m = mk[k];
+    const cplx<T> m = mkk;
     const cplx<T> S = {za.x + pa.x, za.y - pa.y};
     const cplx<T> D = {za.x - pa.x, za.y + pa.y};
     y0a = m.x * S.x - m.y * S.y;
     y1a = m.x * D.y + m.y * D.x;
   }
   {
-    const cplx<T> m = mk[kb];
+    const cplx<T> m = mk_mirror<T, N>(mk, k, mkk);
     const cplx<T> S = {zb.x + qa.x, zb.y - qa.y};
     const cplx<T> D = {zb.x - qa.x, zb.y + qa.y};
     y0b = m.x * S.x - m.y * S.y;
@@ -540,10 +554,11 @@ __device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int 
     cc.y = -cc.y;
     zb = cmul(cc, cplx<T>{rn.x + rn.y, rn.y - rn.x});
   } else {
-    cplx<T> cc = mk[k];
+    const cplx<T> mkk = mk[k];
+    cplx<T> cc = mkk;
     cc.y = -cc.y;
     zk = cmul(cc, cplx<T>{rk.x + rn.y, rk.y - rn.x});
-    cc = mk[N - k];
+    cc = mk_mirror<T, N>(mk, k, mkk);
     cc.y = -cc.y;
     zb = cmul(cc, cplx<T>{rn.x + rk.y, rn.y - rk.x});
   }
