@@ -92,7 +92,7 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       const int r = sq / P, cc = sq - r * P;
       const int kb = (k == 0) ? HN : N - k;
       A y0a, y1a, y0b, y1b;
-      makhoul_post<A, N>(mk, k, kb, buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+      makhoul_post<A, N>(mk, k, kb, mk[k], buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
       const bool sine = r == NF - 1;
       const int oa = sine ? N - 1 - k : k;
       const int ob = sine ? N - 1 - kb : kb;

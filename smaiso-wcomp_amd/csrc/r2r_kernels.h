@@ -519,13 +519,13 @@ __device__ __forceinline__ cplx<T> mk_mirror(const cplx<T>* __restrict__ mk, int
 
 // Makhoul post-processing of item k of one sequence: Z at k and kb (kb = N-k,
 // or N/2 for k = 0) -> the forward outputs of both lines at k and at kb.
+// mkk = mk[k].
 template <typename T, int N>
-__device__ __forceinline__ void makhoul_post(const cplx<T>* __restrict__ mk, int k, int kb, cplx<T> za,
-                                             cplx<T> zb, T& y0a, T& y1a, T& y0b, T& y1b) {
+__device__ __forceinline__ void makhoul_post(const cplx<T>* __restrict__ mk, int k, int kb, cplx<T> mkk,
+                                             cplx<T> za, cplx<T> zb, T& y0a, T& y1a, T& y0b, T& y1b) {
   // partner of k is N-k: Z_k pairs with conj Z_{N-k}; item 0 pairs 0 with 0 and N/2 with N/2
   const cplx<T> pa = (k == 0) ? za : zb;
   const cplx<T> qa = (k == 0) ? zb : za;
-  const cplx<T> mkk = mk[k];
   {
     const cplx<T> m = mkk;
     const cplx<T> S = {za.x + pa.x, za.y - pa.y};
@@ -543,10 +543,11 @@ __device__ __forceinline__ void makhoul_post(const cplx<T>* __restrict__ mk, int
 }
 
 // Makhoul pre-processing of the inverse: (Y0, Y1) at k (rk) and at kb (rn) ->
-// z_k, z_kb of U0 + i U1 with U_k = e^{+i pi k/2N} (Y_k - i Y_{N-k}).
+// z_k, z_kb of U0 + i U1 with U_k = e^{+i pi k/2N} (Y_k - i Y_{N-k}); mkk =
+// mk[k].
 template <typename T, int N>
-__device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int k, cplx<T> rk, cplx<T> rn,
-                                            cplx<T>& zk, cplx<T>& zb) {
+__device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int k, cplx<T> mkk, cplx<T> rk,
+                                            cplx<T> rn, cplx<T>& zk, cplx<T>& zb) {
   constexpr int HN = N / 2;
   if (k == 0) {
     zk = rk;
@@ -554,7 +555,6 @@ __device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int 
     cc.y = -cc.y;
     zb = cmul(cc, cplx<T>{rn.x + rn.y, rn.y - rn.x});
   } else {
-    const cplx<T> mkk = mk[k];
     cplx<T> cc = mkk;
     cc.y = -cc.y;
     zk = cmul(cc, cplx<T>{rk.x + rn.y, rk.y - rn.x});
@@ -563,6 +563,33 @@ __device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int 
     zb = cmul(cc, cplx<T>{rn.x + rk.y, rn.y - rk.x});
   }
 }
+
+// mk[k] for the items k a thread visits in the loops over it (w = tid + it
+// NT).  Along z, k = w mod N/2 and the value is a table load.  Along y and x,
+// k = w / P advances by NT/P per iteration: one load, then products with
+// mk[NT/P] (at most 7, a few ulp).
+template <typename A, int N, int P, int NT>
+struct MkSeq {
+  // a thread whose first item is past the table has no valid item (k0 >=
+  // N/2), and when NT/P >= N no thread has a valid item after the first
+  // iteration: both indexes are clamped into the N-entry table
+  static constexpr int SI = NT / P < N ? NT / P : 0;
+  cplx<A> cur, step;
+  __device__ __forceinline__ MkSeq(const cplx<A>* __restrict__ mk, int axis, int tid) {
+    if (axis != 2) {
+      const int k0 = tid / P;
+      cur = mk[k0 < N ? k0 : 0];
+      step = mk[SI];
+    } else {
+      cur = step = cplx<A>{A(0), A(0)};
+    }
+  }
+  // mk[k] of this iteration's item; advance() moves to the next iteration
+  __device__ __forceinline__ cplx<A> get(const cplx<A>* __restrict__ mk, int axis, int k) const {
+    return axis == 2 ? mk[k] : cur;
+  }
+  __device__ __forceinline__ void advance() { cur = cmul(cur, step); }
+};
 
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
 // form complex sequence c.  OP selects a fused operation (FuseOp); the
@@ -692,6 +719,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
         }
       }
     }
+    MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
       const int w = tid + it * NT;
@@ -701,10 +729,11 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
         else { k = w / P; c = w - k * P; }
         const int kb = (k == 0) ? HN : N - k;
         cplx<A> zk, zb;
-        makhoul_pre<A, N>(a.mk, k, rk[it], rn[it], zk, zb);
+        makhoul_pre<A, N>(a.mk, k, ms.get(a.mk, a.axis, k), rk[it], rn[it], zk, zb);
         buf[c * SEQ + lp<SH>(k)] = zk;
         buf[c * SEQ + lp<SH>(kb)] = zb;
       }
+      ms.advance();
     }
   }
   __syncthreads();
@@ -716,6 +745,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
     if constexpr (MID) {
       // forward outputs -> diagonal operator -> inverse pre-twiddle, in place:
       // item k owns positions k and kb of its sequence, so no barrier is needed
+      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -724,8 +754,9 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
+          const cplx<A> mkk = ms.get(a.mk, a.axis, k);
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          makhoul_post<A, N>(a.mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
           const int jj = g.outer;
           const int kc0 = g.k0 + 2 * c < a.nz ? g.k0 + 2 * c : a.nz - 1;
@@ -733,10 +764,11 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           const cplx<A> rk = {mid_apply<OP>(a.f, k, jj, kc0, y0a), mid_apply<OP>(a.f, k, jj, kc1, y1a)};
           const cplx<A> rn = {mid_apply<OP>(a.f, kb, jj, kc0, y0b), mid_apply<OP>(a.f, kb, jj, kc1, y1b)};
           cplx<A> zk, zb;
-          makhoul_pre<A, N>(a.mk, k, rk, rn, zk, zb);
+          makhoul_pre<A, N>(a.mk, k, mkk, rk, rn, zk, zb);
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
+        ms.advance();
       }
       __syncthreads();
     } else if constexpr (AX) {
@@ -748,6 +780,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
       // the one-mode shifts compose with the DST index reversal.  So no
       // exchange between items is needed, and item k rewrites only the
       // positions k, kb it read
+      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -756,8 +789,9 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
+          const cplx<A> mkk = ms.get(a.mk, a.axis, k);
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          makhoul_post<A, N>(a.mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           const int fa = sine ? N - 1 - k : k;
           const int pa = sine_out ? N - 1 - k : k;
           const int pb = sine_out ? N - 1 - kb : kb;
@@ -774,14 +808,16 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
             xb = cplx<A>{wgt * y.x, wgt * y.y};
           }
           cplx<A> zk, zb;
-          makhoul_pre<A, N>(a.mk, k, xk, xb, zk, zb);
+          makhoul_pre<A, N>(a.mk, k, mkk, xk, xb, zk, zb);
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
+        ms.advance();
       }
       __syncthreads();
     } else {
       // ---- post: item k < N/2 writes outputs k and N-k (item 0: 0 and N/2)
+      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -791,7 +827,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          makhoul_post<A, N>(a.mk, k, kb, ms.get(a.mk, a.axis, k), buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)],
+                             y0a, y1a, y0b, y1b);
           const int oa = sine ? (N - 1 - k) : k;
           const int ob = sine ? (N - 1 - kb) : kb;
           const int l0 = 2 * c;
@@ -811,6 +848,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
             store_pair(a, g, l0, ob, y0b, y1b);
           }
         }
+        ms.advance();
       }
       return;
     }
