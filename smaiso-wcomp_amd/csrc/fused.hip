@@ -12,7 +12,9 @@ constexpr int kForceOut = 5;  // fgrad, erho, S_x, S_y, S_z
 template <typename T, int N>
 struct ForcesTile {
   static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
-  static constexpr int NT = 256;
+  // 5 waves: the radix-8 stages of the 5P sequences (N <= 256: 10 x N/8
+  // butterflies) give every thread one butterfly at N = 256 and 512
+  static constexpr int NT = 320;
   static constexpr size_t lds = (size_t)kForceOut * P * seq_len(N, pick_maxr(N, kForceOut * P, NT)) * sizeof(cplx<double>);
   static constexpr bool fits = lds <= 65536;
 };
