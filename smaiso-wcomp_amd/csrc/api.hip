@@ -213,7 +213,7 @@ struct smaiso_handle {
     Fuse f;
     const double dd[3] = {p.dx, p.dy, p.dz};
     const int nn[3] = {p.nx, p.ny, p.nz};
-    for (int a = 0; a < 3; ++a) { f.vq[a] = vq[a]; f.vs[a] = vs[a]; f.dn[a] = dd[a] * nn[a]; }
+    for (int a = 0; a < 3; ++a) { f.vq[a] = vq[a]; f.vs[a] = vs[a]; f.qs[a] = M_PI / (dd[a] * nn[a]); }
     f.q02 = p.q0 * p.q0;
     f.alpha = p.alpha;
     f.scale = scale;

@@ -61,10 +61,11 @@ struct Fuse {
   double c0 = 1.0;       // multiplier of OP_LD_GRAD_* and OP_LD_DIV
   const double* vq[3] = {nullptr, nullptr, nullptr};  // DCT-lattice wavenumbers
   const double* vs[3] = {nullptr, nullptr, nullptr};  // DST-lattice wavenumbers
-  // d_a N_a per axis: vq_a[n] = pi n/(d_a N_a), vs_a[n] = pi (n+1)/(d_a N_a)
-  // (S:447-481); the one-axis passes evaluate these instead of loading the
-  // tables when set (0: use the tables)
-  double dn[3] = {0, 0, 0};
+  // wavenumber spacing pi/(d_a N_a) per axis: vq_a[n] = pi n/(d_a N_a), vs_a[n]
+  // = pi (n+1)/(d_a N_a) (S:447-481); when set the passes evaluate n qs_a
+  // (within an ulp of the tables) instead of loading the tables (0: use the
+  // tables)
+  double qs[3] = {0, 0, 0};
   double q02 = 0, alpha = 0, scale = 0, dt = 0, rho_m = 0, eps = 0;
   // OP_LD_PSI
   const double* psiq = nullptr;

@@ -331,8 +331,18 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
+// Wavenumbers along axis a: from the spacing when set (every fused pass of
+// the stepper), else from the tables
+__device__ __forceinline__ double vq_at(const Fuse& f, int a, int n) {
+  return f.qs[a] != 0.0 ? (double)n * f.qs[a] : f.vq[a][n];
+}
+__device__ __forceinline__ double vs_at(const Fuse& f, int a, int n) {
+  return f.qs[a] != 0.0 ? (double)(n + 1) * f.qs[a] : f.vs[a][n];
+}
+
+// |q|^2 at mode (i, j, k)
 __device__ __forceinline__ double mq2c(const Fuse& f, int i, int j, int k) {
-  return sqd(f.vq[0][i]) + sqd(f.vq[1][j]) + sqd(f.vq[2][k]);
+  return sqd(vq_at(f, 0, i)) + sqd(vq_at(f, 1, j)) + sqd(vq_at(f, 2, k));
 }
 
 // psi update at one mode (S:2177-2190) with FP64 C1, C2: the new psiq from
@@ -383,7 +393,7 @@ __device__ __forceinline__ double load_elem(const PassArgs<T, TO>& a, int i, int
     const int nmax = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
     const long long off = ax == 0 ? (long long)a.ny * a.nz : (ax == 1 ? (long long)a.nz : 1LL);
     const bool ok = na < nmax - 1;
-    const double w = a.f.vs[ax][ok ? na : 0];
+    const double w = vs_at(a.f, ax, ok ? na : 0);
     const long long src_idx = ok ? idx + off : idx;
     double src;
     if constexpr (OP == OP_LD_GRAD_D) src = static_cast<const double*>(a.f.src[0])[src_idx];
@@ -401,8 +411,8 @@ __device__ __forceinline__ double load_elem(const PassArgs<T, TO>& a, int i, int
     const double vx = (double)sx[i > 0 ? idx - (long long)a.ny * a.nz : idx];
     const double vy = (double)sy[j > 0 ? idx - a.nz : idx];
     const double vz = (double)sz[m > 0 ? idx - 1 : idx];
-    return a.f.c0 * (a.f.vq[0][i] * (i > 0 ? vx : 0.0) + a.f.vq[1][j] * (j > 0 ? vy : 0.0) +
-                     a.f.vq[2][m] * (m > 0 ? vz : 0.0));
+    return a.f.c0 * (vq_at(a.f, 0, i) * (i > 0 ? vx : 0.0) + vq_at(a.f, 1, j) * (j > 0 ? vy : 0.0) +
+                     vq_at(a.f, 2, m) * (m > 0 ? vz : 0.0));
   } else if constexpr (OP == OP_LD_PSI) {
     // the new psiq is the transform's input and is also stored (over psiq_old)
     const double r = psi_update_value<T>(a.f, idx, i, j, m);
@@ -487,11 +497,11 @@ __device__ __forceinline__ void store_pair(const PassArgs<T, TO>& a, const TileG
 template <typename T, int OP>
 __device__ __forceinline__ double ax_weight(const Fuse& f, int m, int n) {
   const double c = f.c0 / (2.0 * n);
-  // DCT-lattice wavenumber of index i along the pass axis, evaluated with the
-  // expression of the tables (same value) rather than loaded from them: the
-  // load would stall the operator step behind an L2 round trip
-  const double dn = f.dn[f.axis];
-  auto vq = [&](int i) { return dn != 0.0 ? M_PI * (double)i / dn : f.vq[f.axis][i]; };
+  // DCT-lattice wavenumber of index i along the pass axis, evaluated from the
+  // spacing rather than loaded from the table: the load would stall the
+  // operator step behind an L2 round trip
+  const double qs = f.qs[f.axis];
+  auto vq = [&](int i) { return qs != 0.0 ? (double)i * qs : f.vq[f.axis][i]; };
   if constexpr (OP == OP_AX_GRAD) return -c * vq(m + 1);          // Vs[m] = Vq[m+1]
   else if constexpr (OP == OP_AX_DIV) return c * vq(m);
   else if constexpr (OP == OP_AX_GRAD_T) return -c * vq(m);       // Vs[m-1]
