@@ -251,9 +251,14 @@ __device__ __forceinline__ TileGeo tile_geometry(int axis, int nx, int ny, int n
     t.outer = (int)first;
     t.k0 = 0;
   } else {
-    int tpo = (nz + TL - 1) / TL;
-    int o = blockIdx.x / tpo;
-    int k0 = (blockIdx.x - o * tpo) * TL;
+    // y passes: the plane index i varies fastest over the workgroups, so the
+    // tiles in flight spread over the whole field (as along x) rather than
+    // covering a few consecutive planes
+    const int tpo = (nz + TL - 1) / TL;
+    int o, kc;
+    if (axis == 1) { kc = blockIdx.x / nx; o = blockIdx.x - kc * nx; }
+    else { o = blockIdx.x / tpo; kc = blockIdx.x - o * tpo; }
+    const int k0 = kc * TL;
     t.nvalid = (nz - k0) < TL ? (nz - k0) : TL;
     t.line_step = 1;
     t.outer = o;
