@@ -1095,7 +1095,8 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
 
 // Tile and block size per (length, axis); the LDS tile is FP64 for both
 // storage types, so T does not change the choice.  Measured at 256^3 on
-// MI355X: one wavefront per 2 line pairs along z; along y and x 8 pairs x 256
+// MI355X: one wavefront per line pair along z (half the LDS of two pairs, so
+// more wavefronts per CU hide the loads); along y and x 8 pairs x 256
 // threads.
 template <typename T, int N>
 struct TileChoice {
@@ -1105,7 +1106,7 @@ struct TileChoice {
     while (q * 2 <= p) q *= 2;
     return q;
   }
-  static constexpr int PZ = cap(budget(512) < 2 ? budget(512) : 2);
+  static constexpr int PZ = 1;  // z: one line pair per wavefront
   static constexpr int PS = cap(budget(2048) < 8 ? budget(2048) : 8);
   static constexpr int NTZ = 64;
   static constexpr int NTS = 256;
@@ -1123,7 +1124,7 @@ hipError_t launch_axis_op(int n, const PassIO& io, void* tw, void* mk, int axis,
 // ---------------------------------------------------------------------------
 template <typename T, bool Z, int V>
 struct TileVariant;
-template <typename T> struct TileVariant<T, true, 1> { static constexpr int P = 1, NT = 64; };
+template <typename T> struct TileVariant<T, true, 1> { static constexpr int P = 2, NT = 64; };
 template <typename T> struct TileVariant<T, true, 2> { static constexpr int P = 2, NT = 128; };
 template <typename T> struct TileVariant<T, true, 3> { static constexpr int P = 4, NT = 128; };
 template <typename T> struct TileVariant<T, false, 1> { static constexpr int P = 4, NT = 128; };
