@@ -1096,8 +1096,8 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
 // Tile and block size per (length, axis); the LDS tile is FP64 for both
 // storage types, so T does not change the choice.  Measured at 256^3 on
 // MI355X: one wavefront per line pair along z (half the LDS of two pairs, so
-// more wavefronts per CU hide the loads); along y and x 8 pairs x 256
-// threads.
+// more wavefronts per CU hide the loads), i.e. 256 elements per wavefront;
+// along y and x 8 pairs x 256 threads.
 template <typename T, int N>
 struct TileChoice {
   static constexpr int cap(int p) { return p < 1 ? 1 : p; }
@@ -1106,7 +1106,9 @@ struct TileChoice {
     while (q * 2 <= p) q *= 2;
     return q;
   }
-  static constexpr int PZ = 1;  // z: one line pair per wavefront
+  // z: about 256 elements per wavefront (one line pair at N = 256, more for
+  // shorter lines so every thread keeps a butterfly)
+  static constexpr int PZ = cap(budget(256));
   static constexpr int PS = cap(budget(2048) < 8 ? budget(2048) : 8);
   static constexpr int NTZ = 64;
   static constexpr int NTS = 256;
