@@ -145,6 +145,7 @@ struct smaiso_handle {
     c.q0 = p.q0; c.q02 = p.q0 * p.q0; c.rho_m = rho_m;
     for (int a = 0; a < 3; ++a) {
       c.vq[a] = vq[a]; c.vs[a] = vs[a];
+      c.qs[a] = M_PI / ((a == 0 ? p.dx : a == 1 ? p.dy : p.dz) * (a == 0 ? p.nx : a == 1 ? p.ny : p.nz));
       c.gq[a] = gq[a]; c.gs[a] = gs[a];
     }
     c.mc = mc;

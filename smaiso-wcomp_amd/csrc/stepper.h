@@ -28,6 +28,7 @@ struct StepConsts {
   double dt, alpha, beta, gamma, eps, q0, q02, rho_m;
   const double* vq[3];  // DCT-lattice wavenumbers pi n/(d N), S:447-463
   const double* vs[3];  // DST-lattice wavenumbers pi (n+1)/(d N), S:465-481
+  double qs[3];         // their spacing pi/(d N): vq[n] ~ n qs, vs[n] ~ (n+1) qs (within an ulp)
   // one-axis factors of the Gaussian exp(-1.57*1.57*2 |q|^2) (S:1281 and the
   // velocity solve's exponentials): gq_a[n] = exp(-1.57*1.57*2 vq_a[n]^2),
   // gs_a[n] the same of vs_a[n]; the 3-D factor is the product over the axes

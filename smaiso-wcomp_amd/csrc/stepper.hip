@@ -146,8 +146,10 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const MemberConsts mm = c.mc[blockIdx.y];
   const Idx p = decompose(idx, c.ny, c.nz);
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
-  const double qx = c.vq[0][p.i], qy = c.vq[1][p.j], qz = c.vq[2][p.k];
-  const double wx = c.vs[0][p.i], wy = c.vs[1][p.j], wz = c.vs[2][p.k];
+  // wavenumbers from the spacings (no table loads: this kernel is bound by its
+  // memory instructions)
+  const double qx = p.i * c.qs[0], qy = p.j * c.qs[1], qz = p.k * c.qs[2];
+  const double wx = (p.i + 1) * c.qs[0], wy = (p.j + 1) * c.qs[1], wz = (p.k + 1) * c.qs[2];
   const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
   const double rnu = mm.rnu, rlam = mm.rlam;
   const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
@@ -156,14 +158,14 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same double),
   // which is also the G of its fgrad^ term: one evaluation serves both, three
   // per point.  FP32-mixed fields take them as products of the one-axis
-  // tables (the kernel is ALU-bound there); FP64 fields evaluate exp (the
-  // table loads measured slower there)
+  // tables (measured faster there); FP64 fields evaluate exp (the table loads
+  // measured slower there)
   constexpr bool TAB = sizeof(T) == 4;
   auto g3 = [&](int i, int j, int k) -> double {
     if constexpr (TAB) {
       return c.gq[0][i] * c.gq[1][j] * c.gq[2][k];
     } else {
-      const double x = c.vq[0][i], y = c.vq[1][j], z = c.vq[2][k];
+      const double x = i * c.qs[0], y = j * c.qs[1], z = k * c.qs[2];
       return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
     }
   };
