@@ -32,6 +32,7 @@ struct StepConsts {
   // one-axis factors of the Gaussian exp(-1.57*1.57*2 |q|^2) (S:1281 and the
   // velocity solve's exponentials): gq_a[n] = exp(-1.57*1.57*2 vq_a[n]^2),
   // gs_a[n] the same of vs_a[n]; the 3-D factor is the product over the axes
+  // (the adjoint's kernels take it so; the forward velocity solve evaluates exp)
   const double* gq[3];
   const double* gs[3];
   const MemberConsts* mc;  // device array, nmem entries

@@ -155,19 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
   // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q|^2).  A
   // component's factor on its own (shifted) lattice at n equals the factor on
-  // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same double),
-  // which is also the G of its fgrad^ term: one evaluation serves both, three
-  // per point.  FP32-mixed fields take them as products of the one-axis
-  // tables (measured faster there); FP64 fields evaluate exp (the table loads
-  // measured slower there)
-  constexpr bool TAB = sizeof(T) == 4;
+  // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same
+  // wavenumber), which is also the G of its fgrad^ term: one evaluation serves
+  // both, three per point.  exp is evaluated rather than taken as a product of
+  // the one-axis tables: the table loads measured slower in both precisions
   auto g3 = [&](int i, int j, int k) -> double {
-    if constexpr (TAB) {
-      return c.gq[0][i] * c.gq[1][j] * c.gq[2][k];
-    } else {
-      const double x = i * c.qs[0], y = j * c.qs[1], z = k * c.qs[2];
-      return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
-    }
+    const double x = i * c.qs[0], y = j * c.qs[1], z = k * c.qs[2];
+    return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
   };
 
   // x component, i <= Nx-2 (quirk Q2), S:1583-1709
