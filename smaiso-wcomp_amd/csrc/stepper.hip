@@ -122,10 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
   // XCD-aware order: workgroups are dealt to the 8 XCDs round robin.  When
   // an i plane splits into 8 equal runs of whole workgroups, XCD x takes run
-  // x of every plane (a band of j lines), plane after plane: the i+1 lines it
-  // reads as neighbours are read again as centres one band later (~1/8 plane
-  // of traffic), inside its own 4 MiB L2.  Otherwise each XCD walks a
-  // contiguous range of lines.
+  // x of every plane (a band of j lines), plane after plane, so the j+-1
+  // neighbour lines it reads are mostly its own band's, in its own 4 MiB L2.
+  // Otherwise each XCD walks a contiguous range of lines.
   const unsigned nb = gridDim.x;
   const unsigned plane = (unsigned)c.ny * c.nz;
   unsigned b = blockIdx.x;
@@ -149,10 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   // wavenumbers from the spacings (no table loads: this kernel is bound by its
   // memory instructions)
   const double qx = p.i * c.qs[0], qy = p.j * c.qs[1], qz = p.k * c.qs[2];
-  const double wx = (p.i + 1) * c.qs[0], wy = (p.j + 1) * c.qs[1], wz = (p.k + 1) * c.qs[2];
+  const double wy = (p.j + 1) * c.qs[1], wz = (p.k + 1) * c.qs[2];
   const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
   const double rnu = mm.rnu, rlam = mm.rlam;
-  const double Sx = (double)sxq[idx], Sy = (double)syq[idx], Sz = (double)szq[idx];
+  const double Sy = (double)syq[idx], Sz = (double)szq[idx];
   // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q|^2).  A
   // component's factor on its own (shifted) lattice at n equals the factor on
   // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same
@@ -164,18 +163,28 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
   };
 
-  // x component, i <= Nx-2 (quirk Q2), S:1583-1709
-  double ox = 0, rx = 0;
-  if (p.i < c.nx - 1) {
-    const double Syp = (p.j > 0) ? (double)syq[idx + sI - sJ] : 0.0;
-    const double Szp = (p.k > 0) ? (double)szq[idx + sI - 1] : 0.0;
+  // x component (S:1583-1709) of point (i-1, j, k), computed by this thread:
+  // its i+1 neighbours (S_y, S_z, fgrad^, erho^) are then on this thread's
+  // plane, and only S_x comes from plane i-1, read by this thread alone (the
+  // components at one point read no other plane's S_y, S_z, fgrad^ or erho^).
+  // It is zero on the plane i = Nx-1 (quirk Q2), written by the i = 0 threads.
+  if (p.i > 0) {
+    const double Sx = (double)sxq[idx - sI];
+    const double wx = p.i * c.qs[0];  // vs_x[i-1]
+    const double Syp = (p.j > 0) ? (double)syq[idx - sJ] : 0.0;
+    const double Szp = (p.k > 0) ? (double)szq[idx - 1] : 0.0;
     const double dot = wx * Sx + qy * Syp + qz * Szp;
     const double inv = 1.0 / (wx * wx + qy2 + qz2);
     const double cm1 = c.scale * inv, cm2 = wx * inv;
-    const double gx = g3(p.i + 1, p.j, p.k);
-    const double fg = (double)fgq[idx + sI] + (double)erq[idx + sI] * gx;
-    ox = (cm1 * rnu) * (Sx - cm2 * dot);
-    rx = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
+    const double gx = g3(p.i, p.j, p.k);
+    const double fg = (double)fgq[idx] + (double)erq[idx] * gx;
+    const long long ox = mo + idx - sI;
+    vsolx[ox] = (T)((cm1 * rnu) * (Sx - cm2 * dot));
+    virrx[ox] = (T)((cm1 * rlam) * (wx * fg + cm2 * dot * gx));
+  } else {
+    const long long ox = mo + idx + (long long)(c.nx - 1) * sI;
+    vsolx[ox] = (T)0;
+    virrx[ox] = (T)0;
   }
   double oy = 0, ry = 0, oz = 0, rz = 0;
   if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
@@ -209,7 +218,6 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
     }
   }
   const long long o = mo + idx;
-  vsolx[o] = (T)ox; virrx[o] = (T)rx;
   vsoly[o] = (T)oy; virry[o] = (T)ry;
   vsolz[o] = (T)oz; virrz[o] = (T)rz;
 }
