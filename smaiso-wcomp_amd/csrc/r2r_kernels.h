@@ -580,30 +580,52 @@ __device__ __forceinline__ void makhoul_pre(const cplx<T>* __restrict__ mk, int 
 }
 
 // mk[k] for the items k a thread visits in the loops over it (w = tid + it
-// NT).  Along z, k = w mod N/2 and the value is a table load.  Along y and x,
-// k = w / P advances by NT/P per iteration: one load, then products with
-// mk[NT/P] (at most 7, a few ulp).
+// NT), from one table load and products with a uniform step (a few ulp):
+// - along y and x, k = w / P advances by NT/P per iteration (at most 7
+//   products);
+// - along z, k = w mod N/2: when NT is a multiple of N/2, k = tid mod N/2 in
+//   every iteration; when N/2 is a multiple of NT, k = tid + (it mod
+//   (N/2)/NT) NT, restarting at tid with each new sequence; other lengths
+//   load mk[k] per item.
 template <typename A, int N, int P, int NT>
 struct MkSeq {
+  static constexpr int HN = N / 2;
+  static constexpr bool ZREP = NT % HN == 0;
+  static constexpr bool ZCYC = !ZREP && HN % NT == 0;
+  static constexpr int ZQ = ZCYC ? HN / NT : 1;
   // a thread whose first item is past the table has no valid item (k0 >=
   // N/2), and when NT/P >= N no thread has a valid item after the first
   // iteration: both indexes are clamped into the N-entry table
   static constexpr int SI = NT / P < N ? NT / P : 0;
-  cplx<A> cur, step;
+  cplx<A> base, cur, step;
   __device__ __forceinline__ MkSeq(const cplx<A>* __restrict__ mk, int axis, int tid) {
+    base = cur = step = cplx<A>{A(0), A(0)};
     if (axis != 2) {
       const int k0 = tid / P;
       cur = mk[k0 < N ? k0 : 0];
       step = mk[SI];
-    } else {
-      cur = step = cplx<A>{A(0), A(0)};
+    } else if constexpr (ZREP) {
+      cur = mk[tid % HN];
+    } else if constexpr (ZCYC) {
+      base = cur = mk[tid];  // tid < NT <= N/2
+      step = mk[NT];
     }
   }
-  // mk[k] of this iteration's item; advance() moves to the next iteration
+  // mk[k] of this iteration's item
   __device__ __forceinline__ cplx<A> get(const cplx<A>* __restrict__ mk, int axis, int k) const {
-    return axis == 2 ? mk[k] : cur;
+    if constexpr (!ZREP && !ZCYC) {
+      if (axis == 2) return mk[k];
+    }
+    return cur;
   }
-  __device__ __forceinline__ void advance() { cur = cmul(cur, step); }
+  // move to iteration it + 1
+  __device__ __forceinline__ void advance(int axis, int it) {
+    if (axis != 2) {
+      cur = cmul(cur, step);
+    } else if constexpr (ZCYC) {
+      cur = ((it + 1) % ZQ == 0) ? base : cmul(cur, step);
+    }
+  }
 };
 
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
@@ -748,7 +770,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
         buf[c * SEQ + lp<SH>(k)] = zk;
         buf[c * SEQ + lp<SH>(kb)] = zb;
       }
-      ms.advance();
+      ms.advance(a.axis, it);
     }
   }
   __syncthreads();
@@ -783,7 +805,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
-        ms.advance();
+        ms.advance(a.axis, it);
       }
       __syncthreads();
     } else if constexpr (AX) {
@@ -827,7 +849,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
-        ms.advance();
+        ms.advance(a.axis, it);
       }
       __syncthreads();
     } else {
@@ -863,7 +885,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
             store_pair(a, g, l0, ob, y0b, y1b);
           }
         }
-        ms.advance();
+        ms.advance(a.axis, it);
       }
       return;
     }
