@@ -43,6 +43,8 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
   const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
   const long long base = (long long)blockIdx.y * nl * N + first * N;
   const MemberConsts mc = c.mc[blockIdx.y];
+  StageTw<A, N, NF * P, NT, MAXR> stw;
+  stw.load(tw, tid);
 
   constexpr int E = TL * N;
   constexpr int IT = (E + NT - 1) / NT;
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
     }
   }
   __syncthreads();
-  fft_stage<A, N, NF * P, NT, MAXR, -1, 0>(buf, tw);
+  fft_stage<A, N, NF * P, NT, MAXR, -1, 0>(buf, tw, stw);
 
   constexpr int W = NF * P * HN;
   constexpr int ITW = (W + NT - 1) / NT;

@@ -274,11 +274,37 @@ __device__ __forceinline__ TileGeo tile_geometry(int axis, int nx, int ny, int n
   return t;
 }
 
+// A thread's twiddle base w = tw[k STRIDE] of each stage, loaded once before
+// the tile's data: in a stage whose butterfly count per sequence J divides
+// NT, every iteration of the thread takes butterfly j = tid mod J and so the
+// same k = j mod NS.  Stages where J does not divide NT load w in the stage.
+template <typename T, int N, int P, int NT, int MAXR>
+struct StageTw {
+  static constexpr int kStages = 6;
+  cplx<T> w[kStages];
+  template <int S>
+  static constexpr bool fixed() {
+    return fft_radix(N, MAXR, S) != 0 && ns_before(N, MAXR, S) > 1 && NT % (N / fft_radix(N, MAXR, S)) == 0;
+  }
+  __device__ __forceinline__ void load(const cplx<T>* __restrict__ tw, int tid) { load_s<0>(tw, tid); }
+  template <int S>
+  __device__ __forceinline__ void load_s(const cplx<T>* __restrict__ tw, int tid) {
+    if constexpr (S < kStages && fft_radix(N, MAXR, S) != 0) {
+      if constexpr (fixed<S>()) {
+        constexpr int R = fft_radix(N, MAXR, S), NS = ns_before(N, MAXR, S), J = N / R;
+        w[S] = tw[((tid % J) % NS) * (N / (NS * R))];
+      }
+      load_s<S + 1>(tw, tid);
+    }
+  }
+};
+
 // Sequence p of the tile lives at buf[p * SEQ], item x of it at
 // buf[p * SEQ + lp<SH>(x)] (SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR)).
 template <typename T, int N, int P, int NT, int MAXR, int SIGN, int S, int SEQ = seq_len(N, MAXR),
           int SH = pad_shift(N, MAXR)>
-__device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restrict__ tw,
+                                          const StageTw<T, N, P, NT, MAXR>& pre) {
   constexpr int R = fft_radix(N, MAXR, S);
   if constexpr (R != 0) {
     constexpr int NS = ns_before(N, MAXR, S);
@@ -299,8 +325,9 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
         if constexpr (NS > 1) {
           // twiddles w^q, q = 1..R-1, of w = tw[k STRIDE]: one table load and
           // powers by products (a few ulp from the table's entries)
-          const int k = j % NS;
-          cplx<T> w1 = tw[k * STRIDE];
+          cplx<T> w1;
+          if constexpr (StageTw<T, N, P, NT, MAXR>::template fixed<S>()) w1 = pre.w[S];
+          else w1 = tw[(j % NS) * STRIDE];
           if constexpr (SIGN > 0) w1.y = -w1.y;
           cplx<T> t = w1;
           v[it][1] = cmul(v[it][1], t);
@@ -327,7 +354,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
       }
     }
     __syncthreads();
-    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1, SEQ, SH>(buf, tw);
+    fft_stage<T, N, P, NT, MAXR, SIGN, S + 1, SEQ, SH>(buf, tw, pre);
   }
 }
 
@@ -657,6 +684,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
   // sine_out: DST on the store side
   const bool sine = AX ? (OP == OP_AX_DIV || OP == OP_AX_GRAD_T) : (!MID && (a.kind == RODFT10 || a.kind == RODFT01));
   const bool sine_out = AX ? (OP == OP_AX_GRAD || OP == OP_AX_DIV_T) : sine;
+  StageTw<A, N, P, NT, MAXR> stw;
+  stw.load(a.tw, tid);
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
@@ -776,7 +805,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
   __syncthreads();
 
   if (fwd) {
-    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, a.tw);
+    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, a.tw, stw);
     constexpr int W = P * HN;
     constexpr int ITW = (W + NT - 1) / NT;
     if constexpr (MID) {
@@ -892,7 +921,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
   }
 
   // ---- inverse FFT and store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n}
-  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, a.tw);
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, a.tw, stw);
   constexpr int WO = P * HN;
   constexpr int ITO = (WO + NT - 1) / NT;
   const A s = sine_out ? A(-1) : A(1);
