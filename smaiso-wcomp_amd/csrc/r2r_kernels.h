@@ -686,6 +686,8 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
   const bool sine_out = AX ? (OP == OP_AX_GRAD || OP == OP_AX_DIV_T) : sine;
   StageTw<A, N, P, NT, MAXR> stw;
   stw.load(a.tw, tid);
+  // the Makhoul factors' first loads, also ahead of the tile's data
+  const MkSeq<A, N, P, NT> ms0(a.mk, a.axis, tid);
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
         }
       }
     }
-    MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
+    MkSeq<A, N, P, NT> ms = ms0;
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
       const int w = tid + it * NT;
@@ -811,7 +813,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
     if constexpr (MID) {
       // forward outputs -> diagonal operator -> inverse pre-twiddle, in place:
       // item k owns positions k and kb of its sequence, so no barrier is needed
-      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
+      MkSeq<A, N, P, NT> ms = ms0;
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -846,7 +848,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
       // the one-mode shifts compose with the DST index reversal.  So no
       // exchange between items is needed, and item k rewrites only the
       // positions k, kb it read
-      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
+      MkSeq<A, N, P, NT> ms = ms0;
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
@@ -883,7 +885,7 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
       __syncthreads();
     } else {
       // ---- post: item k < N/2 writes outputs k and N-k (item 0: 0 and N/2)
-      MkSeq<A, N, P, NT> ms(a.mk, a.axis, tid);
+      MkSeq<A, N, P, NT> ms = ms0;
 #pragma unroll
       for (int it = 0; it < ITW; ++it) {
         const int w = tid + it * NT;
