@@ -655,11 +655,21 @@ struct MkSeq {
   }
 };
 
+// Minimum waves per SIMD a pass is compiled for (its register cap), for
+// tiles whose threads hold at most 8 values (2 P N / NT): the 256-thread y/x
+// tiles are held to 4 by their LDS anyway, so 128 VGPRs cost nothing there;
+// one-wave z tiles fit 6 waves (5 for the one-axis operators, which would
+// spill at 6).  Larger per-thread tiles and other block sizes keep the
+// compiler's choice.
+constexpr int pass_min_waves(int nt, int op, int per_thread) {
+  return per_thread > 8 ? 1 : nt >= 256 ? 4 : nt == 64 ? (op >= OP_AX_GRAD ? 5 : 6) : 1;
+}
+
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
 // form complex sequence c.  OP selects a fused operation (FuseOp); the
 // output is stored as TO.
 template <typename T, int N, int P, int NT, int OP, typename TO = T>
-__global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs<T, TO> a) {
+__global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void fft_pass_kernel(PassArgs<T, TO> a) {
   constexpr int TL = 2 * P;
   constexpr int MAXR = pick_maxr(N, P, NT);
   constexpr int SEQ = seq_len(N, MAXR);
