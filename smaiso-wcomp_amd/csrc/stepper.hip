@@ -113,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(
 // ---------------------------------------------------------------------------
 // a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974)
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_velocity(
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_velocity(
     StepConsts c, const T* __restrict__ sxq_, const T* __restrict__ syq_,
     const T* __restrict__ szq_, const T* __restrict__ fgq_, const T* __restrict__ erq_,
     T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
@@ -128,13 +128,13 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   const unsigned nb = gridDim.x;
   const unsigned plane = (unsigned)c.ny * c.nz;
   unsigned b = blockIdx.x;
-  if (plane % (8u * kBlock) == 0) {
-    const unsigned bpp = plane / kBlock, bpx = bpp >> 3, s = b >> 3;
+  if (plane % (8u * BS) == 0) {
+    const unsigned bpp = plane / BS, bpx = bpp >> 3, s = b >> 3;
     b = (s / bpx) * bpp + (b & 7u) * bpx + (s % bpx);
   } else if ((nb & 7u) == 0) {
     b = (b & 7u) * (nb >> 3) + (b >> 3);
   }
-  const unsigned idx = b * kBlock + threadIdx.x;
+  const unsigned idx = b * BS + threadIdx.x;
   if (idx >= n3) return;
   const long long mo = member_offset(c);
   const T* sxq = sxq_ + mo;
@@ -397,8 +397,18 @@ template <typename T>
 void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                               const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
                               T* virry, T* virrz, hipStream_t s) {
-  hipLaunchKernelGGL(k_velocity<T>, grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
-                     vsolx, vsoly, vsolz, virrx, virry, virrz);
+  // 1024-point workgroups (four lines of 256) when they tile the field: the
+  // j+-1 lines a point reads are then mostly read by its own workgroup, on
+  // the same CU at the same time
+  constexpr int BS = 1024;
+  const long long n3 = n3_of(c);
+  if (n3 % BS == 0) {
+    hipLaunchKernelGGL((k_velocity<T, BS>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s, c, sxq,
+                       syq, szq, fgq, erq, vsolx, vsoly, vsolz, virrx, virry, virrz);
+  } else {
+    hipLaunchKernelGGL((k_velocity<T, kBlock>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
+                       vsolx, vsoly, vsolz, virrx, virry, virrz);
+  }
 }
 template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
