@@ -168,7 +168,11 @@ int smaiso_set_graph(smaiso_handle* h, int32_t enable);
 
 /* Copy a field (SMAISO_FIELD_*) of the last completed step to host FP64 out
  * (Nx*Ny*Nz values, reference layout).  Replaces reading the reference's
- * arrays (psi_local, rho_local, ...) and its slice writers (S:2323-2446). */
+ * arrays (psi_local, rho_local, ...) and its slice writers (S:2323-2446).
+ * The state fields (PSI, RHO, PSIQ, PSIQ_OLD, F_RHO) are readable once the
+ * state is set; every other field is produced by a step and returns
+ * SMAISO_E_STATE until a step has completed since the last set_state or
+ * checkpoint load (NLQ is also restored by a checkpoint load). */
 int smaiso_get_field(smaiso_handle* h, int32_t field, double* out);
 
 /* As smaiso_get_field into a device FP64 buffer. */
@@ -176,9 +180,10 @@ int smaiso_get_field_device(smaiso_handle* h, int32_t field, void* out_dev);
 
 /* Diagnostics of the last completed step, S:2202-2263: L1 = sum|psi-psi_old|/
  * (dt sum|psi|), mass = sum(rho) dx dy dz, free energy (S:2227-2233).  FP64
- * compensated sums on the device.  Returns SMAISO_E_NONFINITE (values still
- * written) when any of them is NaN or Inf (the reference's loop silently
- * exits in that case, S:1081). */
+ * compensated sums on the device.  SMAISO_E_STATE until a step has completed
+ * since the last set_state or checkpoint load.  Returns SMAISO_E_NONFINITE
+ * (values still written) when any of them is NaN or Inf (the reference's loop
+ * silently exits in that case, S:1081). */
 int smaiso_diagnostics(smaiso_handle* h, double* l1, double* mass, double* energy);
 
 /* Standalone 3-D r2r transform with FFTW semantics (unnormalised), the
@@ -285,7 +290,10 @@ int smaiso_surface_height(smaiso_handle* h, double threshold, double* surf);
 /* Full-state checkpoint of a single handle (fixes the reference's psi-only
  * restart, S:727-796 and S:2334-2345): psiq, psiq_old, rho, f_rho, psi, the
  * last step's N^l and nLoop, binary.  Loading needs the same grid and
- * precision; stepping on after a load is bit-identical to never stopping. */
+ * precision; stepping on after a load is bit-identical to never stopping.
+ * A file that is not a checkpoint, or one of another grid or precision,
+ * returns SMAISO_E_INVALID and leaves the handle as it was; a truncated
+ * one returns SMAISO_E_INVALID and leaves the handle without a state. */
 int smaiso_save_checkpoint(smaiso_handle* h, const char* path);
 int smaiso_load_checkpoint(smaiso_handle* h, const char* path);
 
@@ -319,8 +327,9 @@ const char* smaiso_ensemble_last_error(const smaiso_ensemble* e);
 int smaiso_ensemble_groups(const smaiso_ensemble* e);
 /* As smaiso_set_state for one member; every member must be set before the
  * first step.  Setting a member restarts the step count (nLoop, with quirk
- * Q5's first-step history) of the batch it belongs to, so set every member of
- * a batch before stepping it again. */
+ * Q5's first-step history) of the batch it belongs to.  If that batch has
+ * stepped since it was last set, its other members count as unset again and
+ * smaiso_ensemble_step returns SMAISO_E_STATE until each has been set. */
 int smaiso_ensemble_set_state(smaiso_ensemble* e, int32_t member, const double* psi,
                               const double* rho);
 int smaiso_ensemble_set_graph(smaiso_ensemble* e, int32_t enable);

@@ -70,6 +70,11 @@ struct smaiso_handle {
   std::vector<char> member_set;
   long long nloop = 0;  // steps since the last set_state (the reference's nLoop)
   long long epoch = 0;  // steps since creation: its parity names the buffer rotation
+  // The work fields hold the intermediates of a completed step (amp, L psi,
+  // v, mu, psi_prev, ...): false after set_state or a checkpoint load until
+  // the next step, so getters and diagnostics never read fields no step wrote.
+  bool derived = false;
+  bool nlq_valid = false;  // N^l of the last step: written by a step or a checkpoint load
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
 
@@ -686,6 +691,8 @@ int run_step(smaiso_handle* h) {
     else HIPTRY(h, h->xwf(kICT, g, h->psi));
   }
   std::swap(h->psiq, h->psiq_old);
+  h->derived = true;
+  h->nlq_valid = true;
   return SMAISO_OK;
 }
 
@@ -714,15 +721,29 @@ int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, 
   HIPTRY(h, hipGetLastError());
   HIPTRY(h, hipStreamSynchronize(s));
   h->collect_timing();
-  h->nloop = 0;  // the whole group restarts its step count (quirk Q5)
+  // the whole group restarts its step count (quirk Q5): members of a group
+  // that has stepped since its last set lose their history too, so they must
+  // be set again before the group steps
+  if (h->nloop > 0)
+    for (int q = 0; q < h->nmem; ++q)
+      if (q != m) h->member_set[q] = 0;
+  h->nloop = 0;
   h->member_set[m] = 1;
   h->diag_epoch = -1;
+  h->derived = false;
+  h->nlq_valid = false;
   return SMAISO_OK;
 }
 
 // Produce field `fid` of all members as FP64 on the device; returns the
 // device pointer of member 0 (possibly a handle scratch buffer).
 int field_device(smaiso_handle* h, int fid, const double** out) {
+  const bool state = fid == SMAISO_FIELD_PSI || fid == SMAISO_FIELD_RHO || fid == SMAISO_FIELD_PSIQ ||
+                     fid == SMAISO_FIELD_PSIQ_OLD || fid == SMAISO_FIELD_F_RHO;
+  if (fid >= 0 && fid < SMAISO_FIELD_COUNT && !state &&
+      !(fid == SMAISO_FIELD_NLQ ? h->nlq_valid : h->derived))
+    return fail(h, SMAISO_E_STATE,
+                fmt("field %d is produced by a step: none has run since the state was set or loaded", fid));
   const long long nt = h->ntot();
   hipStream_t s = h->stream;
   auto as64 = [&](void* work) -> const double* {
@@ -789,8 +810,9 @@ int get_field(smaiso_handle* h, int m, int field, void* out, hipMemcpyKind kind)
 }
 
 int diagnostics(smaiso_handle* h, int m, double* l1, double* mass, double* energy) {
-  if (!h->all_set() || h->nloop < 1)
-    return fail(h, SMAISO_E_STATE, "diagnostics need at least one completed step");
+  if (!h->all_set() || h->nloop < 1 || !h->derived)
+    return fail(h, SMAISO_E_STATE,
+                "diagnostics need a completed step since the state was set or loaded");
   HIPTRY(h, hipSetDevice(h->device));
   if (h->diag_epoch != h->epoch) {
     const StepConsts c = h->consts();
@@ -1621,6 +1643,8 @@ int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const 
   // the stepper holds an intermediate state now
   h->member_set[0] = 0;
   h->diag_epoch = -1;
+  h->derived = false;
+  h->nlq_valid = false;
   if (loss) *loss = acc[6] + acc[7];
   if (dzeta) *dzeta = acc[0] + acc[1];
   if (dnu) *dnu = acc[2] + acc[3];
@@ -1761,6 +1785,10 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
     std::fclose(f);
     return fail(h, SMAISO_E_INVALID, fmt("%s is not a smaiso checkpoint", path));
   }
+  if (hd.nloop < 0) {
+    std::fclose(f);
+    return fail(h, SMAISO_E_INVALID, fmt("%s: negative step count %lld", path, (long long)hd.nloop));
+  }
   if (hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz || hd.precision != h->p.precision ||
       hd.elem != (int32_t)h->tsz) {
     std::fclose(f);
@@ -1769,6 +1797,12 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
                     hd.nx, hd.ny, hd.nz, hd.precision, h->g.nx, h->g.ny, h->g.nz, h->p.precision));
   }
   HIPTRY(h, hipStreamSynchronize(h->stream));
+  // from here on the handle's state is being overwritten: it counts as set
+  // again only when the whole file has been read
+  h->member_set[0] = 0;
+  h->derived = false;
+  h->nlq_valid = false;
+  h->diag_epoch = -1;
   const size_t n3 = (size_t)h->n3();
   std::vector<char> buf(n3 * 8);
   void* arrays[6] = {h->psiq, h->psiq_old, h->rho, h->f_rho, h->psi, h->nlq};
@@ -1782,7 +1816,7 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
   if (!ok) return fail(h, SMAISO_E_INVALID, fmt("%s is truncated", path));
   h->nloop = hd.nloop;
   h->member_set[0] = 1;
-  h->diag_epoch = -1;
+  h->nlq_valid = true;  // the checkpoint holds the last step's N^l
   return SMAISO_OK;
 }
 

@@ -212,6 +212,33 @@ def test_ensemble_errors():
         assert en.diagnostics(0) == en.diagnostics(1)
 
 
+
+def test_ensemble_setting_one_member_of_a_stepped_batch_unsets_the_others():
+    """Setting one member restarts its batch's step count (and with it the
+    first-step history of every member), so the batch's other members must be
+    set again before it steps."""
+    p = S.Params(nx=16, ny=16, nz=16, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
+    q = S.Params(nx=16, ny=16, nz=32, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
+    with S.Ensemble([p, p, q]) as en:
+        ics = [ic_for(m_p, 70 + m) for m, m_p in enumerate((p, p, q))]
+        for m, (psi, rho) in enumerate(ics):
+            en.set_state(m, psi, rho)
+        en.step(3)
+        en.set_state(0, *ics[0])
+        with pytest.raises(S.SmaisoError) as e:
+            en.step(1)
+        assert e.value.code == S.E_STATE
+        with pytest.raises(S.SmaisoError) as e:
+            en.field(1, "psi")
+        assert e.value.code == S.E_STATE
+        assert en.nloop(2) == 3  # the other batch keeps its state
+        en.set_state(1, *ics[1])
+        en.step(2)
+        st = S.Stepper(p)
+        st.set_state(*ics[1])
+        st.step(2)
+        assert np.array_equal(en.field(1, "psi"), st.field("psi"))
+
 @pytest.mark.parametrize("precision", [S.FP64, S.FP32_MIXED])
 def test_autotune_picks_fastest_and_keeps_parity(precision):
     sp, op = params_pair(nx=64, ny=48, nz=32, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0,

@@ -81,6 +81,54 @@ def test_checkpoint_continuation_is_bit_identical(tmp_path, precision):
     assert e.value.code == S.E_INVALID
 
 
+
+def test_derived_fields_need_a_step_after_set_or_load(tmp_path):
+    """Fields a step produces (amp, L psi, v, psi_old, ...) and the diagnostics
+    are refused until a step has run since set_state or a checkpoint load; the
+    state fields are readable at once."""
+    p = _params()
+    psi, rho = _ic(p)
+    a = S.Stepper(p)
+    a.set_state(psi, rho)
+    for name in ("amp", "lpsi", "vx", "psi_old", "energy", "nlq", "mu"):
+        with pytest.raises(S.SmaisoError) as e:
+            a.field(name)
+        assert e.value.code == S.E_STATE, name
+    with pytest.raises(S.SmaisoError) as e:
+        a.diagnostics()
+    assert e.value.code == S.E_STATE
+    assert np.array_equal(a.field("rho"), rho)
+    a.step(4)
+    ck = tmp_path / "state.ck"
+    a.save_checkpoint(ck)
+    b = S.Stepper(p)
+    b.load_checkpoint(ck)
+    with pytest.raises(S.SmaisoError) as e:
+        b.diagnostics()
+    assert e.value.code == S.E_STATE
+    with pytest.raises(S.SmaisoError) as e:
+        b.field("amp")
+    assert e.value.code == S.E_STATE
+    assert np.array_equal(b.field("nlq"), a.field("nlq"))  # restored by the load
+    assert np.array_equal(b.field("psi"), a.field("psi"))
+    a.step(1)
+    b.step(1)
+    assert a.diagnostics() == b.diagnostics()
+    assert np.array_equal(a.field("amp"), b.field("amp"))
+    # a truncated checkpoint leaves the handle without a state
+    data = ck.read_bytes()
+    cut = tmp_path / "cut.ck"
+    cut.write_bytes(data[: len(data) // 2])
+    with pytest.raises(S.SmaisoError) as e:
+        b.load_checkpoint(cut)
+    assert e.value.code == S.E_INVALID and "truncated" in str(e.value)
+    with pytest.raises(S.SmaisoError) as e:
+        b.field("psi")
+    assert e.value.code == S.E_STATE
+    with pytest.raises(S.SmaisoError) as e:
+        b.step(1)
+    assert e.value.code == S.E_STATE
+
 def _lines(path):
     with open(path) as f:
         return [float(x) for x in f.read().split()]
