@@ -199,6 +199,22 @@ int smaiso_r2r3d(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
 int smaiso_r2r3d_device(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
                         const void* in_dev, void* out_dev, int32_t dtype, void* stream);
 
+/* No counterpart in the reference: smaiso_r2r3d for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and results
+ * (bit-identical) as smaiso_r2r3d, but the device staging of the data and
+ * the transform tables live in a fixed device array of about 24 MiB that
+ * each GPU the library runs on holds until the process ends; the call makes
+ * no hipMalloc/hipFree.  Limits: at most SMAISO_NOALLOC_MAX_POINTS points
+ * and SMAISO_NOALLOC_MAX_LEN per axis, else SMAISO_E_INVALID.  Every call on
+ * one GPU uses that GPU's one array: calls from different threads would race
+ * on it, so the library serialises them (each call holds a process-wide lock
+ * until its result is on the host).  Uses the null stream; synchronous. */
+#define SMAISO_NOALLOC_MAX_POINTS 2097152 /* 128^3 */
+#define SMAISO_NOALLOC_MAX_LEN 1024
+int smaiso_r2r3d_noalloc(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
+                         const double* in, double* out, int32_t dtype);
+
 /* Builder-defined initial condition (the reference's own IC is S:571-590):
  *   profile = 1/2 (1 - tanh((z - h(x,y))/width)),  z = k dz, x = i dx, y = j dy
  *   h(x,y)  = h0 + undulation cos(2 pi x/Lx) cos(2 pi y/Ly)  (or height[i*ny+j])

@@ -6,6 +6,7 @@
 // the member as blockIdx.y.  One call of run_step<T>() is one iteration of the
 // reference's loop body (smaiso-wcomp.cpp:1081-2198) for all members.
 #include "../../include/smaiso.h"
+#include "abi.h"
 #include "common.h"
 #include "adjoint.h"
 #include "fused.h"
@@ -366,6 +367,12 @@ int fail(smaiso_handle* h, int code, const std::string& msg) {
   else g_create_error = msg;
   return code;
 }
+
+}  // namespace
+
+int smaiso::abi_fail(int code, const std::string& msg) { return fail(nullptr, code, msg); }
+
+namespace {
 
 #define HIPTRY(h, expr)                                                                    \
   do {                                                                                     \

@@ -151,4 +151,16 @@ class R2R {
   std::mutex mu_;
 };
 
+// Host values of the transform tables of length n, as R2R uploads them:
+// tw[2m], tw[2m+1] = cos, sin(-2 pi m/n) and mk likewise of -pi m/(2n) for
+// m < n (2n doubles each); ct[t] = cos(pi t/(2n)) for t < 4n.
+void r2r_table_values(int n, double* tw, double* mk, double* ct);
+
+// One plain axis pass (io.f.op must be OP_PLAIN) with the default tiles and
+// device tables the caller provides (the values of r2r_table_values for the
+// pass axis' length): the same kernels as R2R::pass without R2R's table
+// cache, so it allocates nothing.
+hipError_t r2r_plain_pass_with_tables(DType dt, int kind, int axis, const PassIO& io, const Geom& g,
+                                      const void* tw, const void* mk, const void* ct, hipStream_t s);
+
 }  // namespace smaiso

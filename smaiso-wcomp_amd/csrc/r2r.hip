@@ -11,6 +11,18 @@ namespace smaiso {
 // ---------------------------------------------------------------------------
 bool R2R::fft_supported(int n) { return fft_radix(n, 16, 0) != 0; }
 
+void r2r_table_values(int n, double* tw, double* mk, double* ct) {
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    tw[2 * m] = std::cos(a);
+    tw[2 * m + 1] = std::sin(a);
+    const double b = -M_PI * (double)m / (2.0 * n);
+    mk[2 * m] = std::cos(b);
+    mk[2 * m + 1] = std::sin(b);
+  }
+  for (int t = 0; t < 4 * n; ++t) ct[t] = std::cos(M_PI * (double)t / (2.0 * n));
+}
+
 R2R::~R2R() {
   for (auto& kv : cache_) {
     if (kv.second.tw) (void)hipFree(kv.second.tw);
@@ -26,15 +38,7 @@ hipError_t R2R::tables(DType dt, int n, Tables** out) {
   if (it != cache_.end()) { *out = &it->second; return hipSuccess; }
   const size_t es = dt == F64 ? 8 : 4;
   std::vector<double> tw(2 * n), mk(2 * n), ct(4 * n);
-  for (int m = 0; m < n; ++m) {
-    const double a = -2.0 * M_PI * (double)m / (double)n;
-    tw[2 * m] = std::cos(a);
-    tw[2 * m + 1] = std::sin(a);
-    const double b = -M_PI * (double)m / (2.0 * n);
-    mk[2 * m] = std::cos(b);
-    mk[2 * m + 1] = std::sin(b);
-  }
-  for (int t = 0; t < 4 * n; ++t) ct[t] = std::cos(M_PI * (double)t / (2.0 * n));
+  r2r_table_values(n, tw.data(), mk.data(), ct.data());
   Tables tb;
   hipError_t e;
   auto upload = [&](const std::vector<double>& v, void** d) -> hipError_t {
@@ -146,6 +150,17 @@ hipError_t R2R::pass(DType dt, int kind, int axis, const PassIO& io, const Geom&
   }
   if (dt == F64) return dispatch<double>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
   return dispatch<float>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
+}
+
+hipError_t r2r_plain_pass_with_tables(DType dt, int kind, int axis, const PassIO& io, const Geom& g,
+                                      const void* tw, const void* mk, const void* ct, hipStream_t s) {
+  if (io.f.op != OP_PLAIN) return hipErrorInvalidValue;
+  const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  void* t = const_cast<void*>(tw);
+  void* m = const_cast<void*>(mk);
+  void* c = const_cast<void*>(ct);
+  if (dt == F64) return dispatch<double>(n, io, t, m, c, kind, axis, g, s);
+  return dispatch<float>(n, io, t, m, c, kind, axis, g, s);
 }
 
 void R2R::set_variant(DType dt, int axis, const Geom& g, int v) {
