@@ -150,3 +150,33 @@ def test_adjoint_fp32_mixed_close_to_fp64():
             "psi0": rel_err(gm.psi0, g64.psi0)[1], "rho0": rel_err(gm.rho0, g64.rho0)[1]}
     print(errs)
     assert all(v < 1e-4 for v in errs.values()), errs
+
+
+def test_adjoint_fp32_mixed_128_500_steps_vs_fp64_finite_differences():
+    """Configuration 5's FP32-mixed leg: 128^3, 500 steps, a checkpoint every
+    50; the mixed adjoint's dL/dzeta, dL/dnu and dL/dh (one smooth direction)
+    against central differences (relative step 1e-6) of the FP64 forward
+    run, i.e. the FP64 tangent, to 1e-5 relative.  Differences of FP32-mixed
+    runs themselves would be dominated by their rounding at this step."""
+    sp = S.Params(nx=128, ny=128, nz=128, dt=0.01, rho0=0.8, rho_m=1.0, divv_switch=0, nu=0.6, zeta=1.4)
+    spm = dataclasses.replace(sp, precision=S.FP32_MIXED)
+    ic, h, psi0, rho0, target = _problem(sp)
+    g = S.Stepper(spm).adjoint(psi0, rho0, target, 500, 50)
+    dh = S.interface_ic_vjp(sp, ic, g.psi0, g.rho0, height=h)
+    out = {}
+    for name, got in (("zeta", g.zeta), ("nu", g.nu)):
+        v = getattr(sp, name)
+        e = 1e-6 * v
+        lp = _fd_loss(dataclasses.replace(sp, **{name: v + e}), ic, h, target, 500)
+        lm = _fd_loss(dataclasses.replace(sp, **{name: v - e}), ic, h, target, 500)
+        out[name] = ((lp - lm) / (2 * e), got)
+    x = np.arange(sp.nx) * sp.dx
+    y = np.arange(sp.ny) * sp.dy
+    dirn = np.sin(2 * np.pi * x / (sp.nx * sp.dx))[:, None] * np.cos(4 * np.pi * y / (sp.ny * sp.dy))[None, :] + 0.3
+    e = 1e-6
+    lp = _fd_loss(sp, ic, h + e * dirn, target, 500)
+    lm = _fd_loss(sp, ic, h - e * dirn, target, 500)
+    out["h"] = ((lp - lm) / (2 * e), float(np.sum(dh * dirn)))
+    rel = {k: abs(fd - got) / abs(fd) for k, (fd, got) in out.items()}
+    print({k: (fd, got, rel[k]) for k, (fd, got) in out.items()})
+    assert all(v <= 1e-5 for v in rel.values()), rel

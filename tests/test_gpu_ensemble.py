@@ -283,3 +283,34 @@ def test_ensemble_autotune():
             for k in ("psi", "rho", "vz"):
                 mx, l2 = rel_err(en.field(m, k), st.field(k))
                 assert mx < tol and l2 < tol, (m, k, mx, l2)
+
+
+def test_ensemble_48_members_configuration_4():
+    """Configuration 4 at its real size: 48 members (16 each of 64^3, 96^3 and
+    128x128x64), FP32-mixed, seeded draws of rho0, nu, zeta and undulation
+    (the bench's workload, dt = 0.0025), graph replay, 10 steps: one member
+    of each grid equals its single handle bit for bit and all 48 stay
+    finite."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    members, ics = bench.ensemble_members(S, 48)
+    assert len(members) == 48
+    assert sorted({(p.nx, p.ny, p.nz) for p in members}) == [(64, 64, 64), (96, 96, 96), (128, 128, 64)]
+    states = [S.make_interface_ic(p, ic) for p, ic in zip(members, ics)]
+    with S.Ensemble(members) as en:
+        assert en.groups == 3
+        for m, (psi, rho) in enumerate(states):
+            en.set_state(m, psi, rho)
+        en.step(10)
+        finite = sum(all(np.isfinite(v) for v in en.diagnostics(m, allow_nonfinite=True)) for m in range(48))
+        assert finite == 48
+        for m in (0, 16, 47):
+            st = S.Stepper(members[m])
+            st.set_state(*states[m])
+            st.step(10)
+            for k in ("psi", "rho", "vx", "vz", "amp"):
+                assert np.array_equal(en.field(m, k), st.field(k)), (m, k)
+            assert en.diagnostics(m) == st.diagnostics()
+            st.close()

@@ -183,6 +183,39 @@ def test_mixed_precision_flat_interface_256():
     b.close()
 
 
+
+def test_mixed_precision_flat_interface_512x512x256():
+    """The configuration-3 check at its large grid: FP32-mixed against the
+    FP64 stepper from the flat-interface state on 512x512x256, psi within
+    1e-6 relative L2 at steps 100 and 500 (and rho, v reported)."""
+    shape = (512, 512, 256)
+    ic = S.InterfaceIC(amplitude=1.0, width=2.0, rho_iso=0.9, rho_sm=1.0, undulation=0.0,
+                       noise=1e-3, seed=1)
+
+    def params(prec):
+        return S.Params(nx=shape[0], ny=shape[1], nz=shape[2], dt=0.01, rho0=0.9, rho_m=1.0,
+                        divv_switch=0, precision=prec)
+
+    psi, rho = S.make_interface_ic(params(S.FP64), ic)
+    a, b = S.Stepper(params(S.FP64)), S.Stepper(params(S.FP32_MIXED))
+    a.set_state(psi, rho)
+    b.set_state(psi, rho)
+    del psi, rho
+    done = 0
+    for c in (100, 500):
+        a.step(c - done)
+        b.step(c - done)
+        done = c
+        errs = {}
+        for name in ("psi", "rho", "vx"):
+            fa, fb = a.field(name), b.field(name)
+            errs[name] = float(np.linalg.norm(fb - fa) / np.linalg.norm(fa))
+        print(f"step {c}: {errs}")
+        assert errs["psi"] < 1e-6, (c, errs)
+        assert errs["rho"] < 1e-7, (c, errs)
+    a.close()
+    b.close()
+
 def test_mixed_odd_lengths_direct_cast_path():
     """FP32-mixed on a grid whose x (20) and z (40) lengths run on the direct
     path, so the FP32 -> FP64 z pass and the FP64 -> FP32 x pass of psi's
@@ -258,3 +291,37 @@ def test_config2_full_size_two_steps():
     rl1, rmass, ren = o.diagnostics()
     assert abs(mass - rmass) <= 1e-12 * abs(rmass)
     assert abs(en - ren) <= 1e-10 * abs(ren)
+
+
+def test_config2_trace_against_oracle_fixture():
+    """Config 2's diagnostics trace (256^3, perturbed interface, 1.0/0.8,
+    dt = 0.01): L1, mass and free energy every 50 steps, the reference's
+    logging cadence, against the FP64 oracle's trace committed in
+    tests/golden/config2_trace.json (made by make_config2_trace.py)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "config2_trace.json")
+    with open(path) as f:
+        doc = json.load(f)
+    cfg = doc["config"]
+    trace = doc["trace"]
+    assert trace and trace[-1]["step"] >= 50
+    n = cfg["n"]
+    p = S.Params(nx=n, ny=n, nz=n, dt=cfg["dt"], rho0=cfg["rho0"], rho_m=cfg["rho_m"],
+                 divv_switch=cfg["divv_switch"])
+    psi, rho = S.make_interface_ic(p, S.InterfaceIC(**cfg["ic"]))
+    st = S.Stepper(p)
+    st.set_state(psi, rho)
+    done = 0
+    worst = dict(l1=0.0, mass=0.0, energy=0.0)
+    for row in trace:
+        st.step(row["step"] - done)
+        done = row["step"]
+        l1, mass, en = st.diagnostics()
+        errs = dict(l1=abs(l1 - row["l1"]) / abs(row["l1"]), mass=abs(mass - row["mass"]) / abs(row["mass"]),
+                    energy=abs(en - row["energy"]) / abs(row["energy"]))
+        worst = {k: max(worst[k], errs[k]) for k in worst}
+        assert errs["mass"] <= 1e-12, (done, errs)
+        assert errs["energy"] <= 1e-10 and errs["l1"] <= 1e-10, (done, errs)
+    print(f"{done} steps against the oracle trace, worst relative errors {worst}")
+    st.close()
