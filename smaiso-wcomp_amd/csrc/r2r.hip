@@ -108,6 +108,11 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
 template <typename T>
 static hipError_t dispatch(int n, const PassIO& io, void* tw, void* mk, void* ct, int kind,
                            int axis, const Geom& g, hipStream_t s) {
+  if (n == 256 && axis != 2) {
+    const DType dt = sizeof(T) == 8 ? F64 : F32;
+    const hipError_t e = launch_yx256(dt, dt, kind, axis, io, g, tw, mk, s);
+    if (e != hipErrorNotSupported) return e;
+  }
   if (is_axis_op(io.f.op) && R2R::fft_supported(n)) return launch_axis_op<T>(n, io, tw, mk, axis, g, s);
   switch (n) {
     case 16: return launch_fft<T, 16>(io, tw, mk, kind, axis, g, s);

@@ -44,6 +44,11 @@ hipError_t launch_cast_direct(const PassIO& io, void* costab, int kind, int axis
 template <typename T, typename TO>
 hipError_t dispatch_cast(int n, const PassIO& io, void* tw, void* mk, void* ct, int kind, int axis,
                          const Geom& g, hipStream_t s) {
+  if (n == 256 && axis != 2) {
+    const hipError_t e = launch_yx256(sizeof(T) == 8 ? F64 : F32, sizeof(TO) == 8 ? F64 : F32, kind, axis, io, g,
+                                      tw, mk, s);
+    if (e != hipErrorNotSupported) return e;
+  }
   switch (n) {
     case 16: return launch_cast_n<T, TO, 16>(io, tw, mk, kind, axis, g, s);
     case 32: return launch_cast_n<T, TO, 32>(io, tw, mk, kind, axis, g, s);

@@ -1,0 +1,77 @@
+// Launchers of the 256-point y/x passes (r2r_yx256.h); a translation unit of
+// its own so that these instantiations compile in parallel with the rest.
+#include "r2r_yx256.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <type_traits>
+
+namespace smaiso {
+
+namespace {
+
+template <typename T, typename TO, int AXIS, int MODE, int OP>
+hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, const void* mk, hipStream_t s) {
+  constexpr int P = yx256::tile_p<T>();
+  yx256::Args<T, TO> a;
+  a.in = static_cast<const T*>(io.in);
+  a.out = static_cast<TO*>(io.out);
+  a.tw = static_cast<const cplx<double>*>(tw);
+  a.mk = static_cast<const cplx<double>*>(mk);
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.nz = g.nz;
+  a.kind = kind;
+  a.f = io.f;
+  a.tpo = g.nz / (2 * P);
+  a.ntiles = (AXIS == 1 ? g.nx : g.ny) * a.tpo;
+  // two resident workgroups per CU for FP64 tiles (128 threads), one for
+  // FP32 tiles (256 threads); twice that many keep every CU's queue full
+  const int grid = std::min(a.ntiles, 256 * (P == 8 ? 2 : 1) * 2);
+  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<double>);
+  hipLaunchKernelGGL((yx256::pass_kernel<T, TO, AXIS, MODE, OP, P>), dim3((unsigned)grid, (unsigned)g.nb),
+                     dim3(16 * P), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename TO, int AXIS>
+hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
+  switch (io.f.op) {
+    case OP_PLAIN:
+      if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, OP_PLAIN>(io, g, kind, tw, mk, s);
+      return launch<T, TO, AXIS, yx256::kInv, OP_PLAIN>(io, g, kind, tw, mk, s);
+    default:
+      break;
+  }
+  // The forward -> operator -> inverse passes (kFwdOp) measured slower than
+  // the general kernel's at 256^3 (their 32 complex values per thread spill
+  // even with the whole register file): those ops stay on the general kernel.
+  return hipErrorNotSupported;
+}
+
+template <typename T, typename TO>
+hipError_t launch_typed(int kind, int axis, const PassIO& io, const Geom& g, const void* tw, const void* mk,
+                        hipStream_t s) {
+  constexpr int P = yx256::tile_p<T>();
+  const int n = axis == 1 ? g.ny : g.nx;
+  const long long stride = axis == 1 ? (long long)g.nz : (long long)g.ny * g.nz;
+  const bool aligned = ((uintptr_t)io.in % (2 * sizeof(T))) == 0 && ((uintptr_t)io.out % (2 * sizeof(TO))) == 0 &&
+                       (g.n3() % 2) == 0;
+  if (n != yx256::kN || g.nz % (2 * P) != 0 || !aligned || io.in2 != nullptr ||
+      (yx256::kN - 1) * stride + g.nz > 2147483647LL)
+    return hipErrorNotSupported;
+  return axis == 1 ? launch_axis<T, TO, 1>(kind, io, g, tw, mk, s) : launch_axis<T, TO, 0>(kind, io, g, tw, mk, s);
+}
+
+}  // namespace
+
+hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io, const Geom& g,
+                        const void* tw, const void* mk, hipStream_t s) {
+  if (axis != 0 && axis != 1) return hipErrorNotSupported;
+  if (in_dt == F64 && out_dt == F64) return launch_typed<double, double>(kind, axis, io, g, tw, mk, s);
+  if (in_dt == F32 && out_dt == F32) return launch_typed<float, float>(kind, axis, io, g, tw, mk, s);
+  if (in_dt == F64 && out_dt == F32) return launch_typed<double, float>(kind, axis, io, g, tw, mk, s);
+  return hipErrorNotSupported;
+}
+
+}  // namespace smaiso
