@@ -31,7 +31,7 @@ struct AdjArgs {
   const double* rd[3];  // grad rho (FP64 in both precisions)
   const T* v[3];      // total velocity (real space)
   const T* divv;      // div v (real space)
-  const T* vsolq[3];  // solenoidal velocity on the staggered lattices (spectral)
+  const T* vtq[3];    // total velocity vsol^ + virr^ on the staggered lattices (spectral)
   const T* virrq[3];  // irrotational velocity (spectral)
   const double* rho;      // rho at the step start
   const double* rho_new;  // rho after the density update

@@ -164,7 +164,8 @@ __global__ __launch_bounds__(kB) void k_adj_vel_a(StepConsts c, AdjArgs<T> A) {
       // output masks, quirks Q1 and Q2
       const bool zero = a == 0 ? (p.i == c.nx - 1) : (p.j == c.ny - 1 || p.k == c.nz - 1);
       if (zero) vs = vi = 0.0;
-      dn += vs * (-(double)A.vsolq[a][idx] / nu) + vi * (-2.0 * (double)A.virrq[a][idx] / ilam);
+      const double virr = (double)A.virrq[a][idx], vsol = (double)A.vtq[a][idx] - virr;
+      dn += vs * (-vsol / nu) + vi * (-2.0 * virr / ilam);
       double mq2 = 0;
 #pragma unroll
       for (int b = 0; b < 3; ++b) mq2 += sqd(b == a ? wv[b] : qv[b]);

@@ -103,7 +103,9 @@ struct smaiso_handle {
   long long diag_epoch = -1;  // epoch of the cached diagnostics (-1: none)
   // work fields (T = double or float)
   void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
-  void *gp[3] = {}, *S[3] = {}, *f[3] = {}, *vsolq[3] = {}, *virrq[3] = {}, *v[3] = {};
+  // vtq: the total velocity vsol^ + virr^ of the last step (spectral, on the
+  // staggered lattices); virrq: its irrotational part
+  void *gp[3] = {}, *S[3] = {}, *f[3] = {}, *vtq[3] = {}, *virrq[3] = {}, *v[3] = {};
   void *amp = nullptr, *lpsi = nullptr, *laprl = nullptr, *mu = nullptr, *fgrad = nullptr,
        *erho = nullptr, *divv = nullptr, *divf = nullptr, *nl = nullptr, *tmp = nullptr;
 
@@ -528,7 +530,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
                                &h->divf, &h->nl, &h->tmp};
   for (int a = 0; a < 3; ++a) {
     tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->S[a]);
-    tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vsolq[a]); tbufs.push_back(&h->virrq[a]);
+    tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vtq[a]); tbufs.push_back(&h->virrq[a]);
     tbufs.push_back(&h->v[a]);
   }
   for (void** b : tbufs)
@@ -553,7 +555,7 @@ int run_step(smaiso_handle* h) {
   double* const* rd = h->rd;
   T* const* S = reinterpret_cast<T* const*>(h->S);
   T* const* f = reinterpret_cast<T* const*>(h->f);
-  T* const* vsolq = reinterpret_cast<T* const*>(h->vsolq);
+  T* const* vtq = reinterpret_cast<T* const*>(h->vtq);
   T* const* virrq = reinterpret_cast<T* const*>(h->virrq);
   T* const* v = reinterpret_cast<T* const*>(h->v);
   const double fb1 = h->fb(1);
@@ -649,7 +651,7 @@ int run_step(smaiso_handle* h) {
   {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 11);
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
-                vsolq[0], vsolq[1], vsolq[2], virrq[0], virrq[1], virrq[2], s);
+                vtq[0], vtq[1], vtq[2], virrq[0], virrq[1], virrq[2], s);
   }
   // a13 + a14 (S:1979-2117): div v is formed on load by the z pass of its
   // inverse transform; div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum
@@ -668,7 +670,7 @@ int run_step(smaiso_handle* h) {
       HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, f[a], h->divf, d));
     }
   }
-  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vsolq[a], v[a], virrq[a]));
+  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vtq[a], v[a]));
   // a15 + a16 (S:2121-2162)
   {
     smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
@@ -766,8 +768,15 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_VX: case SMAISO_FIELD_VY: case SMAISO_FIELD_VZ:
       *out = as64(h->v[fid - SMAISO_FIELD_VX]); return SMAISO_OK;
     case SMAISO_FIELD_VSOLX: case SMAISO_FIELD_VSOLY: case SMAISO_FIELD_VSOLZ: {
+      // vsol = v - iST(virr^): the step keeps the total and the irrotational part
       const int a = fid - SMAISO_FIELD_VSOLX;
-      HIPTRY(h, h->xw(kIST[a], h->vsolq[a], h->tmp));
+      HIPTRY(h, h->xw(kIST[a], h->virrq[a], h->tmp));
+      if (h->mixed)
+        sub_fields(static_cast<const float*>(h->v[a]), static_cast<const float*>(h->tmp),
+                   static_cast<float*>(h->tmp), nt, s);
+      else
+        sub_fields(static_cast<const double*>(h->v[a]), static_cast<const double*>(h->tmp),
+                   static_cast<double*>(h->tmp), nt, s);
       *out = as64(h->tmp); return SMAISO_OK;
     }
     case SMAISO_FIELD_VIRRX: case SMAISO_FIELD_VIRRY: case SMAISO_FIELD_VIRRZ: {
@@ -1486,7 +1495,7 @@ AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
     A.gp[a] = tp(h->gp[a]);
     A.rd[a] = h->rd[a];
     A.v[a] = tp(h->v[a]);
-    A.vsolq[a] = tp(h->vsolq[a]);
+    A.vtq[a] = tp(h->vtq[a]);
     A.virrq[a] = tp(h->virrq[a]);
   }
   A.rho = rho_n;

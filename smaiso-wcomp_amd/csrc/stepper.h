@@ -52,9 +52,11 @@ struct StepKernels {
                      const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
                      const double* rdx, const double* rdy, const double* rdz, T* fgrad, T* erho, T* mu, T* sx,
                      T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s);
-  // a11 (fgrad += G erho) + a12
+  // a11 (fgrad += G erho) + a12: the total velocity vsol^ + virr^ (the
+  // input of v's inverse transforms, summed in FP64 before the store) and
+  // the irrotational part virr^ (the input of div v)
   static void velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
-                       const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
+                       const T* fgq, const T* erq, T* vx, T* vy, T* vz, T* virrx,
                        T* virry, T* virrz, hipStream_t s);
   // a15 + a16 (FP64 arithmetic)
   static void density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
@@ -77,6 +79,9 @@ void diag_final(const double* partials, int nb, int nmem, double* out, hipStream
 void convert_d2f(const double* in, float* out, long long n, hipStream_t s);
 void convert_f2d(const float* in, double* out, long long n, hipStream_t s);
 void fill_zero_d(double* out, long long n, hipStream_t s);
+// out = a - b over n elements of type T (float or double)
+template <typename T>
+void sub_fields(const T* a, const T* b, T* out, long long n, hipStream_t s);
 // Surface tracker of S:2278-2318 for one nx x ny x nz field rho: for each
 // column (i, j), the largest k with rho > threshold, or -1 when there is none
 // (the reference keeps the column's previous value then; the caller merges).

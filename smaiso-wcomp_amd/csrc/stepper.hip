@@ -117,7 +117,7 @@ template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_velocity(
     StepConsts c, const T* __restrict__ sxq_, const T* __restrict__ syq_,
     const T* __restrict__ szq_, const T* __restrict__ fgq_, const T* __restrict__ erq_,
-    T* __restrict__ vsolx, T* __restrict__ vsoly, T* __restrict__ vsolz,
+    T* __restrict__ vtx, T* __restrict__ vty, T* __restrict__ vtz,
     T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
   // XCD-aware order: workgroups are dealt to the 8 XCDs round robin.  When
@@ -179,11 +179,13 @@ __global__ __launch_bounds__(BS) void k_velocity(
     const double gx = g3(p.i, p.j, p.k);
     const double fg = (double)fgq[idx] + (double)erq[idx] * gx;
     const long long ox = mo + idx - sI;
-    vsolx[ox] = (T)((cm1 * rnu) * (Sx - cm2 * dot));
-    virrx[ox] = (T)((cm1 * rlam) * (wx * fg + cm2 * dot * gx));
+    const double sol = (cm1 * rnu) * (Sx - cm2 * dot);
+    const double irr = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
+    vtx[ox] = (T)(sol + irr);
+    virrx[ox] = (T)irr;
   } else {
     const long long ox = mo + idx + (long long)(c.nx - 1) * sI;
-    vsolx[ox] = (T)0;
+    vtx[ox] = (T)0;
     virrx[ox] = (T)0;
   }
   double oy = 0, ry = 0, oz = 0, rz = 0;
@@ -218,8 +220,8 @@ __global__ __launch_bounds__(BS) void k_velocity(
     }
   }
   const long long o = mo + idx;
-  vsoly[o] = (T)oy; virry[o] = (T)ry;
-  vsolz[o] = (T)oz; virrz[o] = (T)rz;
+  vty[o] = (T)(oy + ry); virry[o] = (T)ry;
+  vtz[o] = (T)(oz + rz); virrz[o] = (T)rz;
 }
 
 // ---------------------------------------------------------------------------
@@ -395,7 +397,7 @@ void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, 
 }
 template <typename T>
 void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
-                              const T* fgq, const T* erq, T* vsolx, T* vsoly, T* vsolz, T* virrx,
+                              const T* fgq, const T* erq, T* vx, T* vy, T* vz, T* virrx,
                               T* virry, T* virrz, hipStream_t s) {
   // 1024-point workgroups (four lines of 256) when they tile the field: the
   // j+-1 lines a point reads are then mostly read by its own workgroup, on
@@ -404,10 +406,10 @@ void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, c
   const long long n3 = n3_of(c);
   if (n3 % BS == 0) {
     hipLaunchKernelGGL((k_velocity<T, BS>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s, c, sxq,
-                       syq, szq, fgq, erq, vsolx, vsoly, vsolz, virrx, virry, virrz);
+                       syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz);
   } else {
     hipLaunchKernelGGL((k_velocity<T, kBlock>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
-                       vsolx, vsoly, vsolz, virrx, virry, virrz);
+                       vx, vy, vz, virrx, virry, virrz);
   }
 }
 template <typename T>
@@ -447,6 +449,19 @@ void convert_f2d(const float* in, double* out, long long n, hipStream_t s) {
 void fill_zero_d(double* out, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, out, n);
 }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out,
+                                                long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    out[i] = a[i] - b[i];
+}
+template <typename T>
+void sub_fields(const T* a, const T* b, T* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub<T>, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, out, n);
+}
+template void sub_fields<float>(const float*, const float*, float*, long long, hipStream_t);
+template void sub_fields<double>(const double*, const double*, double*, long long, hipStream_t);
 
 // One wavefront per column: chunks of 64 k values from the top, the first
 // chunk with a qualifying value gives the highest such k (its lowest lane).
