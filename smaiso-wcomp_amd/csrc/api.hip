@@ -671,16 +671,34 @@ int run_step(smaiso_handle* h) {
     }
   }
   for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vtq[a], v[a]));
-  // a15 + a16 (S:2121-2162)
-  {
-    smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
-    K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
-                  h->f_rho, static_cast<T*>(h->mu), static_cast<T*>(h->amp),
-                  static_cast<T*>(h->psi), static_cast<T*>(h->divf), gp[0], gp[1], gp[2],
-                  static_cast<T*>(h->nl), s);
+  // a15 + a16 (S:2121-2162) and the transform of Nl (S:2166-2168): the z
+  // pass of the transform computes them on load where the length allows
+  if (density_zpass_supported(h->g.nz)) {
+    DensityIO<T> io;
+    for (int a = 0; a < 3; ++a) { io.v[a] = v[a]; io.rd[a] = rd[a]; io.gp[a] = gp[a]; }
+    io.divv = static_cast<const T*>(h->divv);
+    io.rho = h->rho;
+    io.f_rho = h->f_rho;
+    io.mu = static_cast<const T*>(h->mu);
+    io.amp = static_cast<const T*>(h->amp);
+    io.psi = static_cast<const T*>(h->psi);
+    io.divf = static_cast<const T*>(h->divf);
+    io.nlz = static_cast<T*>(h->nlq);
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
+      HIPTRY(h, density_zpass<T>(c, io, h->r2r, s));
+    }
+    HIPTRY(h, h->xw_yx(kCT, h->nlq));
+  } else {
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
+      K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
+                    h->f_rho, static_cast<T*>(h->mu), static_cast<T*>(h->amp),
+                    static_cast<T*>(h->psi), static_cast<T*>(h->divf), gp[0], gp[1], gp[2],
+                    static_cast<T*>(h->nl), s);
+    }
+    HIPTRY(h, h->xw(kCT, h->nl, h->nlq));
   }
-  // a16 tail (S:2166-2172)
-  HIPTRY(h, h->xw(kCT, h->nl, h->nlq));
   // a17 (S:2177-2198): the z pass of psi = iDCT(psiq_new) computes the
   // update on load and stores psiq_new over psiq_old
   std::swap(h->psi, h->psi_prev);  // psi_old_local = psi_local, S:2192

@@ -31,4 +31,24 @@ template <typename T>
 hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hipStream_t s);
 bool forces_zpass_supported(DType dt, int nz);
 
+// a15 + a16 (S:2121-2162) fused with the z pass of the nonlinear term's
+// transform (S:2166-2168, REDFT10 along z): the density update (rho and
+// f_rho in place, FP64) and Nl evaluated on load, Nl's z-transformed lines
+// stored to nlz.
+template <typename T>
+struct DensityIO {
+  const T* v[3];        // total velocity (real space)
+  const double* rd[3];  // grad rho at the step start
+  const T* divv;
+  double* rho;          // in: rho at the step start; out: rho_new
+  double* f_rho;        // in: the last step's source; out: this step's
+  const T *mu, *amp, *psi, *divf;
+  const T* gp[3];       // grad psi
+  T* nlz;               // z pass of Nl (REDFT10)
+};
+
+template <typename T>
+hipError_t density_zpass(const StepConsts& c, const DensityIO<T>& io, R2R& r2r, hipStream_t s);
+bool density_zpass_supported(int nz);
+
 }  // namespace smaiso

@@ -179,6 +179,125 @@ hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hi
   }
 }
 
+namespace {
+
+// One workgroup: one z line pair (P = 1) of 128 threads.  Load phase: every
+// point evaluates a15 and a16 in FP64 registers as k_density_nl does (same
+// expressions, same order), stores rho_new and the source f_rho, and puts Nl
+// at its Makhoul position of the pair's complex sequence in LDS; then the
+// FFT and Makhoul's post-processing store Nl's z pass (DCT-II).
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T> io,
+                                                       const cplx<double>* __restrict__ tw,
+                                                       const cplx<double>* __restrict__ mk) {
+  constexpr int P = 1, TL = 2, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, P, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using A = double;
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
+  A* rbuf = reinterpret_cast<A*>(smem_raw);
+  const int tid = threadIdx.x;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
+  const long long base = (long long)blockIdx.y * nl * N + first * N;
+  const MemberConsts mc = c.mc[blockIdx.y];
+  StageTw<A, N, P, NT, MAXR> stw;
+  stw.load(tw, tid);
+
+  constexpr int E = TL * N;
+  constexpr int IT = (E + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (E % NT == 0 || e < E) {
+      const int l = e / N, m = e - l * N;
+      A v = A(0);
+      if (l < nvalid) {
+        const long long idx = base + e;
+        const A ux = io.v[0][idx], uy = io.v[1][idx], uz = io.v[2][idx];
+        const A r0 = io.rho[idx];
+        const A src = -ux * io.rd[0][idx] - uy * io.rd[1][idx] - uz * io.rd[2][idx] - r0 * (A)io.divv[idx];
+        const A r1 = r0 + c.dt * (1.5 * src - 0.5 * io.f_rho[idx]);
+        io.rho[idx] = r1;
+        io.f_rho[idx] = src;
+        const A am = io.amp[idx];
+        v = -(A)io.mu[idx] + ((mc.kp * mc.zeta) / (am + 0.001)) * (r1 - mc.rho0 - mc.kp * am) * (A)io.psi[idx] -
+            (A)io.divf[idx] - ux * (A)io.gp[0][idx] - uy * (A)io.gp[1][idx] - uz * (A)io.gp[2][idx];
+      }
+      const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+      rbuf[2 * lp<SH>(pos) + (l & 1)] = v;
+    }
+  }
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
+
+  constexpr int W = HN;
+  constexpr int ITW = (W + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int k = tid + it * NT;
+    if (W % NT == 0 || k < W) {
+      const int kb = (k == 0) ? HN : N - k;
+      A y0a, y1a, y0b, y1b;
+      makhoul_post<A, N>(mk, k, kb, mk[k], buf[lp<SH>(k)], buf[lp<SH>(kb)], y0a, y1a, y0b, y1b);
+      if (0 < nvalid) {
+        T* q = io.nlz + base;
+        q[k] = (T)y0a;
+        q[kb] = (T)y0b;
+      }
+      if (1 < nvalid) {
+        T* q = io.nlz + base + N;
+        q[k] = (T)y1a;
+        q[kb] = (T)y1b;
+      }
+    }
+  }
+}
+
+constexpr int kDensityNT = 128;
+
+template <typename T, int N>
+hipError_t launch_density(const StepConsts& c, const DensityIO<T>& io, const void* tw, const void* mk,
+                          hipStream_t s) {
+  const long long blocks = ((long long)c.nx * c.ny + 1) / 2;
+  const size_t lds = (size_t)seq_len(N, pick_maxr(N, 1, kDensityNT)) * sizeof(cplx<double>);
+  hipLaunchKernelGGL((density_z_kernel<T, N, kDensityNT>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                     dim3(kDensityNT), lds, s, c, io, static_cast<const cplx<double>*>(tw),
+                     static_cast<const cplx<double>*>(mk));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool density_zpass_supported(int nz) { return R2R::fft_supported(nz); }
+
+template <typename T>
+hipError_t density_zpass(const StepConsts& c, const DensityIO<T>& io, R2R& r2r, hipStream_t s) {
+  if (!density_zpass_supported(c.nz)) return hipErrorNotSupported;
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
+  if (e != hipSuccess) return e;
+  switch (c.nz) {
+    case 16: return launch_density<T, 16>(c, io, tw, mk, s);
+    case 32: return launch_density<T, 32>(c, io, tw, mk, s);
+    case 48: return launch_density<T, 48>(c, io, tw, mk, s);
+    case 64: return launch_density<T, 64>(c, io, tw, mk, s);
+    case 96: return launch_density<T, 96>(c, io, tw, mk, s);
+    case 128: return launch_density<T, 128>(c, io, tw, mk, s);
+    case 192: return launch_density<T, 192>(c, io, tw, mk, s);
+    case 256: return launch_density<T, 256>(c, io, tw, mk, s);
+    case 384: return launch_density<T, 384>(c, io, tw, mk, s);
+    case 512: return launch_density<T, 512>(c, io, tw, mk, s);
+    case 1024: return launch_density<T, 1024>(c, io, tw, mk, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+template hipError_t density_zpass<double>(const StepConsts&, const DensityIO<double>&, R2R&, hipStream_t);
+template hipError_t density_zpass<float>(const StepConsts&, const DensityIO<float>&, R2R&, hipStream_t);
+
 template hipError_t forces_zpass<double>(const StepConsts&, const ForcesIO<double>&, R2R&, hipStream_t);
 template hipError_t forces_zpass<float>(const StepConsts&, const ForcesIO<float>&, R2R&, hipStream_t);
 
