@@ -39,8 +39,15 @@ extern "C" {
 #define SMAISO_E_NONFINITE (-5) /* diagnostics found NaN/Inf in the state */
 
 /* precision of the stepper */
-#define SMAISO_FP64 0        /* every field and transform in FP64 */
-#define SMAISO_FP32_MIXED 1  /* FP32 transforms/pointwise; FP64 rho, f_rho, reductions, denominators */
+#define SMAISO_FP64 0 /* every field, transform and kernel in FP64 */
+/* FP32 storage of the work fields (psi, grad psi, amp, L psi, mu, forces,
+ * velocities, Nl, ...); FP64 state (psiq, psiq_old), rho, f_rho and grad rho;
+ * FP64 arithmetic in every pointwise and per-mode kernel, the implicit
+ * denominators, the density update and the compensated reductions; the
+ * transform that forms psi from psiq keeps FP64 intermediates.  Transform
+ * passes compute in FP64 (FP64 LDS tiles), except those of the stages
+ * selected by smaiso_set_fp32_stages, which compute in FP32. */
+#define SMAISO_FP32_MIXED 1
 
 /* FFTW r2r kind numbering used by smaiso_r2r3d (S:499-545) */
 #define SMAISO_REDFT10 0  /* DCT-II  */
@@ -159,6 +166,29 @@ int smaiso_synchronize(smaiso_handle* h);
 /* Steps completed since the last set_state (the reference's nLoop). */
 int64_t smaiso_step_count(const smaiso_handle* h);
 
+/* Stages of the SMAISO_FP32_MIXED step whose transform passes compute in
+ * FP32 (FP32 LDS tiles and tables) rather than FP64.  The transform that
+ * forms psi from psiq, the forces and density kernels, the velocity solve,
+ * the psi update and the reductions always compute in FP64.  FP32
+ * arithmetic rounds every mode of a line relative to the line's norm, so
+ * it is kept to the stages where that rounding does not reach psi. */
+#define SMAISO_STAGE_GRAD_PSI (1u << 0) /* grad psi, one pass per axis (S:1104-1236) */
+#define SMAISO_STAGE_AMP (1u << 1)      /* Gaussian amplitude filter (S:1254-1289) */
+#define SMAISO_STAGE_LPSI (1u << 2)     /* L psi (S:1293-1308) */
+#define SMAISO_STAGE_LAPRHOL (1u << 3)  /* lap(rho L psi) (S:1312-1342) */
+#define SMAISO_STAGE_FORCES (1u << 4)   /* y and x passes of fgrad, erho, S_a (S:1427-1457) */
+#define SMAISO_STAGE_DIVV (1u << 5)     /* div v and its transform (S:2042-2088) */
+#define SMAISO_STAGE_DIVF (1u << 6)     /* div f, one pass per axis (S:2042-2092) */
+#define SMAISO_STAGE_VELOCITY (1u << 7) /* v's inverse transforms (S:2094-2117) */
+#define SMAISO_STAGE_NL (1u << 8)       /* y and x passes of Nl's transform (S:2166-2168) */
+#define SMAISO_STAGES_ALL 0x1ffu
+
+/* The FP32-arithmetic stages of a SMAISO_FP32_MIXED handle (SMAISO_STAGE_*
+ * bits; the default is smaiso_fp32_stages_default()).  Ignored by FP64
+ * handles.  No reference counterpart. */
+int smaiso_set_fp32_stages(smaiso_handle* h, uint32_t mask);
+uint32_t smaiso_fp32_stages_default(void);
+
 /* enable != 0: smaiso_step/smaiso_step_async replay pairs of steady steps
  * (past the relaxation branch, S:1312-1380) from one captured hipGraph of two
  * steps instead of launching them one by one (off by default; ignored while
@@ -189,13 +219,16 @@ int smaiso_diagnostics(smaiso_handle* h, double* l1, double* mass, double* energ
 /* Standalone 3-D r2r transform with FFTW semantics (unnormalised), the
  * equivalent of fftw_mpi_plan_r2r_3d + fftw_execute (S:499-545): kinds[a]
  * is the kind on axis a (0 = x, slowest).  in/out: host FP64 arrays of
- * nx*ny*nz; dtype SMAISO_FP64 computes in FP64, SMAISO_FP32_MIXED in FP32.
- * Uses the current HIP device and the null stream; synchronous. */
+ * nx*ny*nz; dtype SMAISO_FP64 stores and computes in FP64,
+ * SMAISO_FP32_MIXED rounds the input to FP32, stores every pass's output in
+ * FP32 and computes each pass in FP64.  Uses the current HIP device and the
+ * null stream; synchronous. */
 int smaiso_r2r3d(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
                  const double* in, double* out, int32_t dtype);
 
-/* As smaiso_r2r3d on device buffers of the given dtype (double or float),
- * enqueued on `stream` (a hipStream_t, NULL = null stream), asynchronous. */
+/* As smaiso_r2r3d on device buffers of the given dtype (double or float;
+ * FP64 arithmetic either way), enqueued on `stream` (a hipStream_t, NULL =
+ * null stream), asynchronous. */
 int smaiso_r2r3d_device(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
                         const void* in_dev, void* out_dev, int32_t dtype, void* stream);
 

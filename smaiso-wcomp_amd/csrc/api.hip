@@ -53,6 +53,14 @@ struct PendingTiming {
   double bytes;
 };
 
+// FP32-arithmetic stages of an FP32-mixed handle at creation: the stages
+// whose FP32 rounding leaves psi as close to the FP64 stepper as FP32
+// storage alone does (forces, div v, div f, v, Nl: psi within 3.4e-8 of
+// FP64 after 2000 steps at 256^3, as with none of them).  With FP32
+// arithmetic grad psi and lap(rho L psi) each reach 3e-5, L psi 1e-6.
+constexpr uint32_t kFp32StagesDefault = SMAISO_STAGE_FORCES | SMAISO_STAGE_DIVV | SMAISO_STAGE_DIVF |
+                                        SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL;
+
 double derived_amp(const smaiso_params& p) {
   return std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.epsilon * p.gamma)) / 5);
 }
@@ -78,6 +86,9 @@ struct smaiso_handle {
   bool nlq_valid = false;  // N^l of the last step: written by a step or a checkpoint load
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
+  uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
+  // FP32 arithmetic for the passes of stage `bit`
+  int a32(uint32_t bit) const { return mixed && (fp32_stages & bit) ? 1 : 0; }
 
   // derived (S:250-255, S:278), shared by the group
   double amp_eq = 0, rho_m = 0, scale = 0;
@@ -273,8 +284,10 @@ struct smaiso_handle {
     if ((e = run_pass(dt, kinds[1], 1, io, G)) != hipSuccess) return e;
     return run_pass(dt, kinds[0], 0, io, G);
   }
-  hipError_t xw(const int kinds[3], const void* in, void* out, const void* in2 = nullptr) {
-    return xform(mixed ? F32 : F64, kinds, in, in2, out);
+  hipError_t xw(const int kinds[3], const void* in, void* out, const void* in2 = nullptr, int arith32 = 0) {
+    Fuse f;
+    f.arith32 = arith32;
+    return xform(mixed ? F32 : F64, kinds, in, in2, out, &f);
   }
   hipError_t xwf(const int kinds[3], const Fuse& f, void* out) {
     return xform(mixed ? F32 : F64, kinds, nullptr, nullptr, out, &f);
@@ -297,11 +310,12 @@ struct smaiso_handle {
     return run_pass_cast(F64, F32, kinds[0], 0, io);
   }
   // the y and x passes of a transform whose z pass has run, in place
-  hipError_t xw_yx(const int kinds[3], void* buf) {
+  hipError_t xw_yx(const int kinds[3], void* buf, int arith32 = 0) {
     const DType dt = mixed ? F32 : F64;
     PassIO io;
     io.in = buf;
     io.out = buf;
+    io.f.arith32 = arith32;
     hipError_t e = run_pass(dt, kinds[1], 1, io, g);
     if (e != hipSuccess) return e;
     return run_pass(dt, kinds[0], 0, io, g);
@@ -451,6 +465,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->g = Geom{p.nx, p.ny, p.nz, n};
   h->mixed = p.precision == SMAISO_FP32_MIXED;
   h->tsz = h->mixed ? 4 : 8;
+  h->fp32_stages = kFp32StagesDefault;
   h->amp_eq = derived_amp(p);
   h->rho_m = rho_m_eff(p);
   h->scale = 0.125 / ((double)p.nx * p.ny * p.nz);
@@ -582,6 +597,7 @@ int run_step(smaiso_handle* h) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
+    g.arith32 = h->a32(SMAISO_STAGE_GRAD_PSI);
     HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, h->psi, gp[a], g));
   }
   // a7 (S:1254-1289): the Gaussian exp(-c |q|^2) is a product over the axes,
@@ -591,6 +607,7 @@ int run_step(smaiso_handle* h) {
     Fuse g = fb;
     g.op = OP_AX_GAUSS_AMP;
     g.axis = 2;
+    g.arith32 = h->a32(SMAISO_STAGE_AMP);
     g.src[0] = h->psi; g.src[1] = gp[0]; g.src[2] = gp[1]; g.src[3] = gp[2];
     HIPTRY(h, h->axpass(dt, 2, nullptr, h->amp, g));
     g.op = OP_AX_GAUSS;
@@ -603,12 +620,14 @@ int run_step(smaiso_handle* h) {
   {
     Fuse g = fb;
     g.op = OP_LD_SH; g.src[0] = h->psiq;
+    g.arith32 = h->a32(SMAISO_STAGE_LPSI);
     HIPTRY(h, h->xwf(kICT, g, h->lpsi));
   }
   // a9 (S:1312-1380)
   if (h->nloop > h->p.divv_switch || h->p.load_mode == 1) {
     Fuse g = fb;
     g.op = OP_LD_PROD; g.src[0] = h->rho; g.src[1] = h->lpsi;
+    g.arith32 = h->a32(SMAISO_STAGE_LAPRHOL);
     HIPTRY(h, h->xmid(g, OP_MID_NEGLAP, h->laprl));
   } else {
     Fuse g = fb;
@@ -632,9 +651,10 @@ int run_step(smaiso_handle* h) {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
       HIPTRY(h, forces_zpass<T>(c, io, h->r2r, s));
     }
-    HIPTRY(h, h->xw_yx(kCT, h->fgrad));
-    HIPTRY(h, h->xw_yx(kCT, h->erho));
-    for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw_yx(kST[a], S[a]));
+    const int fa = h->a32(SMAISO_STAGE_FORCES);
+    HIPTRY(h, h->xw_yx(kCT, h->fgrad, fa));
+    HIPTRY(h, h->xw_yx(kCT, h->erho, fa));
+    for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw_yx(kST[a], S[a], fa));
   } else {
     {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
@@ -660,17 +680,20 @@ int run_step(smaiso_handle* h) {
   {
     Fuse g = fb;
     g.op = OP_LD_DIV; g.c0 = 1.0;
+    g.arith32 = h->a32(SMAISO_STAGE_DIVV);
     g.src[0] = virrq[0]; g.src[1] = virrq[1]; g.src[2] = virrq[2];
     HIPTRY(h, h->xwf(kICT, g, h->divv));
     Fuse d = fb;
     d.op = OP_AX_DIV;
+    d.arith32 = h->a32(SMAISO_STAGE_DIVF);
     for (int a = 2; a >= 0; --a) {
       d.axis = a;
       d.accumulate = a != 2;
       HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, f[a], h->divf, d));
     }
   }
-  for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kIST[a], vtq[a], v[a]));
+  for (int a = 0; a < 3; ++a)
+    HIPTRY(h, h->xw(kIST[a], vtq[a], v[a], nullptr, h->a32(SMAISO_STAGE_VELOCITY)));
   // a15 + a16 (S:2121-2162) and the transform of Nl (S:2166-2168): the z
   // pass of the transform computes them on load where the length allows
   if (density_zpass_supported(h->g.nz)) {
@@ -688,7 +711,7 @@ int run_step(smaiso_handle* h) {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
       HIPTRY(h, density_zpass<T>(c, io, h->r2r, s));
     }
-    HIPTRY(h, h->xw_yx(kCT, h->nlq));
+    HIPTRY(h, h->xw_yx(kCT, h->nlq, h->a32(SMAISO_STAGE_NL)));
   } else {
     {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
@@ -1078,6 +1101,21 @@ int smaiso_set_graph(smaiso_handle* h, int32_t enable) {
 }
 
 int64_t smaiso_step_count(const smaiso_handle* h) { return h ? h->nloop : -1; }
+
+uint32_t smaiso_fp32_stages_default(void) { return kFp32StagesDefault; }
+
+int smaiso_set_fp32_stages(smaiso_handle* h, uint32_t mask) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  if (mask & ~SMAISO_STAGES_ALL) return fail(h, SMAISO_E_INVALID, fmt("unknown stage bits 0x%x", mask));
+  HIPTRY(h, hipSetDevice(h->device));
+  HIPTRY(h, hipStreamSynchronize(h->stream));
+  h->fp32_stages = mask;
+  // a captured step graph holds the old passes
+  if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  h->graph_parity = -1;
+  return SMAISO_OK;
+}
 
 int smaiso_get_field_device(smaiso_handle* h, int32_t field, void* out_dev) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");

@@ -73,7 +73,14 @@ struct Fuse {
   const void* nlq_old = nullptr;
   int first = 0;                  // first step: Nl_old = Nl
   int accumulate = 0;             // OP_AX_DIV: add the result to out
+  // FP32 storage only: compute the pass in FP32 (FP32 tile and tables)
+  // where the length has such instances (r2r_arith32_supported); otherwise,
+  // and for FP64 storage, the pass computes in FP64
+  int arith32 = 0;
 };
+
+// Lengths with FP32-arithmetic instances of the transform passes.
+inline bool r2r_arith32_supported(int n) { return n == 64 || n == 96 || n == 128 || n == 256 || n == 512; }
 
 struct PassIO {
   const void* in = nullptr;

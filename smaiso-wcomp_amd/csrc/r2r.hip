@@ -3,6 +3,7 @@
 #include "r2r_kernels.h"
 
 #include <cstdio>
+#include <type_traits>
 
 namespace smaiso {
 
@@ -59,31 +60,41 @@ hipError_t R2R::tables(DType dt, int n, Tables** out) {
   return hipSuccess;
 }
 
-template <typename T, int N>
+template <typename T, int N, typename A = double>
 static hipError_t launch_fft(const PassIO& io, void* tw, void* mk, int kind, int axis,
                              const Geom& g, hipStream_t s) {
   using C = TileChoice<T, N>;
   constexpr int PZ = C::PZ, NTZ = C::NTZ, PS = C::PS, NTS = C::NTS;
   if (axis == 2) {
     switch (io.f.op) {
-      case OP_LD_GRAD_D: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_D>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_GRAD_T: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_T>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_AMP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_AMP>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_SH: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_SH>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_RELAXLAP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_RELAXLAP>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_PROD: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PROD>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_DIV: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_DIV>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_PSI: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PSI>(io, tw, mk, kind, axis, g, s);
-      case OP_PLAIN: return launch_fft_pn<T, N, PZ, NTZ, OP_PLAIN>(io, tw, mk, kind, axis, g, s);
-      default: return hipErrorInvalidValue;
+      case OP_LD_SH: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_SH, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_RELAXLAP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_RELAXLAP, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_PROD: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PROD, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_DIV: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_DIV, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_PLAIN: return launch_fft_pn<T, N, PZ, NTZ, OP_PLAIN, T, A>(io, tw, mk, kind, axis, g, s);
+      default: break;
     }
+    // load ops the step runs in FP64 arithmetic only
+    if constexpr (std::is_same<A, double>::value) {
+      switch (io.f.op) {
+        case OP_LD_GRAD_D: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_D>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_GRAD_T: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_T>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_AMP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_AMP>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_PSI: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PSI>(io, tw, mk, kind, axis, g, s);
+        default: break;
+      }
+    }
+    return hipErrorInvalidValue;
   }
   switch (io.f.op) {
-    case OP_MID_AMP: return launch_fft_pn<T, N, PS, NTS, OP_MID_AMP>(io, tw, mk, kind, axis, g, s);
-    case OP_MID_NEGLAP: return launch_fft_pn<T, N, PS, NTS, OP_MID_NEGLAP>(io, tw, mk, kind, axis, g, s);
-    case OP_PLAIN: return launch_fft_pn<T, N, PS, NTS, OP_PLAIN>(io, tw, mk, kind, axis, g, s);
-    default: return hipErrorInvalidValue;
+    case OP_MID_NEGLAP: return launch_fft_pn<T, N, PS, NTS, OP_MID_NEGLAP, T, A>(io, tw, mk, kind, axis, g, s);
+    case OP_PLAIN: return launch_fft_pn<T, N, PS, NTS, OP_PLAIN, T, A>(io, tw, mk, kind, axis, g, s);
+    default: break;
   }
+  if constexpr (std::is_same<A, double>::value) {
+    if (io.f.op == OP_MID_AMP) return launch_fft_pn<T, N, PS, NTS, OP_MID_AMP>(io, tw, mk, kind, axis, g, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -105,15 +116,28 @@ static hipError_t launch_direct(const PassIO& io, void* costab, int kind, int ax
   return hipGetLastError();
 }
 
-template <typename T>
+// FP32 arithmetic (A = float, FP32 tables) exists for T = float and the
+// lengths of r2r_arith32_supported only; the caller checks both.
+template <typename T, typename A = double>
 static hipError_t dispatch(int n, const PassIO& io, void* tw, void* mk, void* ct, int kind,
                            int axis, const Geom& g, hipStream_t s) {
+  constexpr bool a32 = std::is_same<A, float>::value;
   if (n == 256 && axis != 2) {
     const DType dt = sizeof(T) == 8 ? F64 : F32;
-    const hipError_t e = launch_yx256(dt, dt, kind, axis, io, g, tw, mk, s);
+    const hipError_t e = launch_yx256(dt, dt, kind, axis, io, g, tw, mk, s, a32);
     if (e != hipErrorNotSupported) return e;
   }
-  if (is_axis_op(io.f.op) && R2R::fft_supported(n)) return launch_axis_op<T>(n, io, tw, mk, axis, g, s);
+  if (is_axis_op(io.f.op) && R2R::fft_supported(n)) return launch_axis_op<T, A>(n, io, tw, mk, axis, g, s);
+  if constexpr (a32) {
+    switch (n) {
+      case 64: return launch_fft<T, 64, A>(io, tw, mk, kind, axis, g, s);
+      case 96: return launch_fft<T, 96, A>(io, tw, mk, kind, axis, g, s);
+      case 128: return launch_fft<T, 128, A>(io, tw, mk, kind, axis, g, s);
+      case 256: return launch_fft<T, 256, A>(io, tw, mk, kind, axis, g, s);
+      case 512: return launch_fft<T, 512, A>(io, tw, mk, kind, axis, g, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (n) {
     case 16: return launch_fft<T, 16>(io, tw, mk, kind, axis, g, s);
     case 32: return launch_fft<T, 32>(io, tw, mk, kind, axis, g, s);
@@ -143,9 +167,13 @@ hipError_t R2R::pass(DType dt, int kind, int axis, const PassIO& io, const Geom&
                      hipStream_t s) {
   if (!op_valid(io.f.op, axis)) return hipErrorInvalidValue;
   const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  // FP64 arithmetic and tables for both storage types unless FP32
+  // arithmetic is asked for (FP32 storage, a length with FP32 instances)
+  const bool a32 = dt == F32 && io.f.arith32 && r2r_arith32_supported(n);
   Tables* tb = nullptr;
-  hipError_t e = tables(F64, n, &tb);  // FP64 tables for both storage types
+  hipError_t e = tables(a32 ? F32 : F64, n, &tb);
   if (e != hipSuccess) return e;
+  if (a32) return dispatch<float, float>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
   if (io.f.op == OP_PLAIN && !tuned_.empty()) {
     const auto it = tuned_.find(tune_key(dt, axis, g));
     if (it != tuned_.end() && it->second > 0) {

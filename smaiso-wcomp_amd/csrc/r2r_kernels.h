@@ -220,8 +220,10 @@ struct PassArgs {
   const T* in;
   const T* in2;
   TO* out;
-  const cplx<double>* tw;  // FP64 tables for both storage types
-  const cplx<double>* mk;
+  // twiddle and Makhoul tables of the tile's arithmetic type (cplx<A>: FP64
+  // unless the pass computes in FP32)
+  const void* tw;
+  const void* mk;
   int nx, ny, nz;
   int axis;
   int kind;
@@ -655,6 +657,12 @@ struct MkSeq {
   }
 };
 
+// A pair loaded in FP64 as the tile's arithmetic type.
+template <typename A>
+__device__ __forceinline__ cplx<A> to_arith(cplx<double> v) {
+  return cplx<A>{(A)v.x, (A)v.y};
+}
+
 // Minimum waves per SIMD a pass is compiled for (its register cap), for
 // tiles whose threads hold at most 8 values (2 P N / NT): the 256-thread y/x
 // tiles are held to 4 by their LDS anyway, so 128 VGPRs cost nothing there;
@@ -668,7 +676,7 @@ constexpr int pass_min_waves(int nt, int op, int per_thread) {
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
 // form complex sequence c.  OP selects a fused operation (FuseOp); the
 // output is stored as TO.
-template <typename T, int N, int P, int NT, int OP, typename TO = T>
+template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double>
 __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void fft_pass_kernel(PassArgs<T, TO> a) {
   constexpr int TL = 2 * P;
   constexpr int MAXR = pick_maxr(N, P, NT);
@@ -681,11 +689,14 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   // load op of the first (z) pass
   constexpr int LD_OP = OP == OP_AX_GAUSS_AMP ? OP_LD_AMP : ((MID || AX) ? OP_PLAIN : OP);
   constexpr int AXOP = OP == OP_AX_GAUSS_AMP ? OP_AX_GAUSS : OP;
-  // the tile and every operation on it are FP64 for both storage types
-  using A = double;
+  // the tile and every operation on it are of type A: FP64 (both storage
+  // types), or FP32 for FP32 storage where FP32 arithmetic is asked for; the
+  // tables are of type A too
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
   const int tid = threadIdx.x;
+  const cplx<A>* const tw = reinterpret_cast<const cplx<A>*>(a.tw);
+  const cplx<A>* const mk = reinterpret_cast<const cplx<A>*>(a.mk);
   TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
   const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
   g.base += mo;
@@ -695,9 +706,9 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   const bool sine = AX ? (OP == OP_AX_DIV || OP == OP_AX_GRAD_T) : (!MID && (a.kind == RODFT10 || a.kind == RODFT01));
   const bool sine_out = AX ? (OP == OP_AX_GRAD || OP == OP_AX_DIV_T) : sine;
   StageTw<A, N, P, NT, MAXR> stw;
-  stw.load(a.tw, tid);
+  stw.load(tw, tid);
   // the Makhoul factors' first loads, also ahead of the tile's data
-  const MkSeq<A, N, P, NT> ms0(a.mk, a.axis, tid);
+  const MkSeq<A, N, P, NT> ms0(mk, a.axis, tid);
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
@@ -744,7 +755,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
         const int e = tid + it * NT;
-        v[it] = (E % NT == 0 || e < E) ? load_pair(a, g, 2 * (e % P), e / P) : cplx<A>{A(0), A(0)};
+        v[it] = (E % NT == 0 || e < E) ? to_arith<A>(load_pair(a, g, 2 * (e % P), e / P)) : cplx<A>{A(0), A(0)};
       }
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
@@ -792,8 +803,8 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
             rn[it].y = load_elem<T, OP>(a, i1, j1, pb, mo + (g0 + 1) * N + pb);
           }
         } else {
-          rk[it] = load_pair(a, g, 2 * c, pa);
-          rn[it] = load_pair(a, g, 2 * c, pb);
+          rk[it] = to_arith<A>(load_pair(a, g, 2 * c, pa));
+          rn[it] = to_arith<A>(load_pair(a, g, 2 * c, pb));
         }
       }
     }
@@ -807,7 +818,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
         else { k = w / P; c = w - k * P; }
         const int kb = (k == 0) ? HN : N - k;
         cplx<A> zk, zb;
-        makhoul_pre<A, N>(a.mk, k, ms.get(a.mk, a.axis, k), rk[it], rn[it], zk, zb);
+        makhoul_pre<A, N>(mk, k, ms.get(mk, a.axis, k), rk[it], rn[it], zk, zb);
         buf[c * SEQ + lp<SH>(k)] = zk;
         buf[c * SEQ + lp<SH>(kb)] = zb;
       }
@@ -817,7 +828,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   __syncthreads();
 
   if (fwd) {
-    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, a.tw, stw);
+    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
     constexpr int W = P * HN;
     constexpr int ITW = (W + NT - 1) / NT;
     if constexpr (MID) {
@@ -832,17 +843,17 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          const cplx<A> mkk = ms.get(a.mk, a.axis, k);
+          const cplx<A> mkk = ms.get(mk, a.axis, k);
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
           const int jj = g.outer;
           const int kc0 = g.k0 + 2 * c < a.nz ? g.k0 + 2 * c : a.nz - 1;
           const int kc1 = g.k0 + 2 * c + 1 < a.nz ? g.k0 + 2 * c + 1 : a.nz - 1;
-          const cplx<A> rk = {mid_apply<OP>(a.f, k, jj, kc0, y0a), mid_apply<OP>(a.f, k, jj, kc1, y1a)};
-          const cplx<A> rn = {mid_apply<OP>(a.f, kb, jj, kc0, y0b), mid_apply<OP>(a.f, kb, jj, kc1, y1b)};
+          const cplx<A> rk = {(A)mid_apply<OP>(a.f, k, jj, kc0, y0a), (A)mid_apply<OP>(a.f, k, jj, kc1, y1a)};
+          const cplx<A> rn = {(A)mid_apply<OP>(a.f, kb, jj, kc0, y0b), (A)mid_apply<OP>(a.f, kb, jj, kc1, y1b)};
           cplx<A> zk, zb;
-          makhoul_pre<A, N>(a.mk, k, mkk, rk, rn, zk, zb);
+          makhoul_pre<A, N>(mk, k, mkk, rk, rn, zk, zb);
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
@@ -867,9 +878,9 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           if (a.axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          const cplx<A> mkk = ms.get(a.mk, a.axis, k);
+          const cplx<A> mkk = ms.get(mk, a.axis, k);
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+          makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           const int fa = sine ? N - 1 - k : k;
           const int pa = sine_out ? N - 1 - k : k;
           const int pb = sine_out ? N - 1 - kb : kb;
@@ -878,15 +889,15 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           if (sa >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pa, N);
             const cplx<A> y = sa == fa ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-            xk = cplx<A>{wgt * y.x, wgt * y.y};
+            xk = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
           }
           if (sb >= 0) {
             const double wgt = ax_weight<T, AXOP>(a.f, pb, N);
             const cplx<A> y = sb == fa ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-            xb = cplx<A>{wgt * y.x, wgt * y.y};
+            xb = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
           }
           cplx<A> zk, zb;
-          makhoul_pre<A, N>(a.mk, k, mkk, xk, xb, zk, zb);
+          makhoul_pre<A, N>(mk, k, mkk, xk, xb, zk, zb);
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
@@ -905,7 +916,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(a.mk, k, kb, ms.get(a.mk, a.axis, k), buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)],
+          makhoul_post<A, N>(mk, k, kb, ms.get(mk, a.axis, k), buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)],
                              y0a, y1a, y0b, y1b);
           const int oa = sine ? (N - 1 - k) : k;
           const int ob = sine ? (N - 1 - kb) : kb;
@@ -933,7 +944,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   }
 
   // ---- inverse FFT and store: y_{2n} = u_n, y_{2n+1} = u_{N-1-n}
-  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, a.tw, stw);
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
   constexpr int WO = P * HN;
   constexpr int ITO = (WO + NT - 1) / NT;
   const A s = sine_out ? A(-1) : A(1);
@@ -1131,8 +1142,8 @@ PassArgs<T, TO> make_args(const PassIO& io, void* tw, void* mk, int kind, int ax
   a.in = static_cast<const T*>(io.in);
   a.in2 = static_cast<const T*>(io.in2);
   a.out = static_cast<TO*>(io.out);
-  a.tw = static_cast<const cplx<double>*>(tw);
-  a.mk = static_cast<const cplx<double>*>(mk);
+  a.tw = tw;
+  a.mk = mk;
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
   a.axis = axis;
   a.kind = kind;
@@ -1142,7 +1153,7 @@ PassArgs<T, TO> make_args(const PassIO& io, void* tw, void* mk, int kind, int ax
   return a;
 }
 
-template <typename T, int N, int P, int NT, int OP, typename TO = T>
+template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double>
 hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
                          hipStream_t s) {
   constexpr int TL = 2 * P;
@@ -1150,8 +1161,8 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
-  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
-  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP, TO>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<A>);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP, TO, A>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 
@@ -1184,11 +1195,11 @@ struct TileChoice {
 // launched.  Storage in_dt -> out_dt: equal, or FP64 -> FP32 for a plain
 // pass.  Defined in r2r_yx256.hip.
 hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io, const Geom& g,
-                        const void* tw, const void* mk, hipStream_t s);
+                        const void* tw, const void* mk, hipStream_t s, bool arith32 = false);
 
 // One-axis operator pass (OP_AX_*) of FFT length n with the default tiles;
 // defined in r2r_axis_ops.hip.
-template <typename T>
+template <typename T, typename A = double>
 hipError_t launch_axis_op(int n, const PassIO& io, void* tw, void* mk, int axis, const Geom& g,
                           hipStream_t s);
 

@@ -47,8 +47,8 @@ template <typename T, typename TO>
 struct Args {
   const T* in;
   TO* out;
-  const cplx<double>* tw;  // exp(-2 pi i m/256)
-  const cplx<double>* mk;  // exp(-i pi m/512)
+  const void* tw;  // exp(-2 pi i m/256), cplx<A>
+  const void* mk;  // exp(-i pi m/512), cplx<A>
   int nx, ny, nz;
   int kind;
   int ntiles;  // tiles per member
@@ -56,10 +56,10 @@ struct Args {
   Fuse f;
 };
 
-template <typename T>
-__device__ __forceinline__ cplx<double> ld2(const T* p) {
+template <typename A, typename T>
+__device__ __forceinline__ cplx<A> ld2(const T* p) {
   const cplx<T> u = *reinterpret_cast<const cplx<T>*>(p);
-  return cplx<double>{(double)u.x, (double)u.y};
+  return cplx<A>{(A)u.x, (A)u.y};
 }
 
 template <typename TO>
@@ -73,12 +73,13 @@ __device__ __forceinline__ void st2(TO* p, double a, double b, bool acc) {
 }
 
 // v[q] *= w^q, q = 1..15, powers of w by a product tree of depth 3
-__device__ __forceinline__ void stage_twiddle(cplx<double>* v, cplx<double> w1) {
-  using C = cplx<double>;
+template <typename A>
+__device__ __forceinline__ void stage_twiddle(cplx<A>* v, cplx<A> w1) {
+  using C = cplx<A>;
   const C w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
   const C w8 = cmul(w4, w4), w12 = cmul(w8, w4);
-  const C lo[4] = {C{1.0, 0.0}, w1, w2, w3};
-  const C hi[4] = {C{1.0, 0.0}, w4, w8, w12};
+  const C lo[4] = {C{A(1), A(0)}, w1, w2, w3};
+  const C hi[4] = {C{A(1), A(0)}, w4, w8, w12};
 #pragma unroll
   for (int q = 1; q < 16; ++q) {
     const int u = q >> 2, l = q & 3;
@@ -92,10 +93,11 @@ constexpr bool ax_op() {
   return OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GRAD_T || OP == OP_AX_DIV_T;
 }
 
-// AXIS: 1 = y pass, 0 = x pass.  P sequences (2P columns) per tile.
-template <typename T, typename TO, int AXIS, int MODE, int OP, int P>
+// AXIS: 1 = y pass, 0 = x pass.  P sequences (2P columns) per tile.  A:
+// the arithmetic type of the tile and the tables (FP64, or FP32 for FP32
+// storage where FP32 arithmetic is asked for).
+template <typename T, typename TO, int AXIS, int MODE, int OP, int P, typename A = double>
 __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
-  using A = double;
   using C = cplx<A>;
   constexpr int N = kN;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -116,9 +118,11 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
                         : MODE == kInv ? sine_in
                                        : (OP == OP_AX_GRAD || OP == OP_AX_DIV_T);
   const bool acc = AX && a.f.accumulate;
-  const C wf = a.tw[j];                       // stage-2 twiddle base, forward
+  const C* const tw = reinterpret_cast<const C*>(a.tw);
+  const C* const mkt = reinterpret_cast<const C*>(a.mk);
+  const C wf = tw[j];                         // stage-2 twiddle base, forward
   const C wi = C{wf.x, -wf.y};                // and inverse
-  const C mk0 = a.mk[j], mks = a.mk[16];      // Makhoul factors mk[j + 16 q] by products
+  const C mk0 = mkt[j], mks = mkt[16];        // Makhoul factors mk[j + 16 q] by products
 
   auto tile = [&](int t, int& outer, int& k0) -> long long {
     int o, kc;
@@ -132,17 +136,17 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
   auto load = [&](C* v, int t) {
     int outer, k0;
     const T* p = a.in + tile(t, outer, k0) + 2 * c;
-    if (MODE == kInv) {
+    if constexpr (MODE == kInv) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int k = j + 16 * q;
-        v[q] = ld2(p + (sine_in ? N - 1 - k : k) * stride);
+        v[q] = ld2<A>(p + (sine_in ? N - 1 - k : k) * stride);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int row = q < 8 ? 2 * j + 32 * q : 2 * N - 1 - 2 * j - 32 * q;
-        v[q] = ld2(p + row * stride);
+        v[q] = ld2<A>(p + row * stride);
       }
     }
   };
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
     if (kPrefetch && tn < a.ntiles) load(nxt, tn);
     int outer, k0;
     TO* const o = a.out + tile(t, outer, k0) + 2 * c;
-    if (MODE == kInv) {
+    if constexpr (MODE == kInv) {
       // pre-twiddle: z_k = conj(mk[k]) (Y0_k + Y1_{N-k} + i (Y1_k - Y0_{N-k})), Y_N = 0
 #pragma unroll
       for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
       for (int q = 0; q < 16; ++q) {
         const int k = j + 16 * q;
         const C rk = v[q];
-        const C rn = k == 0 ? C{0.0, 0.0} : s[slot((N - k) & (N - 1))];
+        const C rn = k == 0 ? C{A(0), A(0)} : s[slot((N - k) & (N - 1))];
         v[q] = cmul(C{m.x, -m.y}, C{rk.x + rn.y, rk.y - rn.x});
         m = cmul(m, mks);
       }
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
       for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
       __syncthreads();
       C m = mk0;
-      if (MODE == kFwd) {
+      if constexpr (MODE == kFwd) {
         // Y_k = (Re(mk[k] (Z_k + conj Z_{N-k})), Im(mk[k] (Z_k - conj Z_{N-k})))
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -225,11 +229,11 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
         }
       } else {
         // y and z parts of |q|^2 of the pair's two columns (mid op)
-        C qyz = {0.0, 0.0};
+        C qyz = {A(0), A(0)};
         if constexpr (!AX) {
-          const A qy = vq_at(a.f, 1, outer);
-          const A q0 = vq_at(a.f, 2, k0 + 2 * c), q1 = vq_at(a.f, 2, k0 + 2 * c + 1);
-          qyz = C{qy * qy + q0 * q0, qy * qy + q1 * q1};
+          const double qy = vq_at(a.f, 1, outer);
+          const double q0 = vq_at(a.f, 2, k0 + 2 * c), q1 = vq_at(a.f, 2, k0 + 2 * c + 1);
+          qyz = C{(A)(qy * qy + q0 * q0), (A)(qy * qy + q1 * q1)};
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -245,15 +249,15 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
             const int fa = sine_in ? N - 1 - k : k;
             const int pa = sine_out ? N - 1 - k : k, pb = sine_out ? N - 1 - kb : kb;
             const int sa = ax_source<OP>(pa, N), sb = ax_source<OP>(pb, N);
-            xk = C{0.0, 0.0};
-            xb = C{0.0, 0.0};
+            xk = C{A(0), A(0)};
+            xb = C{A(0), A(0)};
             if (sa >= 0) {
-              const A w = ax_weight<T, OP>(a.f, pa, N);
+              const A w = (A)ax_weight<T, OP>(a.f, pa, N);
               const C y = sa == fa ? ya : yb;
               xk = C{w * y.x, w * y.y};
             }
             if (sb >= 0) {
-              const A w = ax_weight<T, OP>(a.f, pb, N);
+              const A w = (A)ax_weight<T, OP>(a.f, pb, N);
               const C y = sb == fa ? ya : yb;
               xb = C{w * y.x, w * y.y};
             }
@@ -261,11 +265,11 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
             // x pass of -scale |q|^2 (OP_MID_NEGLAP): mode index k along x,
             // y index = outer, z columns k0 + 2c and k0 + 2c + 1, in the
             // reference's order (-scale |q|^2) y
-            const A qk = vq_at(a.f, 0, k), qb = vq_at(a.f, 0, kb);
-            xk = C{(-a.f.scale * (qk * qk + qyz.x)) * ya.x, (-a.f.scale * (qk * qk + qyz.y)) * ya.y};
-            xb = C{(-a.f.scale * (qb * qb + qyz.x)) * yb.x, (-a.f.scale * (qb * qb + qyz.y)) * yb.y};
+            const A qk = (A)vq_at(a.f, 0, k), qb = (A)vq_at(a.f, 0, kb), sc = (A)a.f.scale;
+            xk = C{(-sc * (qk * qk + qyz.x)) * ya.x, (-sc * (qk * qk + qyz.y)) * ya.y};
+            xb = C{(-sc * (qb * qb + qyz.x)) * yb.x, (-sc * (qb * qb + qyz.y)) * yb.y};
           }
-          const C rn = k == 0 ? C{0.0, 0.0} : xb;
+          const C rn = k == 0 ? C{A(0), A(0)} : xb;
           v[q] = cmul(C{m.x, -m.y}, C{xk.x + rn.y, xk.y - rn.x});
           m = cmul(m, mks);
           if ((q & 1) == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the registers in flight

@@ -10,14 +10,14 @@ namespace smaiso {
 
 namespace {
 
-template <typename T, typename TO, int AXIS, int MODE, int OP>
+template <typename T, typename TO, int AXIS, int MODE, int OP, typename A>
 hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, const void* mk, hipStream_t s) {
   constexpr int P = yx256::tile_p<T>();
   yx256::Args<T, TO> a;
   a.in = static_cast<const T*>(io.in);
   a.out = static_cast<TO*>(io.out);
-  a.tw = static_cast<const cplx<double>*>(tw);
-  a.mk = static_cast<const cplx<double>*>(mk);
+  a.tw = tw;
+  a.mk = mk;
   a.nx = g.nx;
   a.ny = g.ny;
   a.nz = g.nz;
@@ -28,18 +28,18 @@ hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, con
   // two resident workgroups per CU for FP64 tiles (128 threads), one for
   // FP32 tiles (256 threads); twice that many keep every CU's queue full
   const int grid = std::min(a.ntiles, 256 * (P == 8 ? 2 : 1) * 2);
-  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<double>);
-  hipLaunchKernelGGL((yx256::pass_kernel<T, TO, AXIS, MODE, OP, P>), dim3((unsigned)grid, (unsigned)g.nb),
+  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<A>);
+  hipLaunchKernelGGL((yx256::pass_kernel<T, TO, AXIS, MODE, OP, P, A>), dim3((unsigned)grid, (unsigned)g.nb),
                      dim3(16 * P), lds, s, a);
   return hipGetLastError();
 }
 
-template <typename T, typename TO, int AXIS>
+template <typename T, typename TO, int AXIS, typename A>
 hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
   switch (io.f.op) {
     case OP_PLAIN:
-      if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, OP_PLAIN>(io, g, kind, tw, mk, s);
-      return launch<T, TO, AXIS, yx256::kInv, OP_PLAIN>(io, g, kind, tw, mk, s);
+      if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, OP_PLAIN, A>(io, g, kind, tw, mk, s);
+      return launch<T, TO, AXIS, yx256::kInv, OP_PLAIN, A>(io, g, kind, tw, mk, s);
     default:
       break;
   }
@@ -49,7 +49,7 @@ hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw
   return hipErrorNotSupported;
 }
 
-template <typename T, typename TO>
+template <typename T, typename TO, typename A = double>
 hipError_t launch_typed(int kind, int axis, const PassIO& io, const Geom& g, const void* tw, const void* mk,
                         hipStream_t s) {
   constexpr int P = yx256::tile_p<T>();
@@ -60,14 +60,18 @@ hipError_t launch_typed(int kind, int axis, const PassIO& io, const Geom& g, con
   if (n != yx256::kN || g.nz % (2 * P) != 0 || !aligned || io.in2 != nullptr ||
       (yx256::kN - 1) * stride + g.nz > 2147483647LL)
     return hipErrorNotSupported;
-  return axis == 1 ? launch_axis<T, TO, 1>(kind, io, g, tw, mk, s) : launch_axis<T, TO, 0>(kind, io, g, tw, mk, s);
+  return axis == 1 ? launch_axis<T, TO, 1, A>(kind, io, g, tw, mk, s) : launch_axis<T, TO, 0, A>(kind, io, g, tw, mk, s);
 }
 
 }  // namespace
 
 hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io, const Geom& g,
-                        const void* tw, const void* mk, hipStream_t s) {
+                        const void* tw, const void* mk, hipStream_t s, bool arith32) {
   if (axis != 0 && axis != 1) return hipErrorNotSupported;
+  if (arith32) {
+    if (in_dt != F32 || out_dt != F32) return hipErrorNotSupported;
+    return launch_typed<float, float, float>(kind, axis, io, g, tw, mk, s);
+  }
   if (in_dt == F64 && out_dt == F64) return launch_typed<double, double>(kind, axis, io, g, tw, mk, s);
   if (in_dt == F32 && out_dt == F32) return launch_typed<float, float>(kind, axis, io, g, tw, mk, s);
   if (in_dt == F64 && out_dt == F32) return launch_typed<double, float>(kind, axis, io, g, tw, mk, s);

@@ -33,6 +33,15 @@ FIELDS = {
     "rdx": 24, "rdy": 25, "rdz": 26, "lap_rl": 27, "psi_old": 28,
 }
 TILE_VARIANTS = 4
+# stages of the FP32-mixed step that may compute in FP32 (SMAISO_STAGE_*)
+STAGES = {"grad_psi": 1 << 0, "amp": 1 << 1, "lpsi": 1 << 2, "laprhol": 1 << 3, "forces": 1 << 4,
+          "divv": 1 << 5, "divf": 1 << 6, "velocity": 1 << 7, "nl": 1 << 8}
+
+
+def fp32_stages_default() -> int:
+    return int(lib().smaiso_fp32_stages_default())
+
+
 KERNEL_CLASSES = {"r2r_z": 0, "r2r_y": 1, "r2r_x": 2, "spectral": 3, "pointwise": 4, "reduce": 5, "r2r_axis": 6}
 
 
@@ -90,6 +99,11 @@ def lib():
         L.smaiso_synchronize.argtypes = [vp]
         L.smaiso_step_count.argtypes = [vp]
         L.smaiso_step_count.restype = C.c_int64
+        L.smaiso_set_fp32_stages.argtypes = [vp, C.c_uint32]
+        L.smaiso_fp32_stages_default.argtypes = []
+        L.smaiso_fp32_stages_default.restype = C.c_uint32
+        L.smaiso_r2r3d_noalloc.argtypes = [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, dp, dp,
+                                           C.c_int32]
         L.smaiso_get_field.argtypes = [vp, C.c_int32, dp]
         L.smaiso_get_field_device.argtypes = [vp, C.c_int32, vp]
         L.smaiso_diagnostics.argtypes = [vp, dp, dp, dp]
@@ -271,6 +285,10 @@ class Stepper:
     @property
     def nloop(self) -> int:
         return int(lib().smaiso_step_count(self.h))
+
+    def set_fp32_stages(self, mask: int):
+        """FP32-arithmetic stages of an FP32-mixed handle (STAGES bits)."""
+        _check(lib().smaiso_set_fp32_stages(self.h, int(mask)), self.h)
 
     def field(self, name: str) -> np.ndarray:
         out = np.empty(self.params.shape, dtype=np.float64)

@@ -325,3 +325,51 @@ def test_config2_trace_against_oracle_fixture():
         assert errs["energy"] <= 1e-10 and errs["l1"] <= 1e-10, (done, errs)
     print(f"{done} steps against the oracle trace, worst relative errors {worst}")
     st.close()
+
+
+@pytest.mark.parametrize("mask", ["default", "all"])
+def test_mixed_fp32_stages_one_step_every_field(mask):
+    """FP32 arithmetic in the transform passes (the 256-point y/x kernels and
+    the 64-point z, one-axis and fused-load kernels of this grid) against the
+    FP64 oracle after one step: each FP32 pass rounds relative to its line's
+    norm, so every field stays within 2e-5 of its scale (a wrong kernel
+    would be off by O(1)); with the default stages psi, rho and psiq stay
+    at FP32 storage accuracy."""
+    sp, op = params_pair(nx=256, ny=256, nz=64, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0,
+                         precision=S.FP32_MIXED)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(undulation=0.5, seed=7))
+    o = O.Oracle(op)
+    o.set_state(psi, rho)
+    o.step(1)
+    st = S.Stepper(sp)
+    if mask == "all":
+        st.set_fp32_stages(sum(S.STAGES.values()))
+    st.set_state(psi, rho)
+    st.step(1)
+    ref = oracle_fields(o)
+    sc = group_scales(ref)
+    worst = {}
+    for name in S.FIELDS:
+        mx, l2 = rel_err(st.field(name), ref[name], *sc[name])
+        worst[name] = mx
+        assert mx < 2e-5 and l2 < 2e-5, (name, mx, l2)
+    if mask == "default":
+        for name in ("psi", "rho", "psiq"):
+            assert worst[name] < 1e-6, (name, worst[name])
+    st.close()
+
+
+def test_fp32_stages_api():
+    assert S.fp32_stages_default() == (S.STAGES["forces"] | S.STAGES["divv"] | S.STAGES["divf"]
+                                       | S.STAGES["velocity"] | S.STAGES["nl"])
+    sp, op = config1(32)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=3))
+    a, b = S.Stepper(sp), S.Stepper(sp)
+    b.set_fp32_stages(sum(S.STAGES.values()))  # an FP64 handle ignores the stages
+    for st in (a, b):
+        st.set_state(psi, rho)
+        st.step(3)
+    assert np.array_equal(a.field("psi"), b.field("psi"))
+    with pytest.raises(S.SmaisoError) as e:
+        a.set_fp32_stages(1 << 12)
+    assert e.value.code == S.E_INVALID
