@@ -6,7 +6,7 @@
 // the member as blockIdx.y.  One call of run_step<T>() is one iteration of the
 // reference's loop body (smaiso-wcomp.cpp:1081-2198) for all members.
 #include "../../include/smaiso.h"
-#include "abi.h"
+#include "host_abi.h"
 #include "common.h"
 #include "adjoint.h"
 #include "fused.h"
@@ -28,8 +28,6 @@
 using namespace smaiso;
 
 namespace {
-
-thread_local std::string g_create_error = "no error";
 
 std::string fmt(const char* f, ...) {
   char buf[512];
@@ -379,16 +377,10 @@ struct smaiso_ensemble {
 namespace {
 
 int fail(smaiso_handle* h, int code, const std::string& msg) {
-  if (h) h->err = msg;
-  else g_create_error = msg;
+  if (!h) return abi_fail(code, msg);
+  h->err = msg;
   return code;
 }
-
-}  // namespace
-
-int smaiso::abi_fail(int code, const std::string& msg) { return fail(nullptr, code, msg); }
-
-namespace {
 
 #define HIPTRY(h, expr)                                                                    \
   do {                                                                                     \
@@ -397,45 +389,6 @@ namespace {
       return fail((h), _e == hipErrorOutOfMemory ? SMAISO_E_NOMEM : SMAISO_E_HIP,         \
                   fmt("%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__)); \
   } while (0)
-
-bool finite_all(std::initializer_list<double> v) {
-  for (double x : v)
-    if (!std::isfinite(x)) return false;
-  return true;
-}
-
-int validate(const smaiso_params* p, std::string* why) {
-  if (!p) { *why = "params is NULL"; return SMAISO_E_INVALID; }
-  if (p->nx < 2 || p->ny < 2 || p->nz < 2) {
-    *why = fmt("grid %dx%dx%d: every extent must be >= 2", p->nx, p->ny, p->nz);
-    return SMAISO_E_INVALID;
-  }
-  if ((long long)p->nx * p->ny * p->nz > 2147483647LL / 2) {
-    *why = "grid too large (Nx*Ny*Nz must stay below 2^30)";
-    return SMAISO_E_INVALID;
-  }
-  if (!finite_all({p->dx, p->dy, p->dz, p->dt, p->alpha, p->beta, p->gamma, p->epsilon, p->q0,
-                   p->nu, p->lambda, p->zeta, p->rho0, p->rho_m})) {
-    *why = "non-finite parameter";
-    return SMAISO_E_INVALID;
-  }
-  if (p->dx <= 0 || p->dy <= 0 || p->dz <= 0) { *why = "dx, dy, dz must be > 0"; return SMAISO_E_INVALID; }
-  if (p->dt <= 0) { *why = "dt must be > 0"; return SMAISO_E_INVALID; }
-  if (p->nu <= 0) { *why = "nu must be > 0"; return SMAISO_E_INVALID; }
-  if (p->lambda + 2 * p->nu <= 0) { *why = "lambda + 2 nu must be > 0"; return SMAISO_E_INVALID; }
-  const double disc = 9 * p->beta * p->beta + 40 * p->epsilon * p->gamma;
-  if (disc < 0 || (3 * p->beta + std::sqrt(disc)) <= 0) {
-    *why = "Amp = sqrt((3 beta + sqrt(9 beta^2 + 40 eps gamma))/5) is not a positive real";
-    return SMAISO_E_INVALID;
-  }
-  if (p->precision != SMAISO_FP64 && p->precision != SMAISO_FP32_MIXED) {
-    *why = fmt("unknown precision %d", p->precision);
-    return SMAISO_E_INVALID;
-  }
-  if (p->divv_switch < 0) { *why = "divv_switch must be >= 0"; return SMAISO_E_INVALID; }
-  if (p->slab_n0 < 0) { *why = "slab_n0 must be >= 0"; return SMAISO_E_INVALID; }
-  return SMAISO_OK;
-}
 
 double rho_m_eff(const smaiso_params& p) {
   const double amp = derived_amp(p);
@@ -1013,33 +966,11 @@ int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
 // ===========================================================================
 extern "C" {
 
-int smaiso_abi_version(void) { return SMAISO_ABI_VERSION; }
-
-void smaiso_params_default(smaiso_params* p) {
-  if (!p) return;
-  std::memset(p, 0, sizeof(*p));
-  p->nx = p->ny = p->nz = 512;                 // S:218
-  const double nw = 8, q0 = 1.0;               // S:261, S:244
-  p->dx = p->dy = p->dz = 2.0 * M_PI / (q0 * nw);  // L/N, S:267-272
-  p->dt = 0.001;                               // S:262
-  p->gamma = 1.0; p->beta = 2.0; p->alpha = 1.0;  // S:239-241
-  p->epsilon = -0.001 * 5;                     // ep_arg = 5, S:242-243
-  p->q0 = q0;
-  p->nu = 1.0;                                 // argv[1]
-  p->lambda = 1.0; p->zeta = 1.0;              // S:256-257
-  p->rho0 = 0.01;                              // S:251
-  p->rho_m = 0.0;                              // -> rho_s/2, S:255
-  p->divv_switch = 50;                         // S:214
-  p->load_mode = 0;
-  p->precision = SMAISO_FP64;
-  p->slab_n0 = 0;
-}
-
 int smaiso_create(const smaiso_params* p, smaiso_handle** out) {
   if (!out) return fail(nullptr, SMAISO_E_INVALID, "out is NULL");
   *out = nullptr;
   std::string why;
-  int rc = validate(p, &why);
+  int rc = validate_params(p, &why);
   if (rc != SMAISO_OK) return fail(nullptr, rc, why);
   return create_group(p, 1, out);
 }
@@ -1050,7 +981,7 @@ int smaiso_destroy(smaiso_handle* h) {
 }
 
 const char* smaiso_last_error(const smaiso_handle* h) {
-  return h ? h->err.c_str() : g_create_error.c_str();
+  return h ? h->err.c_str() : abi_error();
 }
 
 int smaiso_set_state(smaiso_handle* h, const double* psi, const double* rho) {
@@ -1135,8 +1066,8 @@ int smaiso_diagnostics(smaiso_handle* h, double* l1, double* mass, double* energ
 // --- ensembles --------------------------------------------------------------
 namespace {
 int efail(smaiso_ensemble* e, int code, const std::string& msg) {
-  if (e) e->err = msg;
-  else g_create_error = msg;
+  if (!e) return abi_fail(code, msg);
+  e->err = msg;
   return code;
 }
 int member_of(smaiso_ensemble* e, int32_t member, smaiso_handle** h, int* m) {
@@ -1155,7 +1086,7 @@ int smaiso_ensemble_create(const smaiso_params* members, int32_t count, smaiso_e
   if (!members || count < 1) return efail(nullptr, SMAISO_E_INVALID, "need at least one member");
   for (int i = 0; i < count; ++i) {
     std::string why;
-    int rc = validate(&members[i], &why);
+    int rc = validate_params(&members[i], &why);
     if (rc != SMAISO_OK) return efail(nullptr, rc, fmt("member %d: %s", i, why.c_str()));
   }
   std::unique_ptr<smaiso_ensemble> e(new smaiso_ensemble());
@@ -1177,7 +1108,7 @@ int smaiso_ensemble_create(const smaiso_params* members, int32_t count, smaiso_e
     }
     smaiso_handle* h = nullptr;
     int rc = create_group(ps.data(), (int)ps.size(), &h);
-    if (rc != SMAISO_OK) return efail(nullptr, rc, g_create_error);
+    if (rc != SMAISO_OK) return efail(nullptr, rc, abi_error());
     e->groups.emplace_back(h);
   }
   hipError_t he = hipGetDevice(&e->device);
@@ -1197,7 +1128,7 @@ int smaiso_ensemble_destroy(smaiso_ensemble* e) {
 }
 
 const char* smaiso_ensemble_last_error(const smaiso_ensemble* e) {
-  return e ? e->err.c_str() : g_create_error.c_str();
+  return e ? e->err.c_str() : abi_error();
 }
 
 int smaiso_ensemble_groups(const smaiso_ensemble* e) { return e ? (int)e->groups.size() : -1; }
@@ -1346,40 +1277,6 @@ int smaiso_r2r3d(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz, con
   return SMAISO_OK;
 }
 
-int smaiso_make_interface_ic(const smaiso_params* p, const smaiso_ic* ic, const double* height,
-                             double* psi, double* rho) {
-  if (!p || !ic || !psi || !rho) return fail(nullptr, SMAISO_E_INVALID, "NULL argument");
-  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(nullptr, SMAISO_E_INVALID, "bad grid");
-  if (!(ic->width > 0)) return fail(nullptr, SMAISO_E_INVALID, "width must be > 0");
-  const int nx = p->nx, ny = p->ny, nz = p->nz;
-  const double lx = nx * p->dx, ly = ny * p->dy, lz = nz * p->dz;
-  const double h0 = ic->h0 < 0 ? lz / 2 : ic->h0;
-  const uint64_t gam = 0x9E3779B97F4A7C15ULL;
-  uint64_t n = 0;
-  for (int i = 0; i < nx; ++i) {
-    const double x = i * p->dx;
-    for (int j = 0; j < ny; ++j) {
-      const double y = j * p->dy;
-      const double h = height ? height[(size_t)i * ny + j]
-                              : h0 + ic->undulation * std::cos(2 * M_PI * x / lx) * std::cos(2 * M_PI * y / ly);
-      for (int k = 0; k < nz; ++k) {
-        const double z = k * p->dz;
-        const double prof = 0.5 * (1.0 - std::tanh((z - h) / ic->width));
-        // splitmix64 (Steele, Lea, Flood 2014), element n uses seed + (n+1) gamma
-        ++n;
-        uint64_t zz = ic->seed + n * gam;
-        zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBULL;
-        zz = zz ^ (zz >> 31);
-        const double u = (double)(zz >> 11) * (1.0 / 9007199254740992.0);
-        const size_t idx = ((size_t)i * ny + j) * nz + k;
-        psi[idx] = ic->amplitude * std::cos(p->q0 * z) * prof + ic->noise * (2.0 * u - 1.0);
-        rho[idx] = ic->rho_iso + (ic->rho_sm - ic->rho_iso) * prof;
-      }
-    }
-  }
-  return SMAISO_OK;
-}
 
 // --- autotuner -----------------------------------------------------------------
 namespace {
@@ -1735,32 +1632,6 @@ int smaiso_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, con
   return run_adjoint(h, psi0, rho0, target, nsteps, checkpoint_every, loss, dpsi0, drho0, dzeta, dnu);
 }
 
-int smaiso_interface_ic_vjp(const smaiso_params* p, const smaiso_ic* ic, const double* height,
-                            const double* dpsi, const double* drho, double* dheight) {
-  if (!p || !ic || !dpsi || !drho || !dheight) return fail(nullptr, SMAISO_E_INVALID, "NULL argument");
-  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(nullptr, SMAISO_E_INVALID, "bad grid");
-  if (!(ic->width > 0)) return fail(nullptr, SMAISO_E_INVALID, "width must be > 0");
-  const int nx = p->nx, ny = p->ny, nz = p->nz;
-  const double lx = nx * p->dx, ly = ny * p->dy, lz = nz * p->dz;
-  const double h0 = ic->h0 < 0 ? lz / 2 : ic->h0;
-  for (int i = 0; i < nx; ++i) {
-    for (int j = 0; j < ny; ++j) {
-      const double hh = height ? height[(size_t)i * ny + j]
-                               : h0 + ic->undulation * std::cos(2 * M_PI * i * p->dx / lx) *
-                                          std::cos(2 * M_PI * j * p->dy / ly);
-      double acc = 0.0;
-      for (int k = 0; k < nz; ++k) {
-        const double z = k * p->dz;
-        const double ch = std::cosh((z - hh) / ic->width);
-        const double dprof = 0.5 / (ic->width * ch * ch);
-        const size_t idx = ((size_t)i * ny + j) * nz + k;
-        acc += (ic->amplitude * std::cos(p->q0 * z) * dpsi[idx] + (ic->rho_sm - ic->rho_iso) * drho[idx]) * dprof;
-      }
-      dheight[(size_t)i * ny + j] = acc;
-    }
-  }
-  return SMAISO_OK;
-}
 
 // --- output writers' data, checkpoints, the reference's psi files -------------
 int smaiso_get_slice(smaiso_handle* h, int32_t field, int32_t j, double* out) {
@@ -1804,17 +1675,6 @@ int smaiso_surface_height(smaiso_handle* h, double threshold, double* surf) {
   return SMAISO_OK;
 }
 
-namespace {
-// Checkpoint layout: this header, then psiq, psiq_old, rho, f_rho (FP64) and
-// psi, N^l of the last step (work precision), each Nx*Ny*Nz values.
-struct CkptHeader {
-  char magic[8];
-  int32_t version, nx, ny, nz, precision, elem;
-  int64_t nloop;
-};
-const char kCkptMagic[8] = {'S', 'M', 'A', 'I', 'S', 'O', 'C', 'K'};
-}  // namespace
-
 int smaiso_save_checkpoint(smaiso_handle* h, const char* path) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
   if (!path) return fail(h, SMAISO_E_INVALID, "path is NULL");
@@ -1822,13 +1682,8 @@ int smaiso_save_checkpoint(smaiso_handle* h, const char* path) {
   if (!h->all_set()) return fail(h, SMAISO_E_STATE, "no state (call smaiso_set_state first)");
   HIPTRY(h, hipSetDevice(h->device));
   HIPTRY(h, hipStreamSynchronize(h->stream));
-  CkptHeader hd{};
-  std::memcpy(hd.magic, kCkptMagic, 8);
-  hd.version = 1;
-  hd.nx = h->g.nx; hd.ny = h->g.ny; hd.nz = h->g.nz;
-  hd.precision = h->p.precision;
-  hd.elem = (int32_t)h->tsz;
-  hd.nloop = h->nloop;
+  CkptHeader hd;
+  ckpt_header_init(&hd, h->g.nx, h->g.ny, h->g.nz, h->p.precision, (int)h->tsz, h->nloop);
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(h, SMAISO_E_INVALID, fmt("cannot open %s for writing", path));
   const size_t n3 = (size_t)h->n3();
@@ -1852,21 +1707,11 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
   HIPTRY(h, hipSetDevice(h->device));
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(h, SMAISO_E_INVALID, fmt("cannot open %s", path));
-  CkptHeader hd{};
-  if (std::fread(&hd, sizeof(hd), 1, f) != 1 || std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1) {
+  CkptHeader hd;
+  std::string why;
+  if (ckpt_header_read(f, h->g.nx, h->g.ny, h->g.nz, h->p.precision, (int)h->tsz, &hd, &why) != SMAISO_OK) {
     std::fclose(f);
-    return fail(h, SMAISO_E_INVALID, fmt("%s is not a smaiso checkpoint", path));
-  }
-  if (hd.nloop < 0) {
-    std::fclose(f);
-    return fail(h, SMAISO_E_INVALID, fmt("%s: negative step count %lld", path, (long long)hd.nloop));
-  }
-  if (hd.nx != h->g.nx || hd.ny != h->g.ny || hd.nz != h->g.nz || hd.precision != h->p.precision ||
-      hd.elem != (int32_t)h->tsz) {
-    std::fclose(f);
-    return fail(h, SMAISO_E_INVALID,
-                fmt("checkpoint %dx%dx%d precision %d does not match the handle (%dx%dx%d precision %d)",
-                    hd.nx, hd.ny, hd.nz, hd.precision, h->g.nx, h->g.ny, h->g.nz, h->p.precision));
+    return fail(h, SMAISO_E_INVALID, fmt("%s: %s", path, why.c_str()));
   }
   HIPTRY(h, hipStreamSynchronize(h->stream));
   // from here on the handle's state is being overwritten: it counts as set
@@ -1892,20 +1737,6 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
   return SMAISO_OK;
 }
 
-int smaiso_read_psi_text(const char* const* paths, int32_t npaths, int64_t count, double* psi) {
-  if (!paths || npaths < 1 || count < 0 || !psi) return fail(nullptr, SMAISO_E_INVALID, "bad argument");
-  int64_t got = 0;
-  for (int32_t r = 0; r < npaths && got < count; ++r) {
-    if (!paths[r]) return fail(nullptr, SMAISO_E_INVALID, "path is NULL");
-    std::ifstream in(paths[r]);
-    if (!in.is_open()) return fail(nullptr, SMAISO_E_INVALID, fmt("cannot open %s", paths[r]));
-    double v;
-    while (got < count && (in >> v)) psi[got++] = v;  // S:739-748 reads with operator>>
-  }
-  if (got < count)
-    return fail(nullptr, SMAISO_E_INVALID, fmt("the files hold %lld values, %lld needed", (long long)got, (long long)count));
-  return SMAISO_OK;
-}
 
 int smaiso_timing_enable(smaiso_handle* h, int32_t enable) {
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");

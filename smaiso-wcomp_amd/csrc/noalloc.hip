@@ -6,7 +6,7 @@
 // ends, and runs the same pass kernels with the same table values, so its
 // results are bit-identical to smaiso_r2r3d's.
 #include "../../include/smaiso.h"
-#include "abi.h"
+#include "host_abi.h"
 #include "r2r.h"
 #include "stepper.h"
 

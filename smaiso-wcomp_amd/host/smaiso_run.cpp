@@ -10,7 +10,9 @@
 //   --rho0 R          isotropic density rho_0 (default 0.01, S:251)
 //   --rho-m R         stabilisation density (default rho_s/2, S:255)
 //   --divv-switch K   relaxation steps (default 50, S:214)
-//   --steps K         run K steps (default: until L1 < 1e-6, S:1081)
+//   --steps K         run K steps (default: until L1 < the stop threshold, S:1081)
+//   --lim-l1 X        stop threshold of L1 when --steps is absent (default
+//                     1e-6, the reference's limL1, S:1029)
 //   --ic reference|interface   IC: the reference's layers (S:571-590) or the
 //                     tanh interface of smaiso_make_interface_ic (default)
 //   --load DIR [R]    the reference's load mode (argv[4] = 1, S:727-748):
@@ -28,6 +30,7 @@
 // Values are written with the reference's ofstream formatting (6 significant
 // digits), one per line, in its array orders.
 #include "smaiso.h"
+#include "run_args.h"
 
 #include <chrono>
 #include <cmath>
@@ -41,7 +44,7 @@
 static void usage() {
   std::fprintf(stderr,
                "usage: smaiso_run nu ep_arg [--n NX NY NZ] [--dt DT] [--rho0 R] [--rho-m R]\n"
-               "                  [--divv-switch K] [--steps K] [--ic reference|interface]\n"
+               "                  [--divv-switch K] [--steps K] [--lim-l1 X] [--ic reference|interface]\n"
                "                  [--load DIR [R]] [--restart FILE] [--checkpoint FILE]\n"
                "                  [--mixed] [--out DIR] [--save-every K]\n");
 }
@@ -61,44 +64,20 @@ static void append(const std::string& path, const std::vector<double>& v) {
 }
 
 int main(int argc, char** argv) {
-  if (argc < 3) { usage(); return 2; }
-  smaiso_params p;
-  smaiso_params_default(&p);
-  p.nx = p.ny = p.nz = 64;
-  p.nu = std::atof(argv[1]);
-  p.epsilon = -0.001 * std::atof(argv[2]);
-  long long steps = -1;
-  int save_every = 500, load_ranks = 1;
-  std::string ic = "interface", out, load_dir, restart, ckpt;
-  for (int a = 3; a < argc; ++a) {
-    std::string s = argv[a];
-    auto need = [&](int k) { if (a + k >= argc) { usage(); std::exit(2); } };
-    if (s == "--n") { need(3); p.nx = std::atoi(argv[++a]); p.ny = std::atoi(argv[++a]); p.nz = std::atoi(argv[++a]); }
-    else if (s == "--dt") { need(1); p.dt = std::atof(argv[++a]); }
-    else if (s == "--rho0") { need(1); p.rho0 = std::atof(argv[++a]); }
-    else if (s == "--rho-m") { need(1); p.rho_m = std::atof(argv[++a]); }
-    else if (s == "--divv-switch") { need(1); p.divv_switch = std::atoi(argv[++a]); }
-    else if (s == "--steps") { need(1); steps = std::atoll(argv[++a]); }
-    else if (s == "--ic") { need(1); ic = argv[++a]; }
-    else if (s == "--load") {
-      need(1);
-      load_dir = argv[++a];
-      if (a + 1 < argc && argv[a + 1][0] != '-') load_ranks = std::atoi(argv[++a]);
-    }
-    else if (s == "--restart") { need(1); restart = argv[++a]; }
-    else if (s == "--checkpoint") { need(1); ckpt = argv[++a]; }
-    else if (s == "--mixed") { p.precision = SMAISO_FP32_MIXED; }
-    else if (s == "--out") { need(1); out = argv[++a]; }
-    else if (s == "--save-every") { need(1); save_every = std::atoi(argv[++a]); }
-    else { usage(); return 2; }
-  }
-  const int step_l1 = 50;      // S:206
-  const double lim_l1 = 1e-6;  // S:1029
-  if (save_every < step_l1 || save_every % step_l1 != 0) {
-    std::fprintf(stderr, "--save-every must be a positive multiple of %d\n", step_l1);
+  RunArgs args;
+  std::string err;
+  if (parse_run_args(argc, argv, &args, &err) != 0) {
+    std::fprintf(stderr, "smaiso_run: %s\n", err.c_str());
+    usage();
     return 2;
   }
-  if (!load_dir.empty()) p.load_mode = 1;
+  smaiso_params& p = args.p;
+  const long long steps = args.steps;
+  const double lim_l1 = args.lim_l1;
+  const int save_every = args.save_every, load_ranks = args.load_ranks;
+  const std::string &ic = args.ic, &out = args.out, &load_dir = args.load_dir, &restart = args.restart,
+                    &ckpt = args.ckpt;
+  const int step_l1 = 50;  // S:206
   smaiso_handle* h = nullptr;
   if (!check(smaiso_create(&p, &h), "smaiso_create", nullptr)) return 1;
 

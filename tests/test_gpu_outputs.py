@@ -195,3 +195,40 @@ def test_driver_writes_the_reference_output_set(tmp_path):
     stdout = _run(common + ["--load", out, "--steps", 50, "--out", out4], tmp_path)
     assert "step 50" in stdout
     assert np.isfinite(_lines(out4 / "L1.dat")).all()
+
+
+def test_driver_runs_until_stationary_reference_ic(tmp_path):
+    """The run-until-stationary loop (S:1081, S:2246) without --steps, from the
+    reference's own IC (Amp cos(q0 z) layers, rho = 0, S:571-590) with the
+    relaxation branch for the first divvSwitch = 50 steps (S:206-214,
+    S:1312-1380): the driver checks L1 every 50 steps and stops at the first
+    check where L1 <= the threshold.  Its L1.dat equals the library's
+    diagnostics of the same run, and it stops where they say it must."""
+    n = (16, 16, 32)
+    p = S.Params(nx=n[0], ny=n[1], nz=n[2], dt=0.001, rho0=0.01, divv_switch=50, nu=1.0, epsilon=-0.005)
+    op = O.Params(nx=n[0], ny=n[1], nz=n[2], dt=0.001, rho0=0.01, divv_switch=50)
+    psi, rho = O.reference_layers_ic(op)
+    st = S.Stepper(p)
+    st.set_state(psi, rho)
+    trace = []
+    for _ in range(12):
+        st.step(50)
+        trace.append(st.diagnostics()[0])
+    st.close()
+    # a threshold the trace crosses: the 7th check's L1; the run must stop at
+    # the first check at or below it
+    lim = trace[6]
+    stop = next(i for i, v in enumerate(trace) if v <= lim)
+    assert stop >= 1, trace
+    out = tmp_path / "run"
+    out.mkdir()
+    stdout = _run(["1", "5", "--n", *n, "--dt", "0.001", "--rho0", "0.01", "--divv-switch", "50",
+                   "--ic", "reference", "--lim-l1", repr(lim), "--out", out], tmp_path)
+    l1 = _lines(out / "L1.dat")
+    assert len(l1) == stop + 1, (len(l1), stop, trace)
+    assert f"{50 * (stop + 1)} steps in" in stdout
+    assert np.allclose(l1, trace[: stop + 1], rtol=1e-5, atol=0), (l1, trace)
+    assert l1[-1] <= lim * (1 + 1e-5) and all(v > lim for v in trace[:stop])
+    # the default threshold is the reference's 1e-6 (S:1029); a bad one is refused
+    r = subprocess.run([S.RUN_PATH, "1", "5", "--lim-l1", "-1"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2 and "lim-l1" in r.stderr
