@@ -2,8 +2,9 @@
 
 Workload: the 256^3 FP64 plain stepper on the perturbed-interface initial
 state (interface height Lz/2 + 0.5 cos(2 pi x/Lx) cos(2 pi y/Ly), density
-1.0/0.8, 1e-3 seeded noise, dt = 0.01), with the diagnostics every 50 steps
-as the reference's loop does.  The FP32-mixed stepper on the same state is
+1.0/0.8, 1e-3 seeded noise, dt = 0.01), with the diagnostics (L1, mass,
+free energy) every 50 steps as the reference's loop logs them and at the
+end of the timed region, so every timed run holds at least one.  The FP32-mixed stepper on the same state is
 timed beside it.  A "step" is one iteration of the reference's loop body
 (smaiso-wcomp.cpp:1081-2198).
 
@@ -67,20 +68,25 @@ def time_stepper(S, params, psi, rho, steps, warmup, cadence=50, graph=True, tun
     t0 = time.perf_counter()
     done = 0
     diag = None
+    ndiag = 0
     while done < steps:
         chunk = min(cadence, steps - done)
         st.step(chunk)
         done += chunk
-        if chunk == cadence:
-            diag = st.diagnostics(allow_nonfinite=True)
+        # every 50 steps (the reference's stepL1) and after the last chunk
+        diag = st.diagnostics(allow_nonfinite=True)
+        ndiag += 1
     S.device_synchronize()
     t1 = time.perf_counter()
     st.tuned = tuned
+    st.ndiag = ndiag
     return st, (t1 - t0), diag
 
 
 def kernel_timing(st, nsteps):
-    """Per-kernel-class HIP-event timing over nsteps (events on the stepper's stream)."""
+    """Per-kernel-class HIP-event timing over nsteps (events on the stepper's
+    stream).  The y and x passes form one class, r2r_yx: they are the same
+    kernels on the two strided axes."""
     st.timing(True)
     st.step(nsteps)
     res = {}
@@ -88,8 +94,37 @@ def kernel_timing(st, nsteps):
         ms, n, by = st.timing_get(kc)
         res[kc] = dict(total_ms=ms, launches=n, bytes=by)
     st.timing(False)
+    y, x = res.pop("r2r_y"), res.pop("r2r_x")
+    res["r2r_yx"] = {k: y[k] + x[k] for k in y}
     return res
 
+
+def class_table(kt, nsteps):
+    """avg launch, launches per step, achieved GB/s and fraction of the HBM
+    peak per kernel class."""
+    out = {}
+    for k, v in kt.items():
+        avg = v["total_ms"] / max(v["launches"], 1)
+        gbs = v["bytes"] / max(v["total_ms"], 1e-12) / 1e6
+        out[k] = dict(avg_ms=avg, launches_per_step=v["launches"] / nsteps, ms_per_step=v["total_ms"] / nsteps,
+                      gbs=gbs, frac=gbs / HBM_PEAK_GBS)
+    return out
+
+
+def copy_peak(nbytes=1 << 30, reps=10):
+    """The HBM rate of a plain copy kernel (smaiso_hbm_copy_rate: four 16-byte
+    elements per lane over 1 GiB, 10 launches timed with HIP events),
+    reported beside the 8 TB/s specification.  The buffers are well above the
+    256 MB memory-side cache, so the figure is DRAM's."""
+    import ctypes as C
+    import smaiso_amd as S
+    L = S.lib()
+    L.smaiso_hbm_copy_rate.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    g = C.c_double()
+    if L.smaiso_hbm_copy_rate(nbytes, reps, C.byref(g)) != 0:
+        return None
+    return dict(gbs=g.value, frac_of_spec=g.value / HBM_PEAK_GBS, bytes_per_copy=2 * nbytes,
+                method="copy kernel (four 16-byte elements per lane) over 1 GiB, 10 launches timed with HIP events")
 
 def mixed_accuracy(S, n, psi, rho, checkpoints=(100, 500, 2000), params=None, label=""):
     """FP32-mixed vs FP64 on the same state: relative L2 errors of psi, rho and
@@ -342,6 +377,7 @@ def main():
     st64, sec64, diag64 = time_stepper(S, p64, psi, rho, args.steps, args.warmup, graph=not args.no_graph,
                                        tune=not args.no_autotune)
     tuned64 = st64.tuned
+    st64_ndiag = st64.ndiag
     sps64 = args.steps / sec64
     log(f"[bench] FP64 {sps64:.2f} steps/s, diag {diag64}")
     kt64 = kernel_timing(st64, args.profile_steps)
@@ -360,10 +396,8 @@ def main():
         log(f"[bench] mixed {spsm:.2f} steps/s, diag {diagm}")
         mixed = dict(value=spsm, ms_per_step=1e3 / spsm, speedup_vs_fp64=spsm / sps64,
                      step_roofline_frac=BYTES_PER_POINT["mixed"] * n ** 3 * spsm / (HBM_PEAK_GBS * 1e9),
-                     kernels={k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
-                                      launches_per_step=v["launches"] / args.profile_steps,
-                                      gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
-                              for k, v in ktm.items()})
+                     fp32_stages=S.fp32_stages_default(),
+                     kernels=class_table(ktm, args.profile_steps))
 
     if mixed is not None and args.accuracy_steps:
         cps = tuple(int(x) for x in args.accuracy_steps.split(","))
@@ -373,31 +407,39 @@ def main():
             log(f"[bench] flat-interface accuracy on {shapes}")
             mixed["rel_l2_vs_fp64_flat"] = flat_interface_accuracy(S, shapes, cps)
 
-    # dominant kernel class of the FP64 stepper
-    dom = max(kt64, key=lambda k: kt64[k]["total_ms"])
-    d = kt64[dom]
-    avg_ms = d["total_ms"] / max(d["launches"], 1)
-    bytes_per_launch = d["bytes"] / max(d["launches"], 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic, traffic_src = None, None
+    # the dominant kernel class of the FP64 stepper (most time per step) and
+    # the one-axis operator passes beside it
+    tr = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic_256cubed_fp64.json")
     if n == 256 and os.path.exists(tpath):
-        # HBM bytes per launch of this kernel class from the committed
+        # HBM bytes per launch of each kernel class from the committed
         # rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this bench (copy-calibrated)
         with open(tpath) as f:
             tr = json.load(f)
-        cls = tr.get(dom) or tr.get("r2r_yx" if dom in ("r2r_y", "r2r_x") else dom)
-        if cls:
-            traffic = cls["read_bytes"] + cls["write_bytes"]
-            traffic_src = tr.get("source")
-    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=traffic, kernel=dom,
-                    avg_launch_ms=avg_ms, algorithmic_bytes_per_launch=bytes_per_launch,
-                    traffic_source=traffic_src)
+    copy = copy_peak()
+
+    def class_roofline(kc):
+        d = kt64[kc]
+        avg_ms = d["total_ms"] / max(d["launches"], 1)
+        bytes_per_launch = d["bytes"] / max(d["launches"], 1)
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        cls = tr.get(kc) if tr else None
+        return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS,
+                    traffic=(cls["read_bytes"] + cls["write_bytes"]) if cls else None, kernel=kc,
+                    avg_launch_ms=avg_ms, launches_per_step=d["launches"] / args.profile_steps,
+                    algorithmic_bytes_per_launch=bytes_per_launch,
+                    frac_of_copy_peak=achieved / copy["gbs"] if copy else None,
+                    traffic_source=tr.get("source") if (tr and cls) else None)
+
+    dom = max(kt64, key=lambda k: kt64[k]["total_ms"])
+    roofline = class_roofline(dom)
+    roofline_axis = class_roofline("r2r_axis") if dom != "r2r_axis" else None
     step_bytes = BYTES_PER_POINT["f64"] * n ** 3
     step_roof = dict(algorithmic_bytes_per_step=step_bytes,
                      achieved=step_bytes * sps64 / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                     frac=step_bytes * sps64 / (HBM_PEAK_GBS * 1e9))
+                     frac=step_bytes * sps64 / (HBM_PEAK_GBS * 1e9),
+                     frac_of_copy_peak=step_bytes * sps64 / 1e9 / copy["gbs"] if copy else None)
 
     ens = None
     if args.ensemble > 0:
@@ -430,15 +472,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic: seeded perturbed-interface initial state built on the host",
         "config": {"workload": f"plain FP64 stepper, {n}^3 grid, interface h=Lz/2+0.5cos(2pi x/Lx)cos(2pi y/Ly), "
-                               f"rho 1.0/0.8, dt=0.01, rho_m=1, diagnostics every 50 steps",
+                               f"rho 1.0/0.8, dt=0.01, rho_m=1, diagnostics every 50 steps and at the end of "
+                               f"the timed region ({st64_ndiag} in it)",
                    "grid": [n, n, n], "precision": "fp64", "dt": 0.01, "rho_m": 1.0,
                    "hipgraph": not args.no_graph, "autotuned_tiles": tuned64},
         "roofline": roofline,
+        "roofline_r2r_axis": roofline_axis,
+        "hbm_copy_peak": copy,
         "step_roofline": step_roof,
-        "kernels_fp64": {k: dict(avg_ms=v["total_ms"] / max(v["launches"], 1),
-                                 launches_per_step=v["launches"] / args.profile_steps,
-                                 gbs=v["bytes"] / max(v["total_ms"], 1e-12) / 1e6)
-                         for k, v in kt64.items()},
+        "kernels_fp64": class_table(kt64, args.profile_steps),
         "fp32_mixed": mixed,
         "ensemble": ens,
         "adjoint": adj,

@@ -248,6 +248,13 @@ int smaiso_r2r3d_device(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t 
 int smaiso_r2r3d_noalloc(const int32_t kinds[3], int32_t nx, int32_t ny, int32_t nz,
                          const double* in, double* out, int32_t dtype);
 
+/* No reference counterpart: the HBM rate (GB/s, bytes read plus bytes
+ * written per second) a plain copy kernel reaches on the current device,
+ * copying `bytes` (a multiple of 16, at least 1 MiB) `reps` times between
+ * two buffers it allocates and frees; the measured reference beside the
+ * 8 TB/s specification.  Synchronous. */
+int smaiso_hbm_copy_rate(int64_t bytes, int32_t reps, double* gbs);
+
 /* Builder-defined initial condition (the reference's own IC is S:571-590):
  *   profile = 1/2 (1 - tanh((z - h(x,y))/width)),  z = k dz, x = i dx, y = j dy
  *   h(x,y)  = h0 + undulation cos(2 pi x/Lx) cos(2 pi y/Ly)  (or height[i*ny+j])

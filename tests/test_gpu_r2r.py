@@ -125,3 +125,16 @@ def test_tile_variants(variant, dtype, shape):
             continue
         mx, _ = rel_err(b.get().astype(np.float64), O.r2r3d(xin.astype(np.float64), kinds))
         assert mx < tol, (kinds, mx)
+
+
+def test_hbm_copy_rate():
+    """smaiso_hbm_copy_rate: a plain copy kernel's HBM rate, and its argument
+    checks."""
+    import ctypes as C
+    L = S.lib()
+    L.smaiso_hbm_copy_rate.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    g = C.c_double()
+    assert L.smaiso_hbm_copy_rate(256 << 20, 3, C.byref(g)) == 0
+    assert 1000.0 < g.value < 20000.0, g.value
+    assert L.smaiso_hbm_copy_rate(1000, 3, C.byref(g)) == S.E_INVALID
+    assert L.smaiso_hbm_copy_rate(256 << 20, 0, C.byref(g)) == S.E_INVALID
