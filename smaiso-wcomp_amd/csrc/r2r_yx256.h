@@ -93,11 +93,22 @@ constexpr bool ax_op() {
   return OP == OP_AX_GRAD || OP == OP_AX_DIV || OP == OP_AX_GAUSS || OP == OP_AX_GRAD_T || OP == OP_AX_DIV_T;
 }
 
+// Register cap, as waves per SIMD, for FP64 storage.  The FP64 tile's LDS
+// (8 sequences, 35 KiB, 128 threads) allows four workgroups per CU, two
+// waves per SIMD; the forward passes fit that without spilling, the inverse
+// and operator passes (one more LDS exchange, or two transforms) keep the
+// full register file.  FP32 tiles (16 sequences, 256 threads) measured
+// fastest with the full register file and the tables held across tiles.
+template <typename T, int MODE>
+constexpr int min_waves() {
+  return sizeof(T) == 8 && MODE == kFwd ? 2 : 1;
+}
+
 // AXIS: 1 = y pass, 0 = x pass.  P sequences (2P columns) per tile.  A:
 // the arithmetic type of the tile and the tables (FP64, or FP32 for FP32
 // storage where FP32 arithmetic is asked for).
 template <typename T, typename TO, int AXIS, int MODE, int OP, int P, typename A = double>
-__global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
+__global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Args<T, TO> a) {
   using C = cplx<A>;
   constexpr int N = kN;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
   };
   // first inverse butterfly on z_{j+16q}, LDS exchange, second butterfly,
   // store y_{2n} = u_n, y_{2n+1} = u_{N-1-n} (times -1 for DST-III)
-  auto inverse_tail = [&](C* v, TO* o) {
+  auto inverse_tail = [&](C* v, TO* o, C wi_t, int j_t) {
     dft<A, 16, +1>(v);
     __syncthreads();  // every read of this tile's slots is done
 #pragma unroll
@@ -160,11 +171,11 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
-    stage_twiddle(v, wi);
+    stage_twiddle(v, wi_t);
     dft<A, 16, +1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int n = j + 16 * q;
+      const int n = j_t + 16 * q;
       if (q < 8) st2(o + (2 * n) * stride, v[q].x, v[q].y, acc);
       else if (sine_out) st2(o + (2 * N - 1 - 2 * n) * stride, -v[q].x, -v[q].y, acc);
       else st2(o + (2 * N - 1 - 2 * n) * stride, v[q].x, v[q].y, acc);
@@ -181,6 +192,17 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
   for (; t < a.ntiles; t += gridDim.x) {
     const int tn = t + gridDim.x;
     if (kPrefetch && tn < a.ntiles) load(nxt, tn);
+    // The twiddle powers, Makhoul factors and operator weights of a thread
+    // are the same for every tile; held across the loop in FP64 they would
+    // take about half the register file.  Passing their seeds through an
+    // empty asm statement each tile makes them values the compiler cannot
+    // hoist, so they are recomputed per tile (a few dozen ALU operations).
+    C wf_t = wf, wi_t = wi, mk0_t = mk0, mks_t = mks;
+    int j_t = j;
+    if constexpr (sizeof(T) == 8) {
+      asm volatile("" : "+v"(j_t), "+v"(wf_t.x), "+v"(wf_t.y), "+v"(wi_t.x), "+v"(wi_t.y));
+      asm volatile("" : "+v"(mk0_t.x), "+v"(mk0_t.y), "+v"(mks_t.x), "+v"(mks_t.y));
+    }
     int outer, k0;
     TO* const o = a.out + tile(t, outer, k0) + 2 * c;
     if constexpr (MODE == kInv) {
@@ -188,16 +210,18 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
       __syncthreads();
-      C m = mk0;
+      C m = mk0_t;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int k = j + 16 * q;
-        const C rk = v[q];
+        const int k = j_t + 16 * q;
+        // FP64: both from LDS, so the loaded values need no registers past
+        // the barrier
+        const C rk = sizeof(T) == 8 ? s[slot(k)] : v[q];
         const C rn = k == 0 ? C{A(0), A(0)} : s[slot((N - k) & (N - 1))];
         v[q] = cmul(C{m.x, -m.y}, C{rk.x + rn.y, rk.y - rn.x});
-        m = cmul(m, mks);
+        m = cmul(m, mks_t);
       }
-      inverse_tail(v, o);
+      inverse_tail(v, o, wi_t, j_t);
     } else {
       if (sine_in) {
 #pragma unroll
@@ -209,23 +233,23 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
-      stage_twiddle(v, wf);
+      stage_twiddle(v, wf_t);
       dft<A, 16, -1>(v);
       // v[q] = Z[j + 16 q]; the slots written are the ones this thread read
 #pragma unroll
       for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
       __syncthreads();
-      C m = mk0;
+      C m = mk0_t;
       if constexpr (MODE == kFwd) {
         // Y_k = (Re(mk[k] (Z_k + conj Z_{N-k})), Im(mk[k] (Z_k - conj Z_{N-k})))
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          const int k = j + 16 * q;
+          const int k = j_t + 16 * q;
           const C za = v[q], pa = s[slot((N - k) & (N - 1))];
           const A y0 = m.x * (za.x + pa.x) - m.y * (za.y - pa.y);
           const A y1 = m.x * (za.y + pa.y) + m.y * (za.x - pa.x);
           st2(o + (sine_out ? N - 1 - k : k) * stride, y0, y1, false);
-          m = cmul(m, mks);
+          m = cmul(m, mks_t);
         }
       } else {
         // y and z parts of |q|^2 of the pair's two columns (mid op)
@@ -237,9 +261,9 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          const int k = j + 16 * q;
+          const int k = j_t + 16 * q;
           const int kb = (N - k) & (N - 1);
-          const C za = v[q], zb = s[slot(kb)];
+          const C za = sizeof(T) == 8 ? s[slot(k)] : v[q], zb = s[slot(kb)];
           const C mb = k == 0 ? m : C{-m.y, -m.x};  // mk[N - k], exact
           // forward outputs of both lines at positions k and kb
           const C ya = {m.x * (za.x + zb.x) - m.y * (za.y - zb.y), m.x * (za.y + zb.y) + m.y * (za.x - zb.x)};
@@ -271,10 +295,10 @@ __global__ __launch_bounds__(16 * P, 1) void pass_kernel(Args<T, TO> a) {
           }
           const C rn = k == 0 ? C{A(0), A(0)} : xb;
           v[q] = cmul(C{m.x, -m.y}, C{xk.x + rn.y, xk.y - rn.x});
-          m = cmul(m, mks);
+          m = cmul(m, mks_t);
           if ((q & 1) == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the registers in flight
         }
-        inverse_tail(v, o);
+        inverse_tail(v, o, wi_t, j_t);
       }
     }
     __syncthreads();  // the next tile's first writes reuse slots other threads read last
