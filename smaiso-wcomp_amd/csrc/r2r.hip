@@ -67,32 +67,32 @@ static hipError_t launch_fft(const PassIO& io, void* tw, void* mk, int kind, int
   constexpr int PZ = C::PZ, NTZ = C::NTZ, PS = C::PS, NTS = C::NTS;
   if (axis == 2) {
     switch (io.f.op) {
-      case OP_LD_SH: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_SH, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_RELAXLAP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_RELAXLAP, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_PROD: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PROD, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_DIV: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_DIV, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_PLAIN: return launch_fft_pn<T, N, PZ, NTZ, OP_PLAIN, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_SH: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_SH, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_RELAXLAP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_RELAXLAP, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_PROD: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PROD, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_LD_DIV: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_DIV, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_PLAIN: return launch_fft_pn<T, N, PZ, NTZ, OP_PLAIN, T, A, 2>(io, tw, mk, kind, axis, g, s);
       default: break;
     }
     // load ops the step runs in FP64 arithmetic only
     if constexpr (std::is_same<A, double>::value) {
       switch (io.f.op) {
-        case OP_LD_GRAD_D: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_D>(io, tw, mk, kind, axis, g, s);
-        case OP_LD_GRAD_T: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_T>(io, tw, mk, kind, axis, g, s);
-        case OP_LD_AMP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_AMP>(io, tw, mk, kind, axis, g, s);
-        case OP_LD_PSI: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PSI>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_GRAD_D: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_D, T, double, 2>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_GRAD_T: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_GRAD_T, T, double, 2>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_AMP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_AMP, T, double, 2>(io, tw, mk, kind, axis, g, s);
+        case OP_LD_PSI: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PSI, T, double, 2>(io, tw, mk, kind, axis, g, s);
         default: break;
       }
     }
     return hipErrorInvalidValue;
   }
   switch (io.f.op) {
-    case OP_MID_NEGLAP: return launch_fft_pn<T, N, PS, NTS, OP_MID_NEGLAP, T, A>(io, tw, mk, kind, axis, g, s);
-    case OP_PLAIN: return launch_fft_pn<T, N, PS, NTS, OP_PLAIN, T, A>(io, tw, mk, kind, axis, g, s);
+    case OP_MID_NEGLAP: return launch_fft_pn<T, N, PS, NTS, OP_MID_NEGLAP, T, A, 3>(io, tw, mk, kind, axis, g, s);
+    case OP_PLAIN: return launch_fft_pn<T, N, PS, NTS, OP_PLAIN, T, A, 3>(io, tw, mk, kind, axis, g, s);
     default: break;
   }
   if constexpr (std::is_same<A, double>::value) {
-    if (io.f.op == OP_MID_AMP) return launch_fft_pn<T, N, PS, NTS, OP_MID_AMP>(io, tw, mk, kind, axis, g, s);
+    if (io.f.op == OP_MID_AMP) return launch_fft_pn<T, N, PS, NTS, OP_MID_AMP, T, double, 3>(io, tw, mk, kind, axis, g, s);
   }
   return hipErrorInvalidValue;
 }

@@ -15,21 +15,21 @@ hipError_t launch_ax(const PassIO& io, void* tw, void* mk, int axis, const Geom&
   const int kind = REDFT10;  // unused by the one-axis operators
   if (axis == 2) {
     switch (io.f.op) {
-      case OP_AX_GRAD: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_AX_DIV: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_AX_GAUSS_AMP: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS_AMP, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD_T, T, A>(io, tw, mk, kind, axis, g, s);
-      case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV_T, T, A>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_GRAD: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_DIV: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_GAUSS_AMP: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GAUSS_AMP, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD_T, T, A, 2>(io, tw, mk, kind, axis, g, s);
+      case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_DIV_T, T, A, 2>(io, tw, mk, kind, axis, g, s);
       default: return hipErrorInvalidValue;
     }
   }
   switch (io.f.op) {
-    case OP_AX_GRAD: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD, T, A>(io, tw, mk, kind, axis, g, s);
-    case OP_AX_DIV: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV, T, A>(io, tw, mk, kind, axis, g, s);
-    case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GAUSS, T, A>(io, tw, mk, kind, axis, g, s);
-    case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD_T, T, A>(io, tw, mk, kind, axis, g, s);
-    case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV_T, T, A>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_GRAD: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD, T, A, 3>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_DIV: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV, T, A, 3>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_GAUSS: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GAUSS, T, A, 3>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_GRAD_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD_T, T, A, 3>(io, tw, mk, kind, axis, g, s);
+    case OP_AX_DIV_T: return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_DIV_T, T, A, 3>(io, tw, mk, kind, axis, g, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -15,11 +15,11 @@ hipError_t launch_cast_n(const PassIO& io, void* tw, void* mk, int kind, int axi
                          hipStream_t s) {
   using C = TileChoice<T, N>;
   if (axis == 2) {
-    if (io.f.op == OP_LD_PSI) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_LD_PSI, TO>(io, tw, mk, kind, axis, g, s);
-    if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_PLAIN, TO>(io, tw, mk, kind, axis, g, s);
+    if (io.f.op == OP_LD_PSI) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_LD_PSI, TO, double, 2>(io, tw, mk, kind, axis, g, s);
+    if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_PLAIN, TO, double, 2>(io, tw, mk, kind, axis, g, s);
     return hipErrorInvalidValue;
   }
-  if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PS, C::NTS, OP_PLAIN, TO>(io, tw, mk, kind, axis, g, s);
+  if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PS, C::NTS, OP_PLAIN, TO, double, 3>(io, tw, mk, kind, axis, g, s);
   return hipErrorInvalidValue;
 }
 

@@ -676,8 +676,11 @@ constexpr int pass_min_waves(int nt, int op, int per_thread) {
 // One axis pass over a tile of TL = 2P lines; lines 2c and 2c+1 of the tile
 // form complex sequence c.  OP selects a fused operation (FuseOp); the
 // output is stored as TO.
-template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double>
+template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double, int AXC = -1>
 __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void fft_pass_kernel(PassArgs<T, TO> a) {
+  // the pass axis: known at compile time for z passes (AXC = 2); y or x
+  // (AXC = 3) is taken as a.axis & 1, so every test for the z axis folds
+  const int axis = AXC == 2 ? 2 : AXC == 3 ? (a.axis & 1) : a.axis;
   constexpr int TL = 2 * P;
   constexpr int MAXR = pick_maxr(N, P, NT);
   constexpr int SEQ = seq_len(N, MAXR);
@@ -697,7 +700,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   const int tid = threadIdx.x;
   const cplx<A>* const tw = reinterpret_cast<const cplx<A>*>(a.tw);
   const cplx<A>* const mk = reinterpret_cast<const cplx<A>*>(a.mk);
-  TileGeo g = tile_geometry<TL, N>(a.axis, a.nx, a.ny, a.nz);
+  TileGeo g = tile_geometry<TL, N>(axis, a.nx, a.ny, a.nz);
   const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);  // member offset
   g.base += mo;
   const bool fwd = MID || AX || (a.kind == REDFT10 || a.kind == RODFT10);
@@ -708,12 +711,12 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
   StageTw<A, N, P, NT, MAXR> stw;
   stw.load(tw, tid);
   // the Makhoul factors' first loads, also ahead of the tile's data
-  const MkSeq<A, N, P, NT> ms0(mk, a.axis, tid);
+  const MkSeq<A, N, P, NT> ms0(mk, axis, tid);
 
   if (fwd) {
     // ---- load: every global load of the tile is issued before the first LDS
     // write, so each thread keeps all of its loads in flight ---------------
-    if (a.axis == 2) {
+    if (axis == 2) {
       // item e = (line pair c, element m): lanes walk m along the two lines
       // of a pair, and each item stores one complex value to LDS
       constexpr int E = P * N;
@@ -782,12 +785,12 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
       rk[it] = rn[it] = cplx<A>{A(0), A(0)};
       if (W % NT == 0 || w < W) {
         int c, k;
-        if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+        if (axis == 2) { c = w / HN; k = w - c * HN; }
         else { k = w / P; c = w - k * P; }
         const int kb = (k == 0) ? HN : N - k;
         const int pa = sine ? N - 1 - k : k;
         const int pb = sine ? N - 1 - kb : kb;
-        if (a.axis == 2) {
+        if (axis == 2) {
           const int l0 = 2 * c;
           const long long g0 = (long long)g.outer + l0;
           int i, j;
@@ -814,15 +817,15 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
       const int w = tid + it * NT;
       if (W % NT == 0 || w < W) {
         int c, k;
-        if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+        if (axis == 2) { c = w / HN; k = w - c * HN; }
         else { k = w / P; c = w - k * P; }
         const int kb = (k == 0) ? HN : N - k;
         cplx<A> zk, zb;
-        makhoul_pre<A, N>(mk, k, ms.get(mk, a.axis, k), rk[it], rn[it], zk, zb);
+        makhoul_pre<A, N>(mk, k, ms.get(mk, axis, k), rk[it], rn[it], zk, zb);
         buf[c * SEQ + lp<SH>(k)] = zk;
         buf[c * SEQ + lp<SH>(kb)] = zb;
       }
-      ms.advance(a.axis, it);
+      ms.advance(axis, it);
     }
   }
   __syncthreads();
@@ -840,10 +843,10 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
         const int w = tid + it * NT;
         if (W % NT == 0 || w < W) {
           int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          if (axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          const cplx<A> mkk = ms.get(mk, a.axis, k);
+          const cplx<A> mkk = ms.get(mk, axis, k);
           A y0a, y1a, y0b, y1b;
           makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           // x pass (axis 0): mode index along x, column j = outer, k = k0 + line
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
-        ms.advance(a.axis, it);
+        ms.advance(axis, it);
       }
       __syncthreads();
     } else if constexpr (AX) {
@@ -875,10 +878,10 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
         const int w = tid + it * NT;
         if (W % NT == 0 || w < W) {
           int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          if (axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
-          const cplx<A> mkk = ms.get(mk, a.axis, k);
+          const cplx<A> mkk = ms.get(mk, axis, k);
           A y0a, y1a, y0b, y1b;
           makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
           const int fa = sine ? N - 1 - k : k;
@@ -901,7 +904,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
           buf[c * SEQ + lp<SH>(k)] = zk;
           buf[c * SEQ + lp<SH>(kb)] = zb;
         }
-        ms.advance(a.axis, it);
+        ms.advance(axis, it);
       }
       __syncthreads();
     } else {
@@ -912,16 +915,16 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
         const int w = tid + it * NT;
         if (W % NT == 0 || w < W) {
           int c, k;
-          if (a.axis == 2) { c = w / HN; k = w - c * HN; }
+          if (axis == 2) { c = w / HN; k = w - c * HN; }
           else { k = w / P; c = w - k * P; }
           const int kb = (k == 0) ? HN : N - k;
           A y0a, y1a, y0b, y1b;
-          makhoul_post<A, N>(mk, k, kb, ms.get(mk, a.axis, k), buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)],
+          makhoul_post<A, N>(mk, k, kb, ms.get(mk, axis, k), buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)],
                              y0a, y1a, y0b, y1b);
           const int oa = sine ? (N - 1 - k) : k;
           const int ob = sine ? (N - 1 - kb) : kb;
           const int l0 = 2 * c;
-          if (a.axis == 2) {
+          if (axis == 2) {
             if (l0 < g.nvalid) {
               TO* o = a.out + g.base + (long long)l0 * N;
               o[oa] = (TO)y0a;
@@ -937,7 +940,7 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
             store_pair(a, g, l0, ob, y0b, y1b);
           }
         }
-        ms.advance(a.axis, it);
+        ms.advance(axis, it);
       }
       return;
     }
@@ -954,12 +957,12 @@ __global__ __launch_bounds__(NT, pass_min_waves(NT, OP, 2 * P * N / NT)) void ff
     const int w = tid + it * NT;
     if (WO % NT == 0 || w < WO) {
       int c, n;
-      if (a.axis == 2) { c = w / HN; n = w - c * HN; }
+      if (axis == 2) { c = w / HN; n = w - c * HN; }
       else { n = w / P; c = w - n * P; }
       const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
       const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
       const int l0 = 2 * c;
-      if (a.axis == 2) {
+      if (axis == 2) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (l0 + q < g.nvalid) {
@@ -1153,16 +1156,17 @@ PassArgs<T, TO> make_args(const PassIO& io, void* tw, void* mk, int kind, int ax
   return a;
 }
 
-template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double>
+template <typename T, int N, int P, int NT, int OP, typename TO = T, typename A = double, int AXC = -1>
 hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
                          hipStream_t s) {
   constexpr int TL = 2 * P;
+  if ((AXC == 2 && axis != 2) || (AXC == 3 && axis != 0 && axis != 1)) return hipErrorInvalidValue;
   PassArgs<T, TO> a = make_args<T, TO>(io, tw, mk, kind, axis, g);
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + TL - 1) / TL;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + TL - 1) / TL);
   const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<A>);
-  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP, TO, A>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
+  hipLaunchKernelGGL((fft_pass_kernel<T, N, P, NT, OP, TO, A, AXC>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(NT), lds, s, a);
   return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ hipError_t launch_tv(const PassIO& io, void* tw, void* mk, int kind, int axis, c
                      hipStream_t s) {
   using TV = TileVariant<T, Z, V>;
   if constexpr (tile_fits<T, N, TV::P, TV::NT>())
-    return launch_fft_pn<T, N, TV::P, TV::NT, OP_PLAIN>(io, tw, mk, kind, axis, g, s);
+    return launch_fft_pn<T, N, TV::P, TV::NT, OP_PLAIN, T, double, Z ? 2 : 3>(io, tw, mk, kind, axis, g, s);
   else
     return hipErrorNotSupported;
 }
