@@ -1192,12 +1192,11 @@ struct TileChoice {
   static constexpr int NTS = 256;
 };
 
-// A y or x pass on the 256-point kernels of r2r_yx256.h when they apply
-// (length 256 along a strided axis, full tiles of 16 FP64 or 32 FP32
-// columns, aligned pointers, element offsets within 32 bits, a plain or
-// one-axis or x-pass mid op); hipErrorNotSupported otherwise, with nothing
-// launched.  Storage in_dt -> out_dt: equal, or FP64 -> FP32 for a plain
-// pass.  Defined in r2r_yx256.hip.
+// A plain y or x pass on the 256-point kernels of r2r_yx256.h when they
+// apply (length 256 along a strided axis, full tiles of 16 FP64 or 32 FP32
+// columns, aligned pointers, element offsets within 32 bits, no fused op);
+// hipErrorNotSupported otherwise, with nothing launched.  Storage in_dt ->
+// out_dt: equal, or FP64 -> FP32.  Defined in r2r_yx256.hip.
 hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io, const Geom& g,
                         const void* tw, const void* mk, hipStream_t s, bool arith32 = false);
 

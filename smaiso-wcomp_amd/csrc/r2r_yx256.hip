@@ -10,7 +10,7 @@ namespace smaiso {
 
 namespace {
 
-template <typename T, typename TO, int AXIS, int MODE, int OP, typename A>
+template <typename T, typename TO, int AXIS, int MODE, typename A>
 hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, const void* mk, hipStream_t s) {
   constexpr int P = yx256::tile_p<T>();
   yx256::Args<T, TO> a;
@@ -22,31 +22,24 @@ hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, con
   a.ny = g.ny;
   a.nz = g.nz;
   a.kind = kind;
-  a.f = io.f;
   a.tpo = g.nz / (2 * P);
   a.ntiles = (AXIS == 1 ? g.nx : g.ny) * a.tpo;
   // two resident workgroups per CU for FP64 tiles (128 threads), one for
   // FP32 tiles (256 threads); twice that many keep every CU's queue full
   const int grid = std::min(a.ntiles, 256 * (P == 8 ? 2 : 1) * 2);
   const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<A>);
-  hipLaunchKernelGGL((yx256::pass_kernel<T, TO, AXIS, MODE, OP, P, A>), dim3((unsigned)grid, (unsigned)g.nb),
+  hipLaunchKernelGGL((yx256::pass_kernel<T, TO, AXIS, MODE, P, A>), dim3((unsigned)grid, (unsigned)g.nb),
                      dim3(16 * P), lds, s, a);
   return hipGetLastError();
 }
 
+// plain passes only: the one-axis and x-pass operators stay on the general
+// pass kernel
 template <typename T, typename TO, int AXIS, typename A>
 hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
-  switch (io.f.op) {
-    case OP_PLAIN:
-      if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, OP_PLAIN, A>(io, g, kind, tw, mk, s);
-      return launch<T, TO, AXIS, yx256::kInv, OP_PLAIN, A>(io, g, kind, tw, mk, s);
-    default:
-      break;
-  }
-  // The forward -> operator -> inverse passes (kFwdOp) measured slower than
-  // the general kernel's at 256^3 (their 32 complex values per thread spill
-  // even with the whole register file): those ops stay on the general kernel.
-  return hipErrorNotSupported;
+  if (io.f.op != OP_PLAIN) return hipErrorNotSupported;
+  if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, A>(io, g, kind, tw, mk, s);
+  return launch<T, TO, AXIS, yx256::kInv, A>(io, g, kind, tw, mk, s);
 }
 
 template <typename T, typename TO, typename A = double>
