@@ -42,11 +42,11 @@ extern "C" {
 #define SMAISO_FP64 0 /* every field, transform and kernel in FP64 */
 /* FP32 storage of the work fields (psi, grad psi, amp, L psi, mu, forces,
  * velocities, Nl, ...); FP64 state (psiq, psiq_old), rho, f_rho and grad rho;
- * FP64 arithmetic in every pointwise and per-mode kernel, the implicit
- * denominators, the density update and the compensated reductions; the
- * transform that forms psi from psiq keeps FP64 intermediates.  Transform
- * passes compute in FP64 (FP64 LDS tiles), except those of the stages
- * selected by smaiso_set_fp32_stages, which compute in FP32. */
+ * FP64 arithmetic in the forces and density kernels, the psi update, the
+ * implicit denominators and the compensated reductions; the transform that
+ * forms psi from psiq keeps FP64 intermediates.  Transform passes and the
+ * velocity solve compute in FP64 (FP64 LDS tiles), except those of the
+ * stages selected by smaiso_set_fp32_stages, which compute in FP32. */
 #define SMAISO_FP32_MIXED 1
 
 /* FFTW r2r kind numbering used by smaiso_r2r3d (S:499-545) */
@@ -166,12 +166,13 @@ int smaiso_synchronize(smaiso_handle* h);
 /* Steps completed since the last set_state (the reference's nLoop). */
 int64_t smaiso_step_count(const smaiso_handle* h);
 
-/* Stages of the SMAISO_FP32_MIXED step whose transform passes compute in
- * FP32 (FP32 LDS tiles and tables) rather than FP64.  The transform that
- * forms psi from psiq, the forces and density kernels, the velocity solve,
- * the psi update and the reductions always compute in FP64.  FP32
- * arithmetic rounds every mode of a line relative to the line's norm, so
- * it is kept to the stages where that rounding does not reach psi. */
+/* Stages of the SMAISO_FP32_MIXED step that compute in FP32 rather than
+ * FP64: the transform passes of a stage (FP32 LDS tiles and tables), or the
+ * velocity solve's arithmetic.  The transform that forms psi from psiq, the
+ * forces and density kernels, the psi update and the reductions always
+ * compute in FP64.  FP32 arithmetic in a transform rounds every mode of a
+ * line relative to the line's norm, so it is kept to the stages where that
+ * rounding does not reach psi. */
 #define SMAISO_STAGE_GRAD_PSI (1u << 0) /* grad psi, one pass per axis (S:1104-1236) */
 #define SMAISO_STAGE_AMP (1u << 1)      /* Gaussian amplitude filter (S:1254-1289) */
 #define SMAISO_STAGE_LPSI (1u << 2)     /* L psi (S:1293-1308) */
@@ -181,7 +182,8 @@ int64_t smaiso_step_count(const smaiso_handle* h);
 #define SMAISO_STAGE_DIVF (1u << 6)     /* div f, one pass per axis (S:2042-2092) */
 #define SMAISO_STAGE_VELOCITY (1u << 7) /* v's inverse transforms (S:2094-2117) */
 #define SMAISO_STAGE_NL (1u << 8)       /* y and x passes of Nl's transform (S:2166-2168) */
-#define SMAISO_STAGES_ALL 0x1ffu
+#define SMAISO_STAGE_VELSOLVE (1u << 9) /* the velocity solve's arithmetic (S:1479-1974) */
+#define SMAISO_STAGES_ALL 0x3ffu
 
 /* The FP32-arithmetic stages of a SMAISO_FP32_MIXED handle (SMAISO_STAGE_*
  * bits; the default is smaiso_fp32_stages_default()).  Ignored by FP64

@@ -52,12 +52,13 @@ struct PendingTiming {
 };
 
 // FP32-arithmetic stages of an FP32-mixed handle at creation: the stages
-// whose FP32 rounding leaves psi as close to the FP64 stepper as FP32
-// storage alone does (forces, div v, div f, v, Nl: psi within 3.4e-8 of
-// FP64 after 2000 steps at 256^3, as with none of them).  With FP32
-// arithmetic grad psi and lap(rho L psi) each reach 3e-5, L psi 1e-6.
+// whose FP32 rounding leaves psi about as close to the FP64 stepper as FP32
+// storage alone does (forces, div v, div f, v's transforms, Nl, the velocity
+// solve: psi within 4.8e-8 of FP64 after 2000 steps at 256^3, against
+// 3.3e-8 with none of them).  The amplitude filter in FP32 gives 2.9e-7 at
+// step 500; grad psi and lap(rho L psi) each reach 3e-5, L psi 1e-6.
 constexpr uint32_t kFp32StagesDefault = SMAISO_STAGE_FORCES | SMAISO_STAGE_DIVV | SMAISO_STAGE_DIVF |
-                                        SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL;
+                                        SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL | SMAISO_STAGE_VELSOLVE;
 
 double derived_amp(const smaiso_params& p) {
   return std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.epsilon * p.gamma)) / 5);
@@ -624,7 +625,7 @@ int run_step(smaiso_handle* h) {
   {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 11);
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
-                vtq[0], vtq[1], vtq[2], virrq[0], virrq[1], virrq[2], s);
+                vtq[0], vtq[1], vtq[2], virrq[0], virrq[1], virrq[2], h->a32(SMAISO_STAGE_VELSOLVE) != 0, s);
   }
   // a13 + a14 (S:1979-2117): div v is formed on load by the z pass of its
   // inverse transform; div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum

@@ -113,7 +113,9 @@ __global__ __launch_bounds__(kBlock) void k_forces(
 // ---------------------------------------------------------------------------
 // a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974)
 // ---------------------------------------------------------------------------
-template <typename T, int BS>
+// A: the arithmetic type (FP64; FP32 for FP32 storage where the velocity
+// solve's FP32 stage is asked for).
+template <typename T, typename A, int BS>
 __global__ __launch_bounds__(BS) void k_velocity(
     StepConsts c, const T* __restrict__ sxq_, const T* __restrict__ syq_,
     const T* __restrict__ szq_, const T* __restrict__ fgq_, const T* __restrict__ erq_,
@@ -147,20 +149,21 @@ __global__ __launch_bounds__(BS) void k_velocity(
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
   // wavenumbers from the spacings (no table loads: this kernel is bound by its
   // memory instructions)
-  const double qx = p.i * c.qs[0], qy = p.j * c.qs[1], qz = p.k * c.qs[2];
-  const double wy = (p.j + 1) * c.qs[1], wz = (p.k + 1) * c.qs[2];
-  const double qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
-  const double rnu = mm.rnu, rlam = mm.rlam;
-  const double Sy = (double)syq[idx], Sz = (double)szq[idx];
+  const A qs0 = (A)c.qs[0], qs1 = (A)c.qs[1], qs2 = (A)c.qs[2], scale = (A)c.scale;
+  const A qx = p.i * qs0, qy = p.j * qs1, qz = p.k * qs2;
+  const A wy = (p.j + 1) * qs1, wz = (p.k + 1) * qs2;
+  const A qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
+  const A rnu = mm.rnu, rlam = mm.rlam;
+  const A Sy = (A)syq[idx], Sz = (A)szq[idx];
   // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q|^2).  A
   // component's factor on its own (shifted) lattice at n equals the factor on
   // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same
   // wavenumber), which is also the G of its fgrad^ term: one evaluation serves
   // both, three per point.  exp is evaluated rather than taken as a product of
   // the one-axis tables: the table loads measured slower in both precisions
-  auto g3 = [&](int i, int j, int k) -> double {
-    const double x = i * c.qs[0], y = j * c.qs[1], z = k * c.qs[2];
-    return exp(-1.57 * 1.57 * 2 * (x * x + y * y + z * z));
+  auto g3 = [&](int i, int j, int k) -> A {
+    const A x = i * qs0, y = j * qs1, z = k * qs2;
+    return exp(A(-1.57 * 1.57 * 2) * (x * x + y * y + z * z));
   };
 
   // x component (S:1583-1709) of point (i-1, j, k), computed by this thread:
@@ -169,18 +172,18 @@ __global__ __launch_bounds__(BS) void k_velocity(
   // components at one point read no other plane's S_y, S_z, fgrad^ or erho^).
   // It is zero on the plane i = Nx-1 (quirk Q2), written by the i = 0 threads.
   if (p.i > 0) {
-    const double Sx = (double)sxq[idx - sI];
-    const double wx = p.i * c.qs[0];  // vs_x[i-1]
-    const double Syp = (p.j > 0) ? (double)syq[idx - sJ] : 0.0;
-    const double Szp = (p.k > 0) ? (double)szq[idx - 1] : 0.0;
-    const double dot = wx * Sx + qy * Syp + qz * Szp;
-    const double inv = 1.0 / (wx * wx + qy2 + qz2);
-    const double cm1 = c.scale * inv, cm2 = wx * inv;
-    const double gx = g3(p.i, p.j, p.k);
-    const double fg = (double)fgq[idx] + (double)erq[idx] * gx;
+    const A Sx = (A)sxq[idx - sI];
+    const A wx = p.i * qs0;  // vs_x[i-1]
+    const A Syp = (p.j > 0) ? (A)syq[idx - sJ] : A(0);
+    const A Szp = (p.k > 0) ? (A)szq[idx - 1] : A(0);
+    const A dot = wx * Sx + qy * Syp + qz * Szp;
+    const A inv = A(1) / (wx * wx + qy2 + qz2);
+    const A cm1 = scale * inv, cm2 = wx * inv;
+    const A gx = g3(p.i, p.j, p.k);
+    const A fg = (A)fgq[idx] + (A)erq[idx] * gx;
     const long long ox = mo + idx - sI;
-    const double sol = (cm1 * rnu) * (Sx - cm2 * dot);
-    const double irr = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
+    const A sol = (cm1 * rnu) * (Sx - cm2 * dot);
+    const A irr = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
     vtx[ox] = (T)(sol + irr);
     virrx[ox] = (T)irr;
   } else {
@@ -188,29 +191,29 @@ __global__ __launch_bounds__(BS) void k_velocity(
     vtx[ox] = (T)0;
     virrx[ox] = (T)0;
   }
-  double oy = 0, ry = 0, oz = 0, rz = 0;
+  A oy = 0, ry = 0, oz = 0, rz = 0;
   if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
-    const double gy = g3(p.i, p.j + 1, p.k);
+    const A gy = g3(p.i, p.j + 1, p.k);
     // y component
     {
-      const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + sJ] : 0.0;
-      const double Szp = (p.k > 0) ? (double)szq[idx + sJ - 1] : 0.0;
-      const double dot = qx * Sxp + wy * Sy + qz * Szp;
-      const double inv = 1.0 / (qx2 + wy * wy + qz2);
-      const double cm1 = c.scale * inv, cm2 = wy * inv;
-      const double fg = (double)fgq[idx + sJ] + (double)erq[idx + sJ] * gy;
+      const A Sxp = (p.i > 0) ? (A)sxq[idx - sI + sJ] : A(0);
+      const A Szp = (p.k > 0) ? (A)szq[idx + sJ - 1] : A(0);
+      const A dot = qx * Sxp + wy * Sy + qz * Szp;
+      const A inv = A(1) / (qx2 + wy * wy + qz2);
+      const A cm1 = scale * inv, cm2 = wy * inv;
+      const A fg = (A)fgq[idx + sJ] + (A)erq[idx + sJ] * gy;
       oy = (cm1 * rnu) * (Sy - cm2 * dot);
       ry = (cm1 * rlam) * (wy * fg + cm2 * dot * gy);
     }
     // z component
     {
-      const double Sxp = (p.i > 0) ? (double)sxq[idx - sI + 1] : 0.0;
-      const double Syp = (p.j > 0) ? (double)syq[idx - sJ + 1] : 0.0;
-      const double dot = qx * Sxp + qy * Syp + wz * Sz;
-      const double inv = 1.0 / (qx2 + qy2 + wz * wz);
-      const double cm1 = c.scale * inv, cm2 = wz * inv;
-      const double gz = g3(p.i, p.j, p.k + 1);
-      const double fg = (double)fgq[idx + 1] + (double)erq[idx + 1] * gz;
+      const A Sxp = (p.i > 0) ? (A)sxq[idx - sI + 1] : A(0);
+      const A Syp = (p.j > 0) ? (A)syq[idx - sJ + 1] : A(0);
+      const A dot = qx * Sxp + qy * Syp + wz * Sz;
+      const A inv = A(1) / (qx2 + qy2 + wz * wz);
+      const A cm1 = scale * inv, cm2 = wz * inv;
+      const A gz = g3(p.i, p.j, p.k + 1);
+      const A fg = (A)fgq[idx + 1] + (A)erq[idx + 1] * gz;
       oz = (cm1 * rnu) * (Sz - cm2 * dot);
       // quirk Q3 (S:1932): CM1y in the exponent at k=0, j>=1, except the
       // first plane of each reference slab (the y component's factor)
@@ -387,6 +390,23 @@ template <typename T>
 void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* rho, hipStream_t s) {
   hipLaunchKernelGGL(k_relax<T>, grid_of(c), dim3(kBlock), 0, s, c, amp, laprl, rho);
 }
+// 1024-point workgroups (four lines of 256) when they tile the field: the
+// j+-1 lines a point reads are then mostly read by its own workgroup, on the
+// same CU at the same time
+template <typename T, typename A>
+static void launch_velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq, const T* fgq,
+                            const T* erq, T* vx, T* vy, T* vz, T* virrx, T* virry, T* virrz, hipStream_t s) {
+  constexpr int BS = 1024;
+  const long long n3 = n3_of(c);
+  if (n3 % BS == 0) {
+    hipLaunchKernelGGL((k_velocity<T, A, BS>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s, c, sxq,
+                       syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz);
+  } else {
+    hipLaunchKernelGGL((k_velocity<T, A, kBlock>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
+                       vx, vy, vz, virrx, virry, virrz);
+  }
+}
+
 template <typename T>
 void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                             const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
@@ -398,19 +418,14 @@ void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, 
 template <typename T>
 void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                               const T* fgq, const T* erq, T* vx, T* vy, T* vz, T* virrx,
-                              T* virry, T* virrz, hipStream_t s) {
-  // 1024-point workgroups (four lines of 256) when they tile the field: the
-  // j+-1 lines a point reads are then mostly read by its own workgroup, on
-  // the same CU at the same time
-  constexpr int BS = 1024;
-  const long long n3 = n3_of(c);
-  if (n3 % BS == 0) {
-    hipLaunchKernelGGL((k_velocity<T, BS>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s, c, sxq,
-                       syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz);
-  } else {
-    hipLaunchKernelGGL((k_velocity<T, kBlock>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
-                       vx, vy, vz, virrx, virry, virrz);
+                              T* virry, T* virrz, bool arith32, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (arith32) {
+      launch_velocity<T, float>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, s);
+      return;
+    }
   }
+  launch_velocity<T, double>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, s);
 }
 template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
