@@ -40,8 +40,9 @@ extern "C" {
 
 /* precision of the stepper */
 #define SMAISO_FP64 0 /* every field, transform and kernel in FP64 */
-/* FP32 storage of the work fields (psi, grad psi, amp, L psi, mu, forces,
- * velocities, Nl, ...); FP64 state (psiq, psiq_old), rho, f_rho and grad rho;
+/* FP32 storage of the work fields (psi, grad psi, grad rho, amp, L psi, mu,
+ * forces, velocities, Nl, ...); FP64 state (psiq, psiq_old), rho and f_rho;
+ * grad rho formed in FP64 from the FP64 rho;
  * FP64 arithmetic in the forces and density kernels, the psi update, the
  * implicit denominators and the compensated reductions; the transform that
  * forms psi from psiq keeps FP64 intermediates.  Transform passes and the

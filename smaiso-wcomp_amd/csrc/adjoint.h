@@ -28,7 +28,7 @@ struct AdjArgs {
   const T* lpsi;      // L psi
   const T* laprl;     // lap(rho L psi)
   const T* gp[3];     // grad psi
-  const double* rd[3];  // grad rho (FP64 in both precisions)
+  const T* rd[3];     // grad rho
   const T* v[3];      // total velocity (real space)
   const T* divv;      // div v (real space)
   const T* vtq[3];    // total velocity vsol^ + virr^ on the staggered lattices (spectral)

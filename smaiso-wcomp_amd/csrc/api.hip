@@ -102,8 +102,10 @@ struct smaiso_handle {
   double* gs[3] = {nullptr, nullptr, nullptr};  // exp(-1.57^2 2 vs^2) per axis
   // FP64 state
   double *psiq = nullptr, *psiq_old = nullptr, *rho = nullptr, *f_rho = nullptr;
-  double* rd[3] = {nullptr, nullptr, nullptr};  // grad rho, FP64 in both precisions (rho is
-                                               // ~constant: its gradient is a small difference)
+  // grad rho: formed in FP64 from the FP64 rho (rho is ~constant, so its
+  // gradient is a small difference of large values) and stored in the work
+  // type, whose rounding is then relative to the gradient itself
+  void* rd[3] = {nullptr, nullptr, nullptr};
   double *partials = nullptr, *diag_out = nullptr;
   // FP64 field of the group's size, FP32 storage only: the field getters'
   // widened copy, and within a step the intermediates of psi's inverse
@@ -266,7 +268,9 @@ struct smaiso_handle {
   hipError_t run_pass_cast(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io) {
     const double n = (double)g.n3() * g.nb;
     const double bytes = pass_bytes(in_dt, io, g) + n * ((out_dt == F64 ? 8 : 4) - (in_dt == F64 ? 8 : 4));
-    Scope sc(this, axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X, bytes);
+    Scope sc(this, is_axis_op(io.f.op) ? SMAISO_KC_R2R_AXIS
+                   : axis == 2 ? SMAISO_KC_R2R_Z : axis == 1 ? SMAISO_KC_R2R_Y : SMAISO_KC_R2R_X,
+             bytes);
     return r2r.pass_cast(in_dt, out_dt, kind, axis, io, g, stream);
   }
   // 3-D transform z, y, x of all members (or of gg's batch); f's load op acts
@@ -347,6 +351,14 @@ struct smaiso_handle {
     io.out = out;
     io.f = f;
     return run_pass(dt, REDFT10, axis, io, g);
+  }
+  // as axpass, reading in_dt and storing out_dt
+  hipError_t axpass_cast(DType in_dt, DType out_dt, int axis, const void* in, void* out, const Fuse& f) {
+    PassIO io;
+    io.in = in;
+    io.out = out;
+    io.f = f;
+    return run_pass_cast(in_dt, out_dt, REDFT10, axis, io);
   }
   double fb(int nfields) const { return (double)ntot() * tsz * nfields; }
 };
@@ -485,7 +497,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->mc = static_cast<MemberConsts*>(tmpp);
   if ((e = hipMemcpy(h->mc, h->mc_host.data(), sizeof(MemberConsts) * n, hipMemcpyHostToDevice)) != hipSuccess)
     return hipfail(e, "hipMemcpy");
-  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho, &h->rd[0], &h->rd[1], &h->rd[2]};
+  double** dbufs[] = {&h->psiq, &h->psiq_old, &h->rho, &h->f_rho};
   for (double** b : dbufs) {
     if ((e = alloc(db, &tmpp)) != hipSuccess) return hipfail(e, "hipMalloc (state)");
     *b = static_cast<double*>(tmpp);
@@ -498,7 +510,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
                                &h->divf, &h->nl, &h->tmp};
   for (int a = 0; a < 3; ++a) {
-    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->S[a]);
+    tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vtq[a]); tbufs.push_back(&h->virrq[a]);
     tbufs.push_back(&h->v[a]);
   }
@@ -521,7 +533,7 @@ int run_step(smaiso_handle* h) {
   hipStream_t s = h->stream;
   const StepConsts c = h->consts();
   T* const* gp = reinterpret_cast<T* const*>(h->gp);
-  double* const* rd = h->rd;
+  T* const* rd = reinterpret_cast<T* const*>(h->rd);
   T* const* S = reinterpret_cast<T* const*>(h->S);
   T* const* f = reinterpret_cast<T* const*>(h->f);
   T* const* vtq = reinterpret_cast<T* const*>(h->vtq);
@@ -545,7 +557,8 @@ int run_step(smaiso_handle* h) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
-    HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
+    if (h->mixed) HIPTRY(h, h->axpass_cast(F64, F32, a, h->rho, rd[a], g));
+    else HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
   }
   for (int a = 0; a < 3; ++a) {
     Fuse g = fb;
@@ -602,7 +615,7 @@ int run_step(smaiso_handle* h) {
     io.zout[0] = static_cast<T*>(h->fgrad);
     io.zout[1] = static_cast<T*>(h->erho);
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 19 + db1);
       HIPTRY(h, forces_zpass<T>(c, io, h->r2r, s));
     }
     const int fa = h->a32(SMAISO_STAGE_FORCES);
@@ -611,7 +624,7 @@ int run_step(smaiso_handle* h) {
     for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw_yx(kST[a], S[a], fa));
   } else {
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 16 + db1 * 4);
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 19 + db1);
       K::forces(c, static_cast<T*>(h->lpsi), static_cast<T*>(h->laprl), h->rho,
                 static_cast<T*>(h->amp), static_cast<T*>(h->psi), gp[0], gp[1], gp[2], rd[0],
                 rd[1], rd[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
@@ -662,13 +675,13 @@ int run_step(smaiso_handle* h) {
     io.divf = static_cast<const T*>(h->divf);
     io.nlz = static_cast<T*>(h->nlq);
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
       HIPTRY(h, density_zpass<T>(c, io, h->r2r, s));
     }
     HIPTRY(h, h->xw_yx(kCT, h->nlq, h->a32(SMAISO_STAGE_NL)));
   } else {
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 12 + db1 * 7);
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
       K::density_nl(c, v[0], v[1], v[2], rd[0], rd[1], rd[2], static_cast<T*>(h->divv), h->rho,
                     h->f_rho, static_cast<T*>(h->mu), static_cast<T*>(h->amp),
                     static_cast<T*>(h->psi), static_cast<T*>(h->divf), gp[0], gp[1], gp[2],
@@ -800,7 +813,7 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_GPSIX: case SMAISO_FIELD_GPSIY: case SMAISO_FIELD_GPSIZ:
       *out = as64(h->gp[fid - SMAISO_FIELD_GPSIX]); return SMAISO_OK;
     case SMAISO_FIELD_GRHOX: case SMAISO_FIELD_GRHOY: case SMAISO_FIELD_GRHOZ:
-      *out = h->rd[fid - SMAISO_FIELD_GRHOX]; return SMAISO_OK;
+      *out = as64(h->rd[fid - SMAISO_FIELD_GRHOX]); return SMAISO_OK;
     case SMAISO_FIELD_LAPRHOL: *out = as64(h->laprl); return SMAISO_OK;
     case SMAISO_FIELD_PSI_OLD: *out = as64(h->psi_prev); return SMAISO_OK;
     default: return fail(h, SMAISO_E_INVALID, fmt("unknown field id %d", fid));
@@ -1447,7 +1460,7 @@ AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
   A.divv = tp(h->divv);
   for (int a = 0; a < 3; ++a) {
     A.gp[a] = tp(h->gp[a]);
-    A.rd[a] = h->rd[a];
+    A.rd[a] = tp(h->rd[a]);
     A.v[a] = tp(h->v[a]);
     A.vtq[a] = tp(h->vtq[a]);
     A.virrq[a] = tp(h->virrq[a]);

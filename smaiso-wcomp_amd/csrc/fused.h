@@ -19,7 +19,7 @@ struct ForcesIO {
   const T *lpsi, *laprl, *amp, *psi;
   const T* gp[3];      // grad psi
   const double* rho;
-  const double* rd[3];  // grad rho
+  const T* rd[3];       // grad rho
   T* mu;                // chemical potential (real space)
   T* f[3];              // forces f_a (real space)
   T* zout[5];           // z pass of fgrad, erho, S_x, S_y, S_z
@@ -38,7 +38,7 @@ bool forces_zpass_supported(DType dt, int nz);
 template <typename T>
 struct DensityIO {
   const T* v[3];        // total velocity (real space)
-  const double* rd[3];  // grad rho at the step start
+  const T* rd[3];       // grad rho at the step start
   const T* divv;
   double* rho;          // in: rho at the step start; out: rho_new
   double* f_rho;        // in: the last step's source; out: this step's

@@ -2,8 +2,9 @@
 // sources into an FP64 intermediate (z pass, plain or with the psi update on
 // load) and an FP64 intermediate into FP32 output (y or x pass, plain).  A 3-D
 // transform between FP32 fields then keeps its partly transformed values in
-// FP64.  A translation unit of its own so that these instantiations compile
-// in parallel with the rest of the library.
+// FP64.  Also the one-axis gradient of an FP64 field stored as FP32 (any
+// axis: grad rho of FP32 storage).  A translation unit of its own so that
+// these instantiations compile in parallel with the rest of the library.
 #include "r2r_kernels.h"
 
 namespace smaiso {
@@ -14,12 +15,19 @@ template <typename T, typename TO, int N>
 hipError_t launch_cast_n(const PassIO& io, void* tw, void* mk, int kind, int axis, const Geom& g,
                          hipStream_t s) {
   using C = TileChoice<T, N>;
+  constexpr bool kNarrowing = sizeof(T) == 8 && sizeof(TO) == 4;
   if (axis == 2) {
     if (io.f.op == OP_LD_PSI) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_LD_PSI, TO, double, 2>(io, tw, mk, kind, axis, g, s);
     if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_PLAIN, TO, double, 2>(io, tw, mk, kind, axis, g, s);
+    if constexpr (kNarrowing) {
+      if (io.f.op == OP_AX_GRAD) return launch_fft_pn<T, N, C::PZ, C::NTZ, OP_AX_GRAD, TO, double, 2>(io, tw, mk, kind, axis, g, s);
+    }
     return hipErrorInvalidValue;
   }
   if (io.f.op == OP_PLAIN) return launch_fft_pn<T, N, C::PS, C::NTS, OP_PLAIN, TO, double, 3>(io, tw, mk, kind, axis, g, s);
+  if constexpr (kNarrowing) {
+    if (io.f.op == OP_AX_GRAD) return launch_fft_pn<T, N, C::PS, C::NTS, OP_AX_GRAD, TO, double, 3>(io, tw, mk, kind, axis, g, s);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -30,13 +38,15 @@ hipError_t launch_cast_direct(const PassIO& io, void* costab, int kind, int axis
   d.p = make_args<T, TO>(io, nullptr, nullptr, kind, axis, g);
   d.costab = static_cast<const double*>(costab);
   d.n = n;
+  // the one-axis operators keep the forward outputs beside the tile
+  const int mul = is_axis_op(io.f.op) ? 2 : 1;
   int tl = 16;  // lines per tile; the LDS tile is FP64
-  while (tl > 1 && (size_t)tl * n * sizeof(double) > 48 * 1024) tl /= 2;
+  while (tl > 1 && (size_t)tl * n * sizeof(double) * mul > 48 * 1024) tl /= 2;
   d.tl = tl;
   long long blocks;
   if (axis == 2) blocks = ((long long)g.nx * g.ny + tl - 1) / tl;
   else blocks = (long long)(axis == 1 ? g.nx : g.ny) * ((g.nz + tl - 1) / tl);
-  const size_t lds = (size_t)tl * n * sizeof(double);
+  const size_t lds = (size_t)tl * n * sizeof(double) * mul;
   hipLaunchKernelGGL((direct_pass_kernel<T, TO>), dim3((unsigned)blocks, (unsigned)g.nb), dim3(kThreads), lds, s, d);
   return hipGetLastError();
 }
@@ -71,10 +81,15 @@ hipError_t R2R::pass_cast(DType in_dt, DType out_dt, int kind, int axis, const P
                           hipStream_t s) {
   if (in_dt == out_dt) return pass(in_dt, kind, axis, io, g, s);
   const int op = io.f.op;
-  if (!(op == OP_PLAIN || (op == OP_LD_PSI && axis == 2))) return hipErrorInvalidValue;
+  if (!(op == OP_PLAIN || (op == OP_LD_PSI && axis == 2) || op == OP_AX_GRAD)) return hipErrorInvalidValue;
   if (axis < 0 || axis > 2) return hipErrorInvalidValue;
-  // FP32 -> FP64 in the first (z) pass, FP64 -> FP32 in a later pass
-  if ((in_dt == F32) != (axis == 2)) return hipErrorNotSupported;
+  if (op == OP_AX_GRAD) {
+    // one pass: FP64 -> FP32 along any axis
+    if (in_dt != F64) return hipErrorNotSupported;
+  } else if ((in_dt == F32) != (axis == 2)) {
+    // FP32 -> FP64 in the first (z) pass, FP64 -> FP32 in a later pass
+    return hipErrorNotSupported;
+  }
   const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
   Tables* tb = nullptr;
   hipError_t e = tables(F64, n, &tb);

@@ -50,7 +50,7 @@ struct StepKernels {
   // a10 (FP64 arithmetic)
   static void forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                      const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
-                     const double* rdx, const double* rdy, const double* rdz, T* fgrad, T* erho, T* mu, T* sx,
+                     const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu, T* sx,
                      T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s);
   // a11 (fgrad += G erho) + a12: the total velocity vsol^ + virr^ (the
   // input of v's inverse transforms, summed before the store) and the
@@ -61,7 +61,7 @@ struct StepKernels {
                        T* virry, T* virrz, bool arith32, hipStream_t s);
   // a15 + a16 (FP64 arithmetic)
   static void density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
-                         const double* rdx, const double* rdy, const double* rdz, const T* divv, double* rho,
+                         const T* rdx, const T* rdy, const T* rdz, const T* divv, double* rho,
                          double* f_rho, const T* mu, const T* amp, const T* psi,
                          const T* divf, const T* gpx, const T* gpy, const T* gpz, T* nl,
                          hipStream_t s);

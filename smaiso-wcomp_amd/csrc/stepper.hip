@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(
     StepConsts c, const T* __restrict__ lpsi, const T* __restrict__ laprl,
     const double* __restrict__ rho, const T* __restrict__ amp, const T* __restrict__ psi,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
-    const double* __restrict__ rdx, const double* __restrict__ rdy, const double* __restrict__ rdz,
+    const T* __restrict__ rdx, const T* __restrict__ rdy, const T* __restrict__ rdz,
     T* __restrict__ fgrad, T* __restrict__ erho, T* __restrict__ mu, T* __restrict__ sx,
     T* __restrict__ sy, T* __restrict__ sz, T* __restrict__ fx, T* __restrict__ fy,
     T* __restrict__ fz) {
@@ -233,8 +233,8 @@ __global__ __launch_bounds__(BS) void k_velocity(
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_density_nl(
     StepConsts c, const T* __restrict__ vx, const T* __restrict__ vy,
-    const T* __restrict__ vz, const double* __restrict__ rdx, const double* __restrict__ rdy,
-    const double* __restrict__ rdz, const T* __restrict__ divv, double* __restrict__ rho,
+    const T* __restrict__ vz, const T* __restrict__ rdx, const T* __restrict__ rdy,
+    const T* __restrict__ rdz, const T* __restrict__ divv, double* __restrict__ rho,
     double* __restrict__ f_rho, const T* __restrict__ mu,
     const T* __restrict__ amp, const T* __restrict__ psi, const T* __restrict__ divf,
     const T* __restrict__ gpx, const T* __restrict__ gpy, const T* __restrict__ gpz,
@@ -410,7 +410,7 @@ static void launch_velocity(const StepConsts& c, const T* sxq, const T* syq, con
 template <typename T>
 void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, const double* rho,
                             const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
-                            const double* rdx, const double* rdy, const double* rdz, T* fgrad, T* erho, T* mu,
+                            const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu,
                             T* sx, T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s) {
   hipLaunchKernelGGL(k_forces<T>, grid_of(c), dim3(kBlock), 0, s, c, lpsi, laprl, rho, amp, psi,
                      gpx, gpy, gpz, rdx, rdy, rdz, fgrad, erho, mu, sx, sy, sz, fx, fy, fz);
@@ -429,7 +429,7 @@ void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, c
 }
 template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
-                                const double* rdx, const double* rdy, const double* rdz, const T* divv,
+                                const T* rdx, const T* rdy, const T* rdz, const T* divv,
                                 double* rho, double* f_rho, const T* mu, const T* amp,
                                 const T* psi, const T* divf, const T* gpx, const T* gpy,
                                 const T* gpz, T* nl, hipStream_t s) {
