@@ -83,6 +83,11 @@ struct smaiso_handle {
   // the next step, so getters and diagnostics never read fields no step wrote.
   bool derived = false;
   bool nlq_valid = false;  // N^l of the last step: written by a step or a checkpoint load
+  // virrq holds the irrotational velocity of the step that ended at epoch
+  // virr_epoch; keep_virr makes the step write it (the adjoint's recomputed
+  // steps read it)
+  long long virr_epoch = -1;
+  bool keep_virr = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -116,7 +121,8 @@ struct smaiso_handle {
   // work fields (T = double or float)
   void *psi = nullptr, *psi_prev = nullptr, *nlq = nullptr, *nlq_old = nullptr;
   // vtq: the total velocity vsol^ + virr^ of the last step (spectral, on the
-  // staggered lattices); virrq: its irrotational part
+  // staggered lattices); virrq: its irrotational part, written only when asked
+  // for (virr_epoch, keep_virr): the step forms div v^ without it
   void *gp[3] = {}, *S[3] = {}, *f[3] = {}, *vtq[3] = {}, *virrq[3] = {}, *v[3] = {};
   void *amp = nullptr, *lpsi = nullptr, *laprl = nullptr, *mu = nullptr, *fgrad = nullptr,
        *erho = nullptr, *divv = nullptr, *divf = nullptr, *nl = nullptr, *tmp = nullptr;
@@ -252,7 +258,6 @@ struct smaiso_handle {
     else if (op == OP_LD_GRAD_D || op == OP_LD_SH || op == OP_LD_RELAXLAP) in = 8;
     else if (op == OP_LD_AMP) in = 4 * es;
     else if (op == OP_LD_PROD) in = 8 + es;
-    else if (op == OP_LD_DIV) in = 3 * es;
     else if (op == OP_LD_PSI) { in = 16 + 2 * es; out = es + 8; }
     else if (op == OP_AX_GAUSS_AMP) in = 4 * es;
     else if (op == OP_AX_DIV && io.f.accumulate) in = 2 * es;
@@ -634,22 +639,19 @@ int run_step(smaiso_handle* h) {
     HIPTRY(h, h->xw(kCT, h->erho, h->erho));
     for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kST[a], S[a], S[a]));
   }
-  // a11 tail + a12 (S:1479-1974)
+  // a11 tail + a12 (S:1479-1974) and the spectral div v^ of a13 (S:2042-2063)
   {
-    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 11);
+    smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * (h->keep_virr ? 12 : 9));
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
-                vtq[0], vtq[1], vtq[2], virrq[0], virrq[1], virrq[2], h->a32(SMAISO_STAGE_VELSOLVE) != 0, s);
+                vtq[0], vtq[1], vtq[2], h->keep_virr ? virrq[0] : nullptr, virrq[1], virrq[2],
+                static_cast<T*>(h->divv), h->a32(SMAISO_STAGE_VELSOLVE) != 0, s);
+    if (h->keep_virr) h->virr_epoch = h->epoch;
   }
-  // a13 + a14 (S:1979-2117): div v is formed on load by the z pass of its
-  // inverse transform; div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum
-  // of one-axis operators, one accumulating pass per component;
-  // v = iST(vsol^ + virr^) by linearity
+  // a13 + a14 (S:2064-2117): div v = iCT(div v^) in place; div f =
+  // iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum of one-axis operators, one
+  // accumulating pass per component; v = iST(vsol^ + virr^) by linearity
   {
-    Fuse g = fb;
-    g.op = OP_LD_DIV; g.c0 = 1.0;
-    g.arith32 = h->a32(SMAISO_STAGE_DIVV);
-    g.src[0] = virrq[0]; g.src[1] = virrq[1]; g.src[2] = virrq[2];
-    HIPTRY(h, h->xwf(kICT, g, h->divv));
+    HIPTRY(h, h->xw(kICT, h->divv, h->divv, nullptr, h->a32(SMAISO_STAGE_DIVV)));
     Fuse d = fb;
     d.op = OP_AX_DIV;
     d.arith32 = h->a32(SMAISO_STAGE_DIVF);
@@ -752,6 +754,27 @@ int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, 
   return SMAISO_OK;
 }
 
+// virr^ of the last step in virrq: the velocity solve run again from the
+// step's spectral S, fgrad and erho (the step itself stores only the total
+// and div v^); its div v^ goes to tmp, which the caller overwrites
+template <typename T>
+hipError_t solve_virr(smaiso_handle* h) {
+  T* const* S = reinterpret_cast<T* const*>(h->S);
+  T* const* vtq = reinterpret_cast<T* const*>(h->vtq);
+  T* const* virrq = reinterpret_cast<T* const*>(h->virrq);
+  StepKernels<T>::velocity(h->consts(), S[0], S[1], S[2], static_cast<T*>(h->fgrad),
+                           static_cast<T*>(h->erho), vtq[0], vtq[1], vtq[2], virrq[0], virrq[1],
+                           virrq[2], static_cast<T*>(h->tmp), h->a32(SMAISO_STAGE_VELSOLVE) != 0,
+                           h->stream);
+  return hipGetLastError();
+}
+hipError_t ensure_virr(smaiso_handle* h) {
+  if (h->virr_epoch == h->epoch) return hipSuccess;
+  const hipError_t e = h->mixed ? solve_virr<float>(h) : solve_virr<double>(h);
+  if (e == hipSuccess) h->virr_epoch = h->epoch;
+  return e;
+}
+
 // Produce field `fid` of all members as FP64 on the device; returns the
 // device pointer of member 0 (possibly a handle scratch buffer).
 int field_device(smaiso_handle* h, int fid, const double** out) {
@@ -776,8 +799,10 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_VX: case SMAISO_FIELD_VY: case SMAISO_FIELD_VZ:
       *out = as64(h->v[fid - SMAISO_FIELD_VX]); return SMAISO_OK;
     case SMAISO_FIELD_VSOLX: case SMAISO_FIELD_VSOLY: case SMAISO_FIELD_VSOLZ: {
-      // vsol = v - iST(virr^): the step keeps the total and the irrotational part
+      // vsol = v - iST(virr^): the step keeps the total; virr^ is solved again
+      // from the step's spectral S, fgrad and erho, which stay until the next step
       const int a = fid - SMAISO_FIELD_VSOLX;
+      HIPTRY(h, ensure_virr(h));
       HIPTRY(h, h->xw(kIST[a], h->virrq[a], h->tmp));
       if (h->mixed)
         sub_fields(static_cast<const float*>(h->v[a]), static_cast<const float*>(h->tmp),
@@ -789,6 +814,7 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     }
     case SMAISO_FIELD_VIRRX: case SMAISO_FIELD_VIRRY: case SMAISO_FIELD_VIRRZ: {
       const int a = fid - SMAISO_FIELD_VIRRX;
+      HIPTRY(h, ensure_virr(h));
       HIPTRY(h, h->xw(kIST[a], h->virrq[a], h->tmp));
       *out = as64(h->tmp); return SMAISO_OK;
     }
@@ -1608,7 +1634,9 @@ int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const 
     }
     for (int n = n1 - 1; n >= n0; --n) {
       HIPTRY(h, restore_state(h, seg, n - n0, n));
+      h->keep_virr = true;  // the transposed velocity solve reads virr^
       rc = run_step_any(h);
+      h->keep_virr = false;
       if (rc != SMAISO_OK) return rc;
       rc = adjoint_step_any(h, seg.field(n - n0, 2), n == 0);
       if (rc != SMAISO_OK) return rc;

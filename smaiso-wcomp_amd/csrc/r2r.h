@@ -30,7 +30,6 @@ enum FuseOp : int {
   OP_LD_SH = 4,        // x = alpha (q0^2 - |q|^2) psiq              (psiq double)
   OP_LD_RELAXLAP = 5,  // x = -|q|^2 alpha (q0^2 - |q|^2) psiq
   OP_LD_PROD = 6,      // x = s0 s1                                  (s0 double)
-  OP_LD_DIV = 7,       // x = c0 sum_a Vq_a[n_a] s_a[n - e_a]        (0 at n_a = 0)
   OP_MID_AMP = 8,      // mode *= scale exp(-1.57*1.57*|q|^2*2)
   OP_MID_NEGLAP = 9,   // mode *= -scale |q|^2
   OP_LD_PSI = 10,      // x = new psiq from psiq, psiq_new (= old), Nl^ (src[0]), Nl^_old; stored to psiq_new
@@ -58,7 +57,7 @@ struct Fuse {
   int op = OP_PLAIN;
   const void* src[4] = {nullptr, nullptr, nullptr, nullptr};
   int axis = 0;          // shift axis of OP_LD_GRAD_*
-  double c0 = 1.0;       // multiplier of OP_LD_GRAD_* and OP_LD_DIV
+  double c0 = 1.0;       // multiplier of OP_LD_GRAD_*
   const double* vq[3] = {nullptr, nullptr, nullptr};  // DCT-lattice wavenumbers
   const double* vs[3] = {nullptr, nullptr, nullptr};  // DST-lattice wavenumbers
   // wavenumber spacing pi/(d_a N_a) per axis: vq_a[n] = pi n/(d_a N_a), vs_a[n]

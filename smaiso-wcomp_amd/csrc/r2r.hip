@@ -70,7 +70,6 @@ static hipError_t launch_fft(const PassIO& io, void* tw, void* mk, int kind, int
       case OP_LD_SH: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_SH, T, A, 2>(io, tw, mk, kind, axis, g, s);
       case OP_LD_RELAXLAP: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_RELAXLAP, T, A, 2>(io, tw, mk, kind, axis, g, s);
       case OP_LD_PROD: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_PROD, T, A, 2>(io, tw, mk, kind, axis, g, s);
-      case OP_LD_DIV: return launch_fft_pn<T, N, PZ, NTZ, OP_LD_DIV, T, A, 2>(io, tw, mk, kind, axis, g, s);
       case OP_PLAIN: return launch_fft_pn<T, N, PZ, NTZ, OP_PLAIN, T, A, 2>(io, tw, mk, kind, axis, g, s);
       default: break;
     }

@@ -438,15 +438,6 @@ __device__ __forceinline__ double load_elem(const PassArgs<T, TO>& a, int i, int
     const double ps = static_cast<const double*>(a.f.src[0])[idx];
     if constexpr (OP == OP_LD_SH) return a.f.alpha * (a.f.q02 - m2) * ps;
     else return -m2 * a.f.alpha * (a.f.q02 - m2) * ps;
-  } else if constexpr (OP == OP_LD_DIV) {
-    const T* sx = static_cast<const T*>(a.f.src[0]);
-    const T* sy = static_cast<const T*>(a.f.src[1]);
-    const T* sz = static_cast<const T*>(a.f.src[2]);
-    const double vx = (double)sx[i > 0 ? idx - (long long)a.ny * a.nz : idx];
-    const double vy = (double)sy[j > 0 ? idx - a.nz : idx];
-    const double vz = (double)sz[m > 0 ? idx - 1 : idx];
-    return a.f.c0 * (vq_at(a.f, 0, i) * (i > 0 ? vx : 0.0) + vq_at(a.f, 1, j) * (j > 0 ? vy : 0.0) +
-                     vq_at(a.f, 2, m) * (m > 0 ? vz : 0.0));
   } else if constexpr (OP == OP_LD_PSI) {
     // the new psiq is the transform's input and is also stored (over psiq_old)
     const double r = psi_update_value<T>(a.f, idx, i, j, m);
@@ -469,7 +460,6 @@ __device__ double load_elem_rt(const PassArgs<T, TO>& a, long long g, int m, lon
     case OP_LD_SH: return load_elem<T, OP_LD_SH>(a, i, j, m, idx);
     case OP_LD_RELAXLAP: return load_elem<T, OP_LD_RELAXLAP>(a, i, j, m, idx);
     case OP_LD_PROD: return load_elem<T, OP_LD_PROD>(a, i, j, m, idx);
-    case OP_LD_DIV: return load_elem<T, OP_LD_DIV>(a, i, j, m, idx);
     case OP_LD_PSI: return load_elem<T, OP_LD_PSI>(a, i, j, m, idx);
     case OP_AX_GAUSS_AMP: return load_elem<T, OP_LD_AMP>(a, i, j, m, idx);
     default: return load_elem<T, OP_PLAIN>(a, i, j, m, idx);

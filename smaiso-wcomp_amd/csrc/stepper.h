@@ -52,13 +52,14 @@ struct StepKernels {
                      const T* amp, const T* psi, const T* gpx, const T* gpy, const T* gpz,
                      const T* rdx, const T* rdy, const T* rdz, T* fgrad, T* erho, T* mu, T* sx,
                      T* sy, T* sz, T* fx, T* fy, T* fz, hipStream_t s);
-  // a11 (fgrad += G erho) + a12: the total velocity vsol^ + virr^ (the
-  // input of v's inverse transforms, summed before the store) and the
-  // irrotational part virr^ (the input of div v); FP64 arithmetic, FP32 for
-  // FP32 storage when arith32
+  // a11 (fgrad += G erho) + a12 + the spectral divergence of a13: the total
+  // velocity vsol^ + virr^ (the input of v's inverse transforms, summed before
+  // the store), the irrotational part virr^ when virrx is not null, and
+  // div v^ (the input of div v's inverse transform) when divq is not null;
+  // FP64 arithmetic, FP32 for FP32 storage when arith32
   static void velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                        const T* fgq, const T* erq, T* vx, T* vy, T* vz, T* virrx,
-                       T* virry, T* virrz, bool arith32, hipStream_t s);
+                       T* virry, T* virrz, T* divq, bool arith32, hipStream_t s);
   // a15 + a16 (FP64 arithmetic)
   static void density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
                          const T* rdx, const T* rdy, const T* rdz, const T* divv, double* rho,

@@ -111,22 +111,40 @@ __global__ __launch_bounds__(kBlock) void k_forces(
 }
 
 // ---------------------------------------------------------------------------
-// a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974)
+// a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974), and
+// the spectral divergence of virr^ (S:2042-2063)
 // ---------------------------------------------------------------------------
+// Thread (i, j, k) computes the x component at (i-1, j, k), the y component at
+// (i, j-1, k) and the z component at (i, j, k-1).  On their staggered lattices
+// these three points carry the wavenumbers of cosine mode (i, j, k): vs_x[i-1]
+// = vq_x[i] and likewise on y and z.  The shifted reads of the three solves
+// (x: S_y(i+1, j-1, k), S_z(i+1, j, k-1), fgrad^(i+1, j, k); y: S_x(i-1, j+1,
+// k), S_z(i, j+1, k-1), fgrad^(i, j+1, k); z: S_x(i-1, j, k+1), S_y(i, j-1,
+// k+1), fgrad^(i, j, k+1)) then all land on the same four values S_x(i-1, j,
+// k), S_y(i, j-1, k), S_z(i, j, k-1) and fgrad^(i, j, k), so the three
+// components share one projection q.S^, one denominator |q|^2 and one
+// Gaussian G(q) (a component's factor exp(-1.57^2 2 |q_a|^2) on its own
+// lattice is G at cosine mode (i, j, k), which is also the G of fgrad^ += G
+// erho^).  The same thread holds virr_x(i-1, j, k), virr_y(i, j-1, k) and
+// virr_z(i, j, k-1), which are exactly the terms of div v^(i, j, k) =
+// sum_a Vq_a virr_a(n - e_a) (S:2042-2063), so the divergence is formed here and
+// virr^ need not be stored.  Each thread reads five values and writes four.
+//
 // A: the arithmetic type (FP64; FP32 for FP32 storage where the velocity
-// solve's FP32 stage is asked for).
-template <typename T, typename A, int BS>
+// solve's FP32 stage is asked for).  VIRR: also store virr^ (for the vsol and
+// virr read-outs and the adjoint), at the same points as the total.
+template <typename T, typename A, int BS, bool VIRR>
 __global__ __launch_bounds__(BS) void k_velocity(
     StepConsts c, const T* __restrict__ sxq_, const T* __restrict__ syq_,
     const T* __restrict__ szq_, const T* __restrict__ fgq_, const T* __restrict__ erq_,
     T* __restrict__ vtx, T* __restrict__ vty, T* __restrict__ vtz,
-    T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz) {
+    T* __restrict__ virrx, T* __restrict__ virry, T* __restrict__ virrz, T* __restrict__ divq) {
   const unsigned n3 = (unsigned)c.nx * c.ny * c.nz;
   // XCD-aware order: workgroups are dealt to the 8 XCDs round robin.  When
   // an i plane splits into 8 equal runs of whole workgroups, XCD x takes run
-  // x of every plane (a band of j lines), plane after plane, so the j+-1
-  // neighbour lines it reads are mostly its own band's, in its own 4 MiB L2.
-  // Otherwise each XCD walks a contiguous range of lines.
+  // x of every plane (a band of j lines), plane after plane, so the j-1 line
+  // and the i-1 plane a point reads are mostly its own band's, in its own 4
+  // MiB L2.  Otherwise each XCD walks a contiguous range of lines.
   const unsigned nb = gridDim.x;
   const unsigned plane = (unsigned)c.ny * c.nz;
   unsigned b = blockIdx.x;
@@ -139,11 +157,6 @@ __global__ __launch_bounds__(BS) void k_velocity(
   const unsigned idx = b * BS + threadIdx.x;
   if (idx >= n3) return;
   const long long mo = member_offset(c);
-  const T* sxq = sxq_ + mo;
-  const T* syq = syq_ + mo;
-  const T* szq = szq_ + mo;
-  const T* fgq = fgq_ + mo;
-  const T* erq = erq_ + mo;
   const MemberConsts mm = c.mc[blockIdx.y];
   const Idx p = decompose(idx, c.ny, c.nz);
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
@@ -151,80 +164,66 @@ __global__ __launch_bounds__(BS) void k_velocity(
   // memory instructions)
   const A qs0 = (A)c.qs[0], qs1 = (A)c.qs[1], qs2 = (A)c.qs[2], scale = (A)c.scale;
   const A qx = p.i * qs0, qy = p.j * qs1, qz = p.k * qs2;
-  const A wy = (p.j + 1) * qs1, wz = (p.k + 1) * qs2;
-  const A qx2 = qx * qx, qy2 = qy * qy, qz2 = qz * qz;
   const A rnu = mm.rnu, rlam = mm.rlam;
-  const A Sy = (A)syq[idx], Sz = (A)szq[idx];
-  // Gaussian factors exp(gfac/CM1_a) = exp(-1.57*1.57*2 |q|^2).  A
-  // component's factor on its own (shifted) lattice at n equals the factor on
-  // the cosine lattice at n + e_a (vs_a[n] and vq_a[n+1] are the same
-  // wavenumber), which is also the G of its fgrad^ term: one evaluation serves
-  // both, three per point.  exp is evaluated rather than taken as a product of
-  // the one-axis tables: the table loads measured slower in both precisions
-  auto g3 = [&](int i, int j, int k) -> A {
-    const A x = i * qs0, y = j * qs1, z = k * qs2;
-    return exp(A(-1.57 * 1.57 * 2) * (x * x + y * y + z * z));
-  };
-
-  // x component (S:1583-1709) of point (i-1, j, k), computed by this thread:
-  // its i+1 neighbours (S_y, S_z, fgrad^, erho^) are then on this thread's
-  // plane, and only S_x comes from plane i-1, read by this thread alone (the
-  // components at one point read no other plane's S_y, S_z, fgrad^ or erho^).
-  // It is zero on the plane i = Nx-1 (quirk Q2), written by the i = 0 threads.
-  if (p.i > 0) {
-    const A Sx = (A)sxq[idx - sI];
-    const A wx = p.i * qs0;  // vs_x[i-1]
-    const A Syp = (p.j > 0) ? (A)syq[idx - sJ] : A(0);
-    const A Szp = (p.k > 0) ? (A)szq[idx - 1] : A(0);
-    const A dot = wx * Sx + qy * Syp + qz * Szp;
-    const A inv = A(1) / (wx * wx + qy2 + qz2);
-    const A cm1 = scale * inv, cm2 = wx * inv;
-    const A gx = g3(p.i, p.j, p.k);
-    const A fg = (A)fgq[idx] + (A)erq[idx] * gx;
-    const long long ox = mo + idx - sI;
-    const A sol = (cm1 * rnu) * (Sx - cm2 * dot);
-    const A irr = (cm1 * rlam) * (wx * fg + cm2 * dot * gx);
-    vtx[ox] = (T)(sol + irr);
-    virrx[ox] = (T)irr;
-  } else {
-    const long long ox = mo + idx + (long long)(c.nx - 1) * sI;
-    vtx[ox] = (T)0;
-    virrx[ox] = (T)0;
+  const bool hx = p.i > 0, hy = p.j > 0, hz = p.k > 0;
+  // shifted reads outside the grid are 0, as on the first rank
+  const A Sx = hx ? (A)sxq_[mo + idx - sI] : A(0);
+  const A Sy = hy ? (A)syq_[mo + idx - sJ] : A(0);
+  const A Sz = hz ? (A)szq_[mo + idx - 1] : A(0);
+  const A q2 = qx * qx + qy * qy + qz * qz;
+  // |q|^2 = 0 only at (0, 0, 0), where no component is computed
+  const A inv = A(1) / (q2 > A(0) ? q2 : A(1));
+  const A cm1 = scale * inv;
+  const A G = exp(A(-1.57 * 1.57 * 2) * q2);
+  const A fg = (A)fgq_[mo + idx] + (A)erq_[mo + idx] * G;
+  const A dot = qx * Sx + qy * Sy + qz * Sz;
+  const A sn = cm1 * rnu, sl = cm1 * rlam;
+  // x component at (i-1, j, k) (S:1583-1709); the plane i = Nx-1 is zero
+  // (quirk Q2) and is written by the i = 0 threads
+  A ox = 0, rx = 0;
+  if (hx) {
+    const A cm2 = qx * inv;
+    ox = sn * (Sx - cm2 * dot);
+    rx = sl * (qx * fg + cm2 * dot * G);
   }
+  // y component at (i, j-1, k) and z component at (i, j, k-1) (S:1759-1974):
+  // computed only for j <= Ny-2 and k <= Nz-2 at their own points (quirk
+  // Q1, S:1760-1761); the line j = Ny-1 of v_y and the column k = Nz-1 of v_z
+  // are written (as 0) by the j = 0 and k = 0 threads
   A oy = 0, ry = 0, oz = 0, rz = 0;
-  if (p.j < c.ny - 1 && p.k < c.nz - 1) {  // quirk Q1, S:1760-1761
-    const A gy = g3(p.i, p.j + 1, p.k);
-    // y component
-    {
-      const A Sxp = (p.i > 0) ? (A)sxq[idx - sI + sJ] : A(0);
-      const A Szp = (p.k > 0) ? (A)szq[idx + sJ - 1] : A(0);
-      const A dot = qx * Sxp + wy * Sy + qz * Szp;
-      const A inv = A(1) / (qx2 + wy * wy + qz2);
-      const A cm1 = scale * inv, cm2 = wy * inv;
-      const A fg = (A)fgq[idx + sJ] + (A)erq[idx + sJ] * gy;
-      oy = (cm1 * rnu) * (Sy - cm2 * dot);
-      ry = (cm1 * rlam) * (wy * fg + cm2 * dot * gy);
+  if (hy && p.k < c.nz - 1) {
+    const A cm2 = qy * inv;
+    oy = sn * (Sy - cm2 * dot);
+    ry = sl * (qy * fg + cm2 * dot * G);
+  }
+  if (hz && p.j < c.ny - 1) {
+    const A cm2 = qz * inv;
+    // quirk Q3 (S:1932): at k = 0, j >= 1, except on the first plane of each
+    // reference slab, the dot term takes CM1y in its exponent: the y
+    // component's factor at the same point, G at cosine mode (i, j+1, 0)
+    const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
+    A gz = G;
+    if (p.k == 1 && p.j > 0 && (p.i % slab) != 0) {
+      const A wy = (p.j + 1) * qs1;
+      gz = exp(A(-1.57 * 1.57 * 2) * (qx * qx + wy * wy));
     }
-    // z component
-    {
-      const A Sxp = (p.i > 0) ? (A)sxq[idx - sI + 1] : A(0);
-      const A Syp = (p.j > 0) ? (A)syq[idx - sJ + 1] : A(0);
-      const A dot = qx * Sxp + qy * Syp + wz * Sz;
-      const A inv = A(1) / (qx2 + qy2 + wz * wz);
-      const A cm1 = scale * inv, cm2 = wz * inv;
-      const A gz = g3(p.i, p.j, p.k + 1);
-      const A fg = (A)fgq[idx + 1] + (A)erq[idx + 1] * gz;
-      oz = (cm1 * rnu) * (Sz - cm2 * dot);
-      // quirk Q3 (S:1932): CM1y in the exponent at k=0, j>=1, except the
-      // first plane of each reference slab (the y component's factor)
-      const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
-      const bool q3 = p.k == 0 && p.j > 0 && (p.i % slab) != 0;
-      rz = (cm1 * rlam) * (wz * fg + cm2 * dot * (q3 ? gy : gz));
-    }
+    oz = sn * (Sz - cm2 * dot);
+    rz = sl * (qz * fg + cm2 * dot * gz);
   }
   const long long o = mo + idx;
-  vty[o] = (T)(oy + ry); virry[o] = (T)ry;
-  vtz[o] = (T)(oz + rz); virrz[o] = (T)rz;
+  const long long pox = hx ? o - sI : o + (long long)(c.nx - 1) * sI;
+  const long long poy = hy ? o - sJ : o + (long long)(c.ny - 1) * sJ;
+  const long long poz = hz ? o - 1 : o + (c.nz - 1);
+  vtx[pox] = (T)(ox + rx);
+  vty[poy] = (T)(oy + ry);
+  vtz[poz] = (T)(oz + rz);
+  if constexpr (VIRR) {
+    virrx[pox] = (T)rx;
+    virry[poy] = (T)ry;
+    virrz[poz] = (T)rz;
+  }
+  // div v^(i, j, k) = sum_a Vq_a[n_a] virr_a(n - e_a), a term being 0 at n_a = 0
+  if (divq) divq[o] = (T)(qx * rx + qy * ry + qz * rz);
 }
 
 // ---------------------------------------------------------------------------
@@ -391,20 +390,30 @@ void StepKernels<T>::relax(const StepConsts& c, const T* amp, T* laprl, double* 
   hipLaunchKernelGGL(k_relax<T>, grid_of(c), dim3(kBlock), 0, s, c, amp, laprl, rho);
 }
 // 1024-point workgroups (four lines of 256) when they tile the field: the
-// j+-1 lines a point reads are then mostly read by its own workgroup, on the
+// j-1 lines a point reads are then mostly read by its own workgroup, on the
 // same CU at the same time
-template <typename T, typename A>
+template <typename T, typename A, bool VIRR>
 static void launch_velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq, const T* fgq,
-                            const T* erq, T* vx, T* vy, T* vz, T* virrx, T* virry, T* virrz, hipStream_t s) {
+                            const T* erq, T* vx, T* vy, T* vz, T* virrx, T* virry, T* virrz, T* divq,
+                            hipStream_t s) {
   constexpr int BS = 1024;
   const long long n3 = n3_of(c);
   if (n3 % BS == 0) {
-    hipLaunchKernelGGL((k_velocity<T, A, BS>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s, c, sxq,
-                       syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz);
+    hipLaunchKernelGGL((k_velocity<T, A, BS, VIRR>), dim3((unsigned)(n3 / BS), (unsigned)c.nmem), dim3(BS), 0, s,
+                       c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, divq);
   } else {
-    hipLaunchKernelGGL((k_velocity<T, A, kBlock>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq, erq,
-                       vx, vy, vz, virrx, virry, virrz);
+    hipLaunchKernelGGL((k_velocity<T, A, kBlock, VIRR>), grid_of(c), dim3(kBlock), 0, s, c, sxq, syq, szq, fgq,
+                       erq, vx, vy, vz, virrx, virry, virrz, divq);
   }
+}
+template <typename T, typename A>
+static void launch_velocity_any(const StepConsts& c, const T* sxq, const T* syq, const T* szq, const T* fgq,
+                                const T* erq, T* vx, T* vy, T* vz, T* virrx, T* virry, T* virrz, T* divq,
+                                hipStream_t s) {
+  if (virrx)
+    launch_velocity<T, A, true>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, divq, s);
+  else
+    launch_velocity<T, A, false>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, divq, s);
 }
 
 template <typename T>
@@ -418,14 +427,14 @@ void StepKernels<T>::forces(const StepConsts& c, const T* lpsi, const T* laprl, 
 template <typename T>
 void StepKernels<T>::velocity(const StepConsts& c, const T* sxq, const T* syq, const T* szq,
                               const T* fgq, const T* erq, T* vx, T* vy, T* vz, T* virrx,
-                              T* virry, T* virrz, bool arith32, hipStream_t s) {
+                              T* virry, T* virrz, T* divq, bool arith32, hipStream_t s) {
   if constexpr (sizeof(T) == 4) {
     if (arith32) {
-      launch_velocity<T, float>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, s);
+      launch_velocity_any<T, float>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, divq, s);
       return;
     }
   }
-  launch_velocity<T, double>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, s);
+  launch_velocity_any<T, double>(c, sxq, syq, szq, fgq, erq, vx, vy, vz, virrx, virry, virrz, divq, s);
 }
 template <typename T>
 void StepKernels<T>::density_nl(const StepConsts& c, const T* vx, const T* vy, const T* vz,
