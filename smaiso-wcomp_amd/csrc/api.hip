@@ -639,19 +639,36 @@ int run_step(smaiso_handle* h) {
     HIPTRY(h, h->xw(kCT, h->erho, h->erho));
     for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw(kST[a], S[a], S[a]));
   }
-  // a11 tail + a12 (S:1479-1974) and the spectral div v^ of a13 (S:2042-2063)
-  {
+  // a11 tail + a12 (S:1479-1974), the spectral div v^ of a13 (S:2042-2063)
+  // and the inverse transforms of div v and v = iST(vsol^ + virr^) (S:2085-
+  // 2117, by linearity): the z pass of the four transforms solves the
+  // velocity on load where the length allows; otherwise the solve stores v^
+  // and div v^ and the transforms follow.  The adjoint's recomputed steps
+  // also keep virr^ (the unfused path).
+  const bool vfused = !h->keep_virr && velocity_zpass_supported(h->g.nz);
+  if (vfused) {
+    VelocityIO<T> io;
+    for (int a = 0; a < 3; ++a) { io.S[a] = S[a]; io.zout[a] = v[a]; }
+    io.fgq = static_cast<const T*>(h->fgrad);
+    io.erq = static_cast<const T*>(h->erho);
+    io.zout[3] = static_cast<T*>(h->divv);
+    {
+      smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 9);
+      HIPTRY(h, velocity_zpass<T>(c, io, h->r2r, h->a32(SMAISO_STAGE_VELSOLVE) != 0,
+                                  h->a32(SMAISO_STAGE_DIVV) && h->a32(SMAISO_STAGE_VELOCITY), s));
+    }
+    HIPTRY(h, h->xw_yx(kICT, h->divv, h->a32(SMAISO_STAGE_DIVV)));
+  } else {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * (h->keep_virr ? 12 : 9));
     K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
                 vtq[0], vtq[1], vtq[2], h->keep_virr ? virrq[0] : nullptr, virrq[1], virrq[2],
                 static_cast<T*>(h->divv), h->a32(SMAISO_STAGE_VELSOLVE) != 0, s);
     if (h->keep_virr) h->virr_epoch = h->epoch;
   }
-  // a13 + a14 (S:2064-2117): div v = iCT(div v^) in place; div f =
-  // iCT(scale sum_a Vq_a shift(ST_a f_a)) is a sum of one-axis operators, one
-  // accumulating pass per component; v = iST(vsol^ + virr^) by linearity
+  if (!vfused) HIPTRY(h, h->xw(kICT, h->divv, h->divv, nullptr, h->a32(SMAISO_STAGE_DIVV)));
+  // div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) (S:2064-2092) is a sum of
+  // one-axis operators, one accumulating pass per component
   {
-    HIPTRY(h, h->xw(kICT, h->divv, h->divv, nullptr, h->a32(SMAISO_STAGE_DIVV)));
     Fuse d = fb;
     d.op = OP_AX_DIV;
     d.arith32 = h->a32(SMAISO_STAGE_DIVF);
@@ -661,8 +678,10 @@ int run_step(smaiso_handle* h) {
       HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, f[a], h->divf, d));
     }
   }
-  for (int a = 0; a < 3; ++a)
-    HIPTRY(h, h->xw(kIST[a], vtq[a], v[a], nullptr, h->a32(SMAISO_STAGE_VELOCITY)));
+  for (int a = 0; a < 3; ++a) {
+    if (vfused) HIPTRY(h, h->xw_yx(kIST[a], v[a], h->a32(SMAISO_STAGE_VELOCITY)));
+    else HIPTRY(h, h->xw(kIST[a], vtq[a], v[a], nullptr, h->a32(SMAISO_STAGE_VELOCITY)));
+  }
   // a15 + a16 (S:2121-2162) and the transform of Nl (S:2166-2168): the z
   // pass of the transform computes them on load where the length allows
   if (density_zpass_supported(h->g.nz)) {

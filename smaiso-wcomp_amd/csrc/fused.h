@@ -51,4 +51,24 @@ template <typename T>
 hipError_t density_zpass(const StepConsts& c, const DensityIO<T>& io, R2R& r2r, hipStream_t s);
 bool density_zpass_supported(int nz);
 
+// a11's fgrad^ += G erho^, the velocity solve a12 (S:1479-1974) and the
+// spectral div v^ of a13 (S:2042-2063) fused with the z passes of the inverse
+// transforms that consume them (S:2085-2117): v_x, v_y and div v (REDFT01
+// along z) and v_z (RODFT01).  The solve at a cosine mode gives the three
+// components one index back along their own axis (velocity.h), so the z
+// lines of v_x go one plane back and those of v_y one line back.  For FP32
+// storage the solve computes in FP32 when arith32 and the transforms when
+// tile32 (lengths with FP32 tables); FP64 otherwise.
+template <typename T>
+struct VelocityIO {
+  const T* S[3];        // spectral S_a^ (after their forward transforms)
+  const T *fgq, *erq;   // spectral fgrad^ and erho^
+  T* zout[4];           // z pass of v_x, v_y, v_z (vsol^ + virr^) and div v
+};
+
+template <typename T>
+hipError_t velocity_zpass(const StepConsts& c, const VelocityIO<T>& io, R2R& r2r, bool arith32, bool tile32,
+                          hipStream_t s);
+bool velocity_zpass_supported(int nz);
+
 }  // namespace smaiso

@@ -2,6 +2,9 @@
 #include "fused.h"
 
 #include "r2r_kernels.h"
+#include "velocity.h"
+
+#include <type_traits>
 
 namespace smaiso {
 
@@ -294,6 +297,216 @@ hipError_t density_zpass(const StepConsts& c, const DensityIO<T>& io, R2R& r2r, 
     default: return hipErrorNotSupported;
   }
 }
+
+namespace {
+
+constexpr int kVelOut = 4;  // v_x, v_y, v_z, div v
+
+template <int N, typename AT = double>
+struct VelocityTile {
+  static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
+  static constexpr int NT = 256;
+  static constexpr size_t lds = (size_t)kVelOut * P * seq_len(N, pick_maxr(N, kVelOut * P, NT)) * sizeof(cplx<AT>);
+  static constexpr bool fits = lds <= 65536;
+};
+
+// One workgroup: 2P consecutive z lines of cosine modes.  Compute phase:
+// every mode of the tile is solved (velocity_mode, in AV arithmetic) from
+// S_x one plane back, S_y one line back, S_z one mode back and fgrad^,
+// erho^ at the mode; its four outputs go to LDS as the inputs of Makhoul's
+// inverse (v_z at its reversed position: DST-III reads its input reversed).
+// Then the pre-twiddle in place, one inverse FFT over the 4P sequences and
+// the stores y_{2n} = u_n, y_{2n+1} = u_{N-1-n} (negated for v_z), each
+// field to its own line: v_x of cosine line (i, j) is line (i-1, j) (cosine
+// plane 0 gives the zero plane Nx-1), v_y is line (i, j-1) (cosine line j =
+// 0 gives the zero line Ny-1).  AV: the solve's arithmetic; AT: the tile's
+// and the tables' (FP64, or FP32 for FP32 storage where the transforms of
+// div v and v compute in FP32).
+template <typename T, typename AV, typename AT, int N, int P, int NT>
+__global__ __launch_bounds__(NT) void velocity_z_kernel(StepConsts c, VelocityIO<T> io,
+                                                        const cplx<AT>* __restrict__ tw,
+                                                        const cplx<AT>* __restrict__ mk) {
+  constexpr int NF = kVelOut, TL = 2 * P, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, NF * P, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using A = AT;
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
+  A* rbuf = reinterpret_cast<A*>(smem_raw);
+  const int tid = threadIdx.x;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
+  const long long mo = (long long)blockIdx.y * nl * N;
+  const long long base = mo + first * N;
+  const MemberConsts mc = c.mc[blockIdx.y];
+  StageTw<A, N, NF * P, NT, MAXR> stw;
+  stw.load(tw, tid);
+
+  constexpr int E = TL * N;
+  constexpr int IT = (E + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (E % NT == 0 || e < E) {
+      const int l = e / N, k = e - l * N;
+      A v[NF] = {A(0), A(0), A(0), A(0)};
+      if (l < nvalid) {
+        int i, j;
+        line_ij(first + l, c.ny, i, j);
+        const long long idx = base + e;
+        // shifted reads outside the grid are 0, as on the first rank
+        const AV Sx = i > 0 ? (AV)io.S[0][idx - (long long)c.ny * N] : AV(0);
+        const AV Sy = j > 0 ? (AV)io.S[1][idx - N] : AV(0);
+        const AV Sz = k > 0 ? (AV)io.S[2][idx - 1] : AV(0);
+        const VelMode<AV> r = velocity_mode<AV>(c, mc, i, j, k, Sx, Sy, Sz, (AV)io.fgq[idx], (AV)io.erq[idx]);
+        v[0] = (A)r.tx;
+        v[1] = (A)r.ty;
+        v[2] = (A)r.tz;
+        v[3] = (A)r.div;
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        // v_z of mode k sits at k-1 (k = 0: the zero at N-1), reversed: N-k
+        const int pos = f == 2 ? (k == 0 ? 0 : N - k) : k;
+        rbuf[2 * ((f * P + (l >> 1)) * SEQ + lp<SH>(pos)) + (l & 1)] = v[f];
+      }
+    }
+  }
+  __syncthreads();
+  // Makhoul's pre-twiddle, item k < N/2 in place at k and kb
+  constexpr int W = NF * P * HN;
+  constexpr int ITW = (W + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int sq = w / HN, k = w - sq * HN;
+      const int kb = (k == 0) ? HN : N - k;
+      cplx<A> zk, zb;
+      makhoul_pre<A, N>(mk, k, mk[k], buf[sq * SEQ + lp<SH>(k)], buf[sq * SEQ + lp<SH>(kb)], zk, zb);
+      buf[sq * SEQ + lp<SH>(k)] = zk;
+      buf[sq * SEQ + lp<SH>(kb)] = zb;
+    }
+  }
+  __syncthreads();
+  fft_stage<A, N, NF * P, NT, MAXR, +1, 0>(buf, tw, stw);
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int sq = w / HN, n = w - sq * HN;
+      const int f = sq / P, cc = sq - f * P;
+      const cplx<A> ua = buf[sq * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[sq * SEQ + lp<SH>(N - 1 - n)];
+      const A sg = f == 2 ? A(-1) : A(1);
+      T* const o = f == 0 ? io.zout[0] : f == 1 ? io.zout[1] : f == 2 ? io.zout[2] : io.zout[3];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int l = 2 * cc + q;
+        if (l < nvalid) {
+          const long long g = first + l;
+          int i, j;
+          line_ij(g, c.ny, i, j);
+          long long ol = g;
+          if (f == 0) ol = i > 0 ? g - c.ny : g + (long long)(c.nx - 1) * c.ny;
+          else if (f == 1) ol = j > 0 ? g - 1 : g + (c.ny - 1);
+          T* const d = o + mo + ol * N + 2 * n;
+          d[0] = (T)(q == 0 ? ua.x : ua.y);
+          d[1] = (T)(sg * (q == 0 ? ub.x : ub.y));
+        }
+      }
+    }
+  }
+}
+
+template <typename T, typename AV, typename AT, int N>
+hipError_t launch_velocity_z(const StepConsts& c, const VelocityIO<T>& io, const void* tw, const void* mk,
+                             hipStream_t s) {
+  using F = VelocityTile<N, AT>;
+  if constexpr (!F::fits) {
+    return hipErrorNotSupported;
+  } else {
+    constexpr int TL = 2 * F::P;
+    const long long blocks = ((long long)c.nx * c.ny + TL - 1) / TL;
+    hipLaunchKernelGGL((velocity_z_kernel<T, AV, AT, N, F::P, F::NT>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                       dim3(F::NT), F::lds, s, c, io, static_cast<const cplx<AT>*>(tw),
+                       static_cast<const cplx<AT>*>(mk));
+    return hipGetLastError();
+  }
+}
+
+template <typename T, typename AV, typename AT>
+hipError_t velocity_z_n(const StepConsts& c, const VelocityIO<T>& io, const void* tw, const void* mk,
+                        hipStream_t s) {
+  if constexpr (std::is_same<AT, float>::value) {
+    // the lengths with FP32 table and tile instances (r2r_arith32_supported)
+    switch (c.nz) {
+      case 64: return launch_velocity_z<T, AV, AT, 64>(c, io, tw, mk, s);
+      case 96: return launch_velocity_z<T, AV, AT, 96>(c, io, tw, mk, s);
+      case 128: return launch_velocity_z<T, AV, AT, 128>(c, io, tw, mk, s);
+      case 256: return launch_velocity_z<T, AV, AT, 256>(c, io, tw, mk, s);
+      case 512: return launch_velocity_z<T, AV, AT, 512>(c, io, tw, mk, s);
+      default: return hipErrorNotSupported;
+    }
+  } else {
+    switch (c.nz) {
+      case 16: return launch_velocity_z<T, AV, AT, 16>(c, io, tw, mk, s);
+      case 32: return launch_velocity_z<T, AV, AT, 32>(c, io, tw, mk, s);
+      case 48: return launch_velocity_z<T, AV, AT, 48>(c, io, tw, mk, s);
+      case 64: return launch_velocity_z<T, AV, AT, 64>(c, io, tw, mk, s);
+      case 96: return launch_velocity_z<T, AV, AT, 96>(c, io, tw, mk, s);
+      case 128: return launch_velocity_z<T, AV, AT, 128>(c, io, tw, mk, s);
+      case 192: return launch_velocity_z<T, AV, AT, 192>(c, io, tw, mk, s);
+      case 256: return launch_velocity_z<T, AV, AT, 256>(c, io, tw, mk, s);
+      case 384: return launch_velocity_z<T, AV, AT, 384>(c, io, tw, mk, s);
+      case 512: return launch_velocity_z<T, AV, AT, 512>(c, io, tw, mk, s);
+      default: return hipErrorNotSupported;
+    }
+  }
+}
+
+template <int N>
+constexpr bool vel_fits() { return VelocityTile<N>::fits; }
+
+}  // namespace
+
+bool velocity_zpass_supported(int nz) {
+  switch (nz) {
+    case 16: return vel_fits<16>();
+    case 32: return vel_fits<32>();
+    case 48: return vel_fits<48>();
+    case 64: return vel_fits<64>();
+    case 96: return vel_fits<96>();
+    case 128: return vel_fits<128>();
+    case 192: return vel_fits<192>();
+    case 256: return vel_fits<256>();
+    case 384: return vel_fits<384>();
+    case 512: return vel_fits<512>();
+    default: return false;
+  }
+}
+
+template <typename T>
+hipError_t velocity_zpass(const StepConsts& c, const VelocityIO<T>& io, R2R& r2r, bool arith32, bool tile32,
+                          hipStream_t s) {
+  if (!velocity_zpass_supported(c.nz)) return hipErrorNotSupported;
+  const bool t32 = sizeof(T) == 4 && tile32 && r2r_arith32_supported(c.nz);
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk, t32 ? F32 : F64);
+  if (e != hipSuccess) return e;
+  if constexpr (sizeof(T) == 4) {
+    if (t32) {
+      if (arith32) return velocity_z_n<T, float, float>(c, io, tw, mk, s);
+      return velocity_z_n<T, double, float>(c, io, tw, mk, s);
+    }
+    if (arith32) return velocity_z_n<T, float, double>(c, io, tw, mk, s);
+  }
+  return velocity_z_n<T, double, double>(c, io, tw, mk, s);
+}
+
+template hipError_t velocity_zpass<double>(const StepConsts&, const VelocityIO<double>&, R2R&, bool, bool, hipStream_t);
+template hipError_t velocity_zpass<float>(const StepConsts&, const VelocityIO<float>&, R2R&, bool, bool, hipStream_t);
 
 template hipError_t density_zpass<double>(const StepConsts&, const DensityIO<double>&, R2R&, hipStream_t);
 template hipError_t density_zpass<float>(const StepConsts&, const DensityIO<float>&, R2R&, hipStream_t);

@@ -119,9 +119,9 @@ class R2R {
   // FP64 for both storage types): tw[m] = exp(-2 pi i m/n), mk[m] =
   // exp(-i pi m/(2n)); for kernels outside R2R that run an FFT pass of their
   // own.
-  hipError_t table_ptrs(int n, const void** tw, const void** mk) {
+  hipError_t table_ptrs(int n, const void** tw, const void** mk, DType dt = F64) {
     Tables* t = nullptr;
-    hipError_t e = tables(F64, n, &t);
+    hipError_t e = tables(dt, n, &t);
     if (e != hipSuccess) return e;
     *tw = t->tw;
     *mk = t->mk;

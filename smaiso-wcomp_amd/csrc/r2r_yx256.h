@@ -233,6 +233,188 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Ar
   }
 }
 
+// One-axis operator passes (OP_AX_GRAD, OP_AX_DIV, OP_AX_GAUSS, OP_AX_GRAD_T,
+// OP_AX_DIV_T) of length 256 along y and x on the same layout: forward
+// transform, operator, inverse transform per tile with three LDS exchanges.
+//  * forward: Makhoul-ordered loads, first butterfly, exchange, second
+//    butterfly: the thread holds Z[k], k = j + 16 q;
+//  * Z[k] goes back to its own slots and Z[256 - k] comes from the partner's:
+//    the thread forms the forward outputs at k and 256 - k, and since the
+//    one-mode shifts of the operators compose with the DST index reversal,
+//    the operator outputs that Makhoul's inverse pre-twiddle of z_k needs
+//    (its inputs at k and 256 - k) take their sources among those two modes:
+//    the operator and the pre-twiddle run in registers, for the k of the
+//    thread's first inverse butterfly;
+//  * inverse: first butterfly, exchange, second butterfly, stores y_{2n} =
+//    u_n, y_{2n+1} = u_{255-n} (added to the output for OP_AX_DIV with
+//    accumulate).
+// Persistent workgroups load the next tile, in the storage type, while
+// transforming the current one.
+template <typename T, typename A>
+constexpr int ax_min_waves() {
+  return 2;
+}
+
+template <typename T, typename TO, int AXIS, int OP, int P, typename A = double>
+__global__ __launch_bounds__(16 * P, (ax_min_waves<T, A>())) void axop_kernel(Args<T, TO> a, Fuse f) {
+  using C = cplx<A>;
+  constexpr int N = kN;
+  // forward DST-II (sine on the load side) and inverse DST-III (sine on the
+  // store side)
+  constexpr bool FS = OP == OP_AX_DIV || OP == OP_AX_GRAD_T;
+  constexpr bool IS = OP == OP_AX_GRAD || OP == OP_AX_DIV_T;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* const buf = reinterpret_cast<C*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % P, j = tid / P;
+  C* const s = buf + c * kSeq;
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);
+  const int stride = AXIS == 1 ? a.nz : a.ny * a.nz;
+  const C* const tw = reinterpret_cast<const C*>(a.tw);
+  const C* const mkt = reinterpret_cast<const C*>(a.mk);
+  const C wf = tw[j];
+  const C wi = C{wf.x, -wf.y};
+  const C mk0 = mkt[j], mks = mkt[16];
+  // the operators that may add to their output
+  constexpr bool kAcc = OP == OP_AX_DIV || OP == OP_AX_GRAD_T || OP == OP_AX_DIV_T;
+  const bool acc = kAcc && f.accumulate != 0;
+  // operator weights: c0/(2N) and the wavenumber spacing along this axis
+  // (set: the launcher sends passes without it to the general kernel)
+  const double cw = f.c0 / (2.0 * N), qs = f.qs[AXIS];
+
+  auto tile = [&](int t) -> long long {
+    int o, kc;
+    if (AXIS == 1) { kc = t / a.nx; o = t - kc * a.nx; }
+    else { o = t / a.tpo; kc = t - o * a.tpo; }
+    return mo + (AXIS == 1 ? (long long)o * a.ny * a.nz : (long long)o * a.nz) + kc * 2 * P;
+  };
+  // Makhoul-ordered loads of the forward transform, in the storage type
+  auto load = [&](cplx<T>* v, int t, int jj) {
+    const T* p = a.in + tile(t) + 2 * c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int row = q < 8 ? 2 * jj + 32 * q : 2 * N - 1 - 2 * jj - 32 * q;
+      v[q] = *reinterpret_cast<const cplx<T>*>(p + row * stride);
+    }
+  };
+
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    C wf_t = wf, wi_t = wi, mk0_t = mk0, mks_t = mks;
+    int j_t = j;
+    // per-tile recomputation of the seeds' powers, the operator weights and
+    // the row offsets: hoisted out of the tile loop they would hold a large
+    // part of the register file
+    asm volatile("" : "+v"(j_t), "+v"(wf_t.x), "+v"(wf_t.y), "+v"(wi_t.x), "+v"(wi_t.y));
+    asm volatile("" : "+v"(mk0_t.x), "+v"(mk0_t.y), "+v"(mks_t.x), "+v"(mks_t.y));
+    C v[16];
+    {
+      cplx<T> ld[16];
+      load(ld, t, j_t);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = C{(A)ld[q].x, (A)ld[q].y};
+    }
+    // ---- forward: DST-II negates the odd rows (q >= 8)
+    if constexpr (FS) {
+#pragma unroll
+      for (int q = 8; q < 16; ++q) { v[q].x = -v[q].x; v[q].y = -v[q].y; }
+    }
+    dft<A, 16, -1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+    stage_twiddle(v, wf_t);
+    dft<A, 16, -1>(v);
+    // v[q] = Z[j + 16 q], into the slots this thread read
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
+    __syncthreads();
+    // ---- forward outputs at k and N - k, operator, inverse pre-twiddle
+    C m = mk0_t;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = j_t + 16 * q;
+      const int kn = (N - k) & (N - 1);  // partner; k = 0 and k = 128 pair with themselves
+      const C za = v[q], pa = s[slot(kn)];
+      // Makhoul outputs (both lines of the pair): Y at k with mk[k], and at
+      // N - k with mk[N - k] = (-mk[k].y, -mk[k].x)
+      const C mb = k == 0 ? m : C{-m.y, -m.x};
+      const C ya = {m.x * (za.x + pa.x) - m.y * (za.y - pa.y), m.x * (za.y + pa.y) + m.y * (za.x - pa.x)};
+      const C yb = {mb.x * (pa.x + za.x) - mb.y * (pa.y - za.y), mb.x * (pa.y + za.y) + mb.y * (pa.x - za.x)};
+      // operator outputs at inverse-input positions k (rk) and N - k (rn;
+      // Y_N = 0 for k = 0).  Position r is mode p = r (N - 1 - r after the
+      // DST-III reversal); its source mode p + 1 (GRAD, DIV_T), p - 1 (DIV,
+      // GRAD_T) or p (GAUSS) sits at forward Makhoul index N - k for r = k
+      // and at k for r = N - k in the shifting operators (the shift composes
+      // with one DST reversal), at r itself for GAUSS.  Weights c0/(2N) Vq[.]
+      // with Vq[i] = i qs; mode k = 0 of a shifting operator has no source.
+      C rk, rn;
+      {
+        const double dk = (double)k * qs, dn = (double)(N - k) * qs;
+        double wk, wn;
+        if constexpr (OP == OP_AX_GRAD) { wk = -cw * dn; wn = -cw * dk; }
+        else if constexpr (OP == OP_AX_DIV) { wk = cw * dk; wn = cw * dn; }
+        else if constexpr (OP == OP_AX_GRAD_T) { wk = -cw * dk; wn = -cw * dn; }
+        else if constexpr (OP == OP_AX_DIV_T) { wk = cw * dn; wn = cw * dk; }
+        else {
+          wk = cw * exp(-1.57 * 1.57 * dk * dk * 2);
+          wn = cw * exp(-1.57 * 1.57 * dn * dn * 2);
+        }
+        constexpr bool kShift = OP != OP_AX_GAUSS;
+        const C sk = kShift ? yb : ya, sn = kShift ? ya : yb;
+        rk = C{(A)(wk * (double)sk.x), (A)(wk * (double)sk.y)};
+        rn = C{(A)(wn * (double)sn.x), (A)(wn * (double)sn.y)};
+        if (k == 0) {
+          if constexpr (kShift) rk = C{A(0), A(0)};
+          rn = C{A(0), A(0)};
+        }
+      }
+      // z_k = conj(mk[k]) (Y0_k + Y1_{N-k} + i (Y1_k - Y0_{N-k}))
+      v[q] = cmul(C{m.x, -m.y}, C{rk.x + rn.y, rk.y - rn.x});
+      m = cmul(m, mks_t);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // ---- inverse: first butterfly, exchange, second butterfly, store
+    dft<A, 16, +1>(v);
+    __syncthreads();  // every partner read of this tile is done
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+    stage_twiddle(v, wi_t);
+    dft<A, 16, +1>(v);
+    // u_n goes to row 2n (q < 8) or 2N - 1 - 2n (q >= 8), negated there for
+    // DST-III; an accumulating pass reads the old outputs in two batches of
+    // eight, each batch's loads issued together
+    TO* const o = a.out + tile(t) + 2 * c;
+    auto row = [&](int q) { const int n = j_t + 16 * q; return q < 8 ? 2 * n : 2 * N - 1 - 2 * n; };
+    if (kAcc && acc) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        cplx<TO> old[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) old[q] = *reinterpret_cast<const cplx<TO>*>(o + row(8 * h + q) * stride);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const C u = v[8 * h + q];
+          const double sg = (IS && h == 1) ? -1.0 : 1.0;
+          *reinterpret_cast<cplx<TO>*>(o + row(8 * h + q) * stride) =
+              cplx<TO>{(TO)((double)old[q].x + sg * (double)u.x), (TO)((double)old[q].y + sg * (double)u.y)};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const C u = (IS && q >= 8) ? C{-v[q].x, -v[q].y} : v[q];
+        *reinterpret_cast<cplx<TO>*>(o + row(q) * stride) = cplx<TO>{(TO)u.x, (TO)u.y};
+      }
+    }
+    __syncthreads();  // the next tile's first writes reuse slots other threads read last
+  }
+}
+
 // Sequences per tile: 16 columns of FP64 or 32 of FP32 (128-byte row
 // segments).
 template <typename T>

@@ -33,13 +33,52 @@ hipError_t launch(const PassIO& io, const Geom& g, int kind, const void* tw, con
   return hipGetLastError();
 }
 
-// plain passes only: the one-axis and x-pass operators stay on the general
-// pass kernel
+template <typename T, typename TO, int AXIS, int OP, typename A>
+hipError_t launch_op(const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
+  constexpr int P = yx256::tile_p<T>();
+  yx256::Args<T, TO> a;
+  a.in = static_cast<const T*>(io.in);
+  a.out = static_cast<TO*>(io.out);
+  a.tw = tw;
+  a.mk = mk;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.nz = g.nz;
+  a.kind = REDFT10;  // unused by the operators
+  a.tpo = g.nz / (2 * P);
+  a.ntiles = (AXIS == 1 ? g.nx : g.ny) * a.tpo;
+  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<A>);
+  // resident workgroups per CU (LDS bound), twice that many in the grid
+  const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+  const int grid = std::min(a.ntiles, 256 * per_cu * 2);
+  hipLaunchKernelGGL((yx256::axop_kernel<T, TO, AXIS, OP, P, A>), dim3((unsigned)grid, (unsigned)g.nb),
+                     dim3(16 * P), lds, s, a, io.f);
+  return hipGetLastError();
+}
+
+// plain passes and the one-axis operators (the x pass's mid operators stay
+// on the general pass kernel); FP32 arithmetic and the narrowing (FP64 ->
+// FP32) pass have the operators the step runs in them
 template <typename T, typename TO, int AXIS, typename A>
 hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
-  if (io.f.op != OP_PLAIN) return hipErrorNotSupported;
-  if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, A>(io, g, kind, tw, mk, s);
-  return launch<T, TO, AXIS, yx256::kInv, A>(io, g, kind, tw, mk, s);
+  const int op = io.f.op;
+  if (op == OP_PLAIN) {
+    if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, A>(io, g, kind, tw, mk, s);
+    return launch<T, TO, AXIS, yx256::kInv, A>(io, g, kind, tw, mk, s);
+  }
+  if (io.f.axis != AXIS || io.f.qs[AXIS] == 0.0) return hipErrorNotSupported;
+  constexpr bool kSame = std::is_same<T, TO>::value;
+  constexpr bool kFull = kSame && std::is_same<T, double>::value;
+  if (op == OP_AX_GRAD) return launch_op<T, TO, AXIS, OP_AX_GRAD, A>(io, g, tw, mk, s);
+  if constexpr (kSame) {
+    if (op == OP_AX_DIV) return launch_op<T, TO, AXIS, OP_AX_DIV, A>(io, g, tw, mk, s);
+    if (op == OP_AX_GAUSS) return launch_op<T, TO, AXIS, OP_AX_GAUSS, A>(io, g, tw, mk, s);
+  }
+  if constexpr (kFull) {
+    if (op == OP_AX_GRAD_T) return launch_op<T, TO, AXIS, OP_AX_GRAD_T, A>(io, g, tw, mk, s);
+    if (op == OP_AX_DIV_T) return launch_op<T, TO, AXIS, OP_AX_DIV_T, A>(io, g, tw, mk, s);
+  }
+  return hipErrorNotSupported;
 }
 
 template <typename T, typename TO, typename A = double>

@@ -9,6 +9,7 @@
 // aLin, C1, C2, CM1a, CM2a) are evaluated in registers from the per-axis
 // wavenumber vectors.
 #include "stepper.h"
+#include "velocity.h"
 
 #include <cmath>
 
@@ -114,21 +115,10 @@ __global__ __launch_bounds__(kBlock) void k_forces(
 // a11 (fgrad^ += erho^ G) fused with a12: velocity solve (S:1479-1974), and
 // the spectral divergence of virr^ (S:2042-2063)
 // ---------------------------------------------------------------------------
-// Thread (i, j, k) computes the x component at (i-1, j, k), the y component at
-// (i, j-1, k) and the z component at (i, j, k-1).  On their staggered lattices
-// these three points carry the wavenumbers of cosine mode (i, j, k): vs_x[i-1]
-// = vq_x[i] and likewise on y and z.  The shifted reads of the three solves
-// (x: S_y(i+1, j-1, k), S_z(i+1, j, k-1), fgrad^(i+1, j, k); y: S_x(i-1, j+1,
-// k), S_z(i, j+1, k-1), fgrad^(i, j+1, k); z: S_x(i-1, j, k+1), S_y(i, j-1,
-// k+1), fgrad^(i, j, k+1)) then all land on the same four values S_x(i-1, j,
-// k), S_y(i, j-1, k), S_z(i, j, k-1) and fgrad^(i, j, k), so the three
-// components share one projection q.S^, one denominator |q|^2 and one
-// Gaussian G(q) (a component's factor exp(-1.57^2 2 |q_a|^2) on its own
-// lattice is G at cosine mode (i, j, k), which is also the G of fgrad^ += G
-// erho^).  The same thread holds virr_x(i-1, j, k), virr_y(i, j-1, k) and
-// virr_z(i, j, k-1), which are exactly the terms of div v^(i, j, k) =
-// sum_a Vq_a virr_a(n - e_a) (S:2042-2063), so the divergence is formed here and
-// virr^ need not be stored.  Each thread reads five values and writes four.
+// Thread (i, j, k) solves cosine mode (i, j, k) (velocity_mode): the x
+// component at (i-1, j, k), the y component at (i, j-1, k), the z component
+// at (i, j, k-1) and div v^(i, j, k), so virr^ need not be stored.  Each thread
+// reads five values and writes four.
 //
 // A: the arithmetic type (FP64; FP32 for FP32 storage where the velocity
 // solve's FP32 stage is asked for).  VIRR: also store virr^ (for the vsol and
@@ -160,70 +150,25 @@ __global__ __launch_bounds__(BS) void k_velocity(
   const MemberConsts mm = c.mc[blockIdx.y];
   const Idx p = decompose(idx, c.ny, c.nz);
   const unsigned sI = (unsigned)c.ny * c.nz, sJ = (unsigned)c.nz;
-  // wavenumbers from the spacings (no table loads: this kernel is bound by its
-  // memory instructions)
-  const A qs0 = (A)c.qs[0], qs1 = (A)c.qs[1], qs2 = (A)c.qs[2], scale = (A)c.scale;
-  const A qx = p.i * qs0, qy = p.j * qs1, qz = p.k * qs2;
-  const A rnu = mm.rnu, rlam = mm.rlam;
   const bool hx = p.i > 0, hy = p.j > 0, hz = p.k > 0;
   // shifted reads outside the grid are 0, as on the first rank
   const A Sx = hx ? (A)sxq_[mo + idx - sI] : A(0);
   const A Sy = hy ? (A)syq_[mo + idx - sJ] : A(0);
   const A Sz = hz ? (A)szq_[mo + idx - 1] : A(0);
-  const A q2 = qx * qx + qy * qy + qz * qz;
-  // |q|^2 = 0 only at (0, 0, 0), where no component is computed
-  const A inv = A(1) / (q2 > A(0) ? q2 : A(1));
-  const A cm1 = scale * inv;
-  const A G = exp(A(-1.57 * 1.57 * 2) * q2);
-  const A fg = (A)fgq_[mo + idx] + (A)erq_[mo + idx] * G;
-  const A dot = qx * Sx + qy * Sy + qz * Sz;
-  const A sn = cm1 * rnu, sl = cm1 * rlam;
-  // x component at (i-1, j, k) (S:1583-1709); the plane i = Nx-1 is zero
-  // (quirk Q2) and is written by the i = 0 threads
-  A ox = 0, rx = 0;
-  if (hx) {
-    const A cm2 = qx * inv;
-    ox = sn * (Sx - cm2 * dot);
-    rx = sl * (qx * fg + cm2 * dot * G);
-  }
-  // y component at (i, j-1, k) and z component at (i, j, k-1) (S:1759-1974):
-  // computed only for j <= Ny-2 and k <= Nz-2 at their own points (quirk
-  // Q1, S:1760-1761); the line j = Ny-1 of v_y and the column k = Nz-1 of v_z
-  // are written (as 0) by the j = 0 and k = 0 threads
-  A oy = 0, ry = 0, oz = 0, rz = 0;
-  if (hy && p.k < c.nz - 1) {
-    const A cm2 = qy * inv;
-    oy = sn * (Sy - cm2 * dot);
-    ry = sl * (qy * fg + cm2 * dot * G);
-  }
-  if (hz && p.j < c.ny - 1) {
-    const A cm2 = qz * inv;
-    // quirk Q3 (S:1932): at k = 0, j >= 1, except on the first plane of each
-    // reference slab, the dot term takes CM1y in its exponent: the y
-    // component's factor at the same point, G at cosine mode (i, j+1, 0)
-    const int slab = c.slab_n0 > 0 ? c.slab_n0 : c.nx;
-    A gz = G;
-    if (p.k == 1 && p.j > 0 && (p.i % slab) != 0) {
-      const A wy = (p.j + 1) * qs1;
-      gz = exp(A(-1.57 * 1.57 * 2) * (qx * qx + wy * wy));
-    }
-    oz = sn * (Sz - cm2 * dot);
-    rz = sl * (qz * fg + cm2 * dot * gz);
-  }
+  const VelMode<A> r = velocity_mode<A>(c, mm, p.i, p.j, p.k, Sx, Sy, Sz, (A)fgq_[mo + idx], (A)erq_[mo + idx]);
   const long long o = mo + idx;
   const long long pox = hx ? o - sI : o + (long long)(c.nx - 1) * sI;
   const long long poy = hy ? o - sJ : o + (long long)(c.ny - 1) * sJ;
   const long long poz = hz ? o - 1 : o + (c.nz - 1);
-  vtx[pox] = (T)(ox + rx);
-  vty[poy] = (T)(oy + ry);
-  vtz[poz] = (T)(oz + rz);
+  vtx[pox] = (T)r.tx;
+  vty[poy] = (T)r.ty;
+  vtz[poz] = (T)r.tz;
   if constexpr (VIRR) {
-    virrx[pox] = (T)rx;
-    virry[poy] = (T)ry;
-    virrz[poz] = (T)rz;
+    virrx[pox] = (T)r.rx;
+    virry[poy] = (T)r.ry;
+    virrz[poz] = (T)r.rz;
   }
-  // div v^(i, j, k) = sum_a Vq_a[n_a] virr_a(n - e_a), a term being 0 at n_a = 0
-  if (divq) divq[o] = (T)(qx * rx + qy * ry + qz * rz);
+  if (divq) divq[o] = (T)r.div;
 }
 
 // ---------------------------------------------------------------------------
