@@ -84,22 +84,32 @@ __device__ __forceinline__ void stage_twiddle(cplx<A>* v, cplx<A> w1) {
   }
 }
 
-// Register cap, as waves per SIMD, for FP64 storage.  The FP64 tile's LDS
-// (8 sequences, 35 KiB, 128 threads) allows four workgroups per CU, two
-// waves per SIMD; the forward pass fits that without spilling, the inverse
-// pass (one more LDS exchange) keeps the full register file.  FP32 tiles
-// (16 sequences, 256 threads) measured fastest with the full register file
-// and the tables held across tiles.
-template <typename T, int MODE>
+// Register cap, as waves per SIMD, and whether the pass loads the next tile
+// while it transforms the current one (measured at 256^3 on MI355X).  An FP64
+// tile of 8 sequences (35 KiB LDS, 128 threads) allows four workgroups per
+// CU, two waves per SIMD; an FP64 tile of 16 sequences over FP32 storage (70
+// KiB, 256 threads) two workgroups, two waves per SIMD.  FP64 arithmetic: the
+// forward pass fits two waves with the prefetch (FP32 storage: y pass 44 ->
+// 36 us against one wave); the inverse pass (one more LDS exchange) keeps
+// the full register file and the prefetch for FP64 storage (50 us against 53
+// at two waves without prefetch) and drops the prefetch for FP32 storage
+// (two waves: 43 -> 37-42 us; with the prefetch at two waves it spilled).
+// FP32 arithmetic (16 sequences, 256 threads) measured fastest with the full
+// register file and the tables held across tiles.
+template <typename T, int MODE, typename A>
 constexpr int min_waves() {
-  return sizeof(T) == 8 && MODE == kFwd ? 2 : 1;
+  return sizeof(A) == 8 && !(MODE == kInv && sizeof(T) == 8) ? 2 : 1;
+}
+template <typename T, int MODE, typename A>
+constexpr bool pass_prefetch() {
+  return !(MODE == kInv && sizeof(A) == 8 && sizeof(T) == 4);
 }
 
 // AXIS: 1 = y pass, 0 = x pass.  MODE: kFwd or kInv.  P sequences (2P
 // columns) per tile.  A: the arithmetic type of the tile and the tables
 // (FP64, or FP32 for FP32 storage where FP32 arithmetic is asked for).
 template <typename T, typename TO, int AXIS, int MODE, int P, typename A = double>
-__global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Args<T, TO> a) {
+__global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel(Args<T, TO> a) {
   using C = cplx<A>;
   constexpr int N = kN;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -126,30 +136,38 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Ar
     else { o = t / a.tpo; kc = t - o * a.tpo; }
     return mo + (AXIS == 1 ? (long long)o * a.ny * a.nz : (long long)o * a.nz) + kc * 2 * P;
   };
-  // the 16 elements a thread loads for tile t
-  auto load = [&](C* v, int t) {
+  // the 16 elements a thread loads for tile t, in the storage type
+  auto load = [&](cplx<T>* v, int t) {
     const T* p = a.in + tile(t) + 2 * c;
     if constexpr (MODE == kInv) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int k = j + 16 * q;
-        v[q] = ld2<A>(p + (sine ? N - 1 - k : k) * stride);
+        v[q] = *reinterpret_cast<const cplx<T>*>(p + (sine ? N - 1 - k : k) * stride);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int row = q < 8 ? 2 * j + 32 * q : 2 * N - 1 - 2 * j - 32 * q;
-        v[q] = ld2<A>(p + row * stride);
+        v[q] = *reinterpret_cast<const cplx<T>*>(p + row * stride);
       }
     }
   };
 
-  C v[16], nxt[16];
+  // the next tile is held in the storage type until it is transformed; the
+  // FP64-arithmetic inverse pass over FP32 storage loads each tile at its
+  // start instead (two waves per SIMD without spilling the prefetch)
+  constexpr bool kPrefetch = pass_prefetch<T, MODE, A>();
+  C v[16];
+  cplx<T> nxt[16];
   int t = blockIdx.x;
-  if (t < a.ntiles) load(v, t);
+  if (kPrefetch && t < a.ntiles) load(nxt, t);
   for (; t < a.ntiles; t += gridDim.x) {
+    if (!kPrefetch) load(nxt, t);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = C{(A)nxt[q].x, (A)nxt[q].y};
     const int tn = t + gridDim.x;
-    if (tn < a.ntiles) load(nxt, tn);
+    if (kPrefetch && tn < a.ntiles) load(nxt, tn);
     // The twiddle powers and Makhoul factors of a thread are the same for
     // every tile; held across the loop in FP64 they would take about half
     // the register file.  Passing their seeds through an empty asm statement
@@ -157,7 +175,7 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Ar
     // recomputed per tile (a few dozen ALU operations).
     C wf_t = wf, wi_t = wi, mk0_t = mk0, mks_t = mks;
     int j_t = j;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(A) == 8) {
       asm volatile("" : "+v"(j_t), "+v"(wf_t.x), "+v"(wf_t.y), "+v"(wi_t.x), "+v"(wi_t.y));
       asm volatile("" : "+v"(mk0_t.x), "+v"(mk0_t.y), "+v"(mks_t.x), "+v"(mks_t.y));
     }
@@ -173,7 +191,7 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Ar
         const int k = j_t + 16 * q;
         // FP64: both from LDS, so the loaded values need no registers past
         // the barrier
-        const C rk = sizeof(T) == 8 ? s[slot(k)] : v[q];
+        const C rk = sizeof(A) == 8 ? s[slot(k)] : v[q];
         const C rn = k == 0 ? C{A(0), A(0)} : s[slot((N - k) & (N - 1))];
         v[q] = cmul(C{m.x, -m.y}, C{rk.x + rn.y, rk.y - rn.x});
         m = cmul(m, mks_t);
@@ -226,10 +244,6 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE>())) void pass_kernel(Ar
       }
     }
     __syncthreads();  // the next tile's first writes reuse slots other threads read last
-    if (tn < a.ntiles) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = nxt[q];
-    }
   }
 }
 
