@@ -565,7 +565,12 @@ int run_step(smaiso_handle* h) {
     if (h->mixed) HIPTRY(h, h->axpass_cast(F64, F32, a, h->rho, rd[a], g));
     else HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
   }
-  for (int a = 0; a < 3; ++a) {
+  // grad psi along x and y, then one z pass forms d_z psi, |amp| at the
+  // same points and the z factor of a7's separable Gaussian filter (FP64
+  // arithmetic; both stages' passes compute in FP64 unless FP32 is asked for
+  // either, which takes the separate passes)
+  const bool afused = amp_zpass_supported(h->g.nz) && !h->a32(SMAISO_STAGE_GRAD_PSI) && !h->a32(SMAISO_STAGE_AMP);
+  for (int a = 0; a < (afused ? 2 : 3); ++a) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
@@ -577,12 +582,26 @@ int run_step(smaiso_handle* h) {
   {
     const DType dt = h->mixed ? F32 : F64;
     Fuse g = fb;
-    g.op = OP_AX_GAUSS_AMP;
-    g.axis = 2;
-    g.arith32 = h->a32(SMAISO_STAGE_AMP);
-    g.src[0] = h->psi; g.src[1] = gp[0]; g.src[2] = gp[1]; g.src[3] = gp[2];
-    HIPTRY(h, h->axpass(dt, 2, nullptr, h->amp, g));
+    if (afused) {
+      AmpIO<T> io;
+      io.psi = static_cast<const T*>(h->psi);
+      io.gpx = gp[0];
+      io.gpy = gp[1];
+      io.gpz = gp[2];
+      io.amp = static_cast<T*>(h->amp);
+      io.q02 = fb.q02;
+      io.qs = fb.qs[2];
+      smaiso_handle::Scope sc(h, SMAISO_KC_R2R_AXIS, fb1 * 5);
+      HIPTRY(h, amp_zpass<T>(c, io, h->r2r, s));
+    } else {
+      g.op = OP_AX_GAUSS_AMP;
+      g.axis = 2;
+      g.arith32 = h->a32(SMAISO_STAGE_AMP);
+      g.src[0] = h->psi; g.src[1] = gp[0]; g.src[2] = gp[1]; g.src[3] = gp[2];
+      HIPTRY(h, h->axpass(dt, 2, nullptr, h->amp, g));
+    }
     g.op = OP_AX_GAUSS;
+    g.arith32 = h->a32(SMAISO_STAGE_AMP);
     for (int a = 1; a >= 0; --a) {
       g.axis = a;
       HIPTRY(h, h->axpass(dt, a, h->amp, h->amp, g));

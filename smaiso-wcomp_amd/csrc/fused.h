@@ -71,4 +71,22 @@ hipError_t velocity_zpass(const StepConsts& c, const VelocityIO<T>& io, R2R& r2r
                           hipStream_t s);
 bool velocity_zpass_supported(int nz);
 
+// The z component of grad psi (a6, S:1104-1250: a one-axis pass DCT-II,
+// shift by -Vs, DST-III along z) and the amplitude a7 (S:1254-1289) in one z
+// pass: the pass computes d_z psi, stores it, forms |amp| = sqrt(q0^2 psi^2
+// + |grad psi|^2) at the same points and applies the z factor of the
+// separable Gaussian filter (DCT-II, exp(-1.57^2 2 Vq^2)/(2N), DCT-III)
+// before storing amp.  FP64 arithmetic for both storage types.
+template <typename T>
+struct AmpIO {
+  const T *psi, *gpx, *gpy;
+  T *gpz, *amp;
+  double q02;  // q0^2
+  double qs;   // wavenumber spacing pi/(dz Nz)
+};
+
+template <typename T>
+hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, hipStream_t s);
+bool amp_zpass_supported(int nz);
+
 }  // namespace smaiso

@@ -508,6 +508,216 @@ hipError_t velocity_zpass(const StepConsts& c, const VelocityIO<T>& io, R2R& r2r
 template hipError_t velocity_zpass<double>(const StepConsts&, const VelocityIO<double>&, R2R&, bool, bool, hipStream_t);
 template hipError_t velocity_zpass<float>(const StepConsts&, const VelocityIO<float>&, R2R&, bool, bool, hipStream_t);
 
+namespace {
+
+// One workgroup: P consecutive z line pairs (the z tile of the one-axis
+// passes).  Two one-axis operators on the same LDS tile, with a pointwise
+// step between them:
+//  1. psi in Makhoul's forward order, FFT, Makhoul post, the gradient
+//     operator y[n] = -Vs[n] Y[n+1]/(2N) and the inverse pre-twiddle in
+//     place, inverse FFT: item n holds d_z psi at 2n and 2n+1 of both lines
+//     (DST-III: the odd output negated), stored to gpz;
+//  2. the item reads psi, grad_x psi, grad_y psi at the same four points and
+//     forms |amp| (the expression of the amplitude load of the z pass);
+//     it writes the values back to its own two slots in Makhoul's forward
+//     order (2n -> n, 2n+1 -> N-1-n), which it alone read;
+//  3. FFT, Makhoul post, exp(-1.57^2 2 Vq[n]^2)/(2N) and the pre-twiddle,
+//     inverse FFT, stores amp.
+template <typename T, int N, int P, int NT>
+__global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, const cplx<double>* __restrict__ tw,
+                                                   const cplx<double>* __restrict__ mk) {
+  using A = double;
+  constexpr int TL = 2 * P, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, P, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nlines - first) < TL ? (nlines - first) : TL);
+  const long long base = (long long)blockIdx.y * nlines * N + first * N;
+  StageTw<A, N, P, NT, MAXR> stw;
+  stw.load(tw, tid);
+  const MkSeq<A, N, P, NT> ms0(mk, 2, tid);
+  const A cw = A(1) / (2.0 * N);
+  auto vq = [&](int i) -> A { return (A)i * io.qs; };
+
+  // 1. psi -> LDS (DCT-II input order: v[m/2] = x[m] for even m, v[N-1-(m-1)/2] = x[m] for odd m)
+  constexpr int E = P * N;
+  constexpr int IT = (E + NT - 1) / NT;
+  {
+    cplx<A> v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      const int c = e / N, m = e - c * N;
+      v[it] = cplx<A>{A(0), A(0)};
+      if (E % NT == 0 || e < E) {
+        const int l0 = 2 * c;
+        if (l0 < nvalid) v[it].x = (A)io.psi[base + (long long)l0 * N + m];
+        if (l0 + 1 < nvalid) v[it].y = (A)io.psi[base + (long long)(l0 + 1) * N + m];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (E % NT == 0 || e < E) {
+        const int c = e / N, m = e - c * N;
+        const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        buf[c * SEQ + lp<SH>(pos)] = v[it];
+      }
+    }
+  }
+  constexpr int W = P * HN;
+  constexpr int ITW = (W + NT - 1) / NT;
+  // forward outputs of item k -> operator -> inverse pre-twiddle, in place
+  // (the sources of the modes the pre-twiddle reads are the item's own two
+  // forward outputs: the one-axis operators of r2r_kernels.h)
+  auto op_phase = [&](auto grad_tag) {
+    constexpr bool GRAD = decltype(grad_tag)::value;
+    MkSeq<A, N, P, NT> ms = ms0;
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int w = tid + it * NT;
+      if (W % NT == 0 || w < W) {
+        const int c = w / HN, k = w - c * HN;
+        const int kb = (k == 0) ? HN : N - k;
+        const cplx<A> mkk = ms.get(mk, 2, k);
+        A y0a, y1a, y0b, y1b;
+        makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+        cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
+        if constexpr (GRAD) {
+          // DST-III output: positions pa = N-1-k, pb = N-1-kb; source p+1
+          const int pa = N - 1 - k, pb = N - 1 - kb;
+          if (pa <= N - 2) {
+            const A wgt = -cw * vq(pa + 1);
+            const cplx<A> y = (pa + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+            xk = cplx<A>{wgt * y.x, wgt * y.y};
+          }
+          if (pb <= N - 2) {
+            const A wgt = -cw * vq(pb + 1);
+            const cplx<A> y = (pb + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+            xb = cplx<A>{wgt * y.x, wgt * y.y};
+          }
+        } else {
+          const A qa = vq(k), qb = vq(kb);
+          const A wa = cw * exp(-1.57 * 1.57 * qa * qa * 2), wb = cw * exp(-1.57 * 1.57 * qb * qb * 2);
+          xk = cplx<A>{wa * y0a, wa * y1a};
+          xb = cplx<A>{wb * y0b, wb * y1b};
+        }
+        cplx<A> zk, zb;
+        makhoul_pre<A, N>(mk, k, mkk, xk, xb, zk, zb);
+        buf[c * SEQ + lp<SH>(k)] = zk;
+        buf[c * SEQ + lp<SH>(kb)] = zb;
+      }
+      ms.advance(2, it);
+    }
+  };
+
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
+  op_phase(std::true_type{});
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
+
+  // 2. item n: d_z psi at 2n, 2n+1 of both lines (DST-III: odd output
+  // negated), stored; |amp| at the same points back into its two slots
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int c = w / HN, n = w - c * HN;
+      const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
+      cplx<A> am0 = {A(0), A(0)}, am1 = {A(0), A(0)};  // (line 0, line 1) at 2n and 2n+1
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int l = 2 * c + q;
+        if (l < nvalid) {
+          const long long o = base + (long long)l * N + 2 * n;
+          const A g0 = q == 0 ? ua.x : ua.y, g1 = -(q == 0 ? ub.x : ub.y);
+          io.gpz[o] = (T)g0;
+          io.gpz[o + 1] = (T)g1;
+          // the amplitude load's expression, from the stored (rounded) d_z psi
+          const A gz0 = (A)(T)g0, gz1 = (A)(T)g1;
+          const A p0 = io.psi[o], p1 = io.psi[o + 1];
+          const A x0 = io.gpx[o], x1 = io.gpx[o + 1], y0 = io.gpy[o], y1 = io.gpy[o + 1];
+          const A a0 = sqrt(io.q02 * (p0 * p0) + x0 * x0 + y0 * y0 + gz0 * gz0);
+          const A a1 = sqrt(io.q02 * (p1 * p1) + x1 * x1 + y1 * y1 + gz1 * gz1);
+          if (q == 0) { am0.x = a0; am1.x = a1; } else { am0.y = a0; am1.y = a1; }
+        }
+      }
+      buf[c * SEQ + lp<SH>(n)] = am0;
+      buf[c * SEQ + lp<SH>(N - 1 - n)] = am1;
+    }
+  }
+  __syncthreads();
+  // 3. the z factor of the Gaussian filter on |amp|
+  fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
+  op_phase(std::false_type{});
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int c = w / HN, n = w - c * HN;
+      const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int l = 2 * c + q;
+        if (l < nvalid) {
+          const long long o = base + (long long)l * N + 2 * n;
+          io.amp[o] = (T)(q == 0 ? ua.x : ua.y);
+          io.amp[o + 1] = (T)(q == 0 ? ub.x : ub.y);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int N>
+hipError_t launch_amp(const StepConsts& c, const AmpIO<T>& io, const void* tw, const void* mk, hipStream_t s) {
+  using C = TileChoice<T, N>;
+  constexpr int P = C::PZ, NT = C::NTZ, TL = 2 * P;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long blocks = (nl + TL - 1) / TL;
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
+  hipLaunchKernelGGL((amp_z_kernel<T, N, P, NT>), dim3((unsigned)blocks, (unsigned)c.nmem), dim3(NT), lds, s, io,
+                     (int)nl, static_cast<const cplx<double>*>(tw), static_cast<const cplx<double>*>(mk));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool amp_zpass_supported(int nz) { return R2R::fft_supported(nz); }
+
+template <typename T>
+hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, hipStream_t s) {
+  if (!amp_zpass_supported(c.nz)) return hipErrorNotSupported;
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
+  if (e != hipSuccess) return e;
+  switch (c.nz) {
+    case 16: return launch_amp<T, 16>(c, io, tw, mk, s);
+    case 32: return launch_amp<T, 32>(c, io, tw, mk, s);
+    case 48: return launch_amp<T, 48>(c, io, tw, mk, s);
+    case 64: return launch_amp<T, 64>(c, io, tw, mk, s);
+    case 96: return launch_amp<T, 96>(c, io, tw, mk, s);
+    case 128: return launch_amp<T, 128>(c, io, tw, mk, s);
+    case 192: return launch_amp<T, 192>(c, io, tw, mk, s);
+    case 256: return launch_amp<T, 256>(c, io, tw, mk, s);
+    case 384: return launch_amp<T, 384>(c, io, tw, mk, s);
+    case 512: return launch_amp<T, 512>(c, io, tw, mk, s);
+    case 1024: return launch_amp<T, 1024>(c, io, tw, mk, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+template hipError_t amp_zpass<double>(const StepConsts&, const AmpIO<double>&, R2R&, hipStream_t);
+template hipError_t amp_zpass<float>(const StepConsts&, const AmpIO<float>&, R2R&, hipStream_t);
+
 template hipError_t density_zpass<double>(const StepConsts&, const DensityIO<double>&, R2R&, hipStream_t);
 template hipError_t density_zpass<float>(const StepConsts&, const DensityIO<float>&, R2R&, hipStream_t);
 
