@@ -111,6 +111,10 @@ struct smaiso_handle {
   // gradient is a small difference of large values) and stored in the work
   // type, whose rounding is then relative to the gradient itself
   void* rd[3] = {nullptr, nullptr, nullptr};
+  // d_z rho of the current rho between steps (work type), where the density
+  // pass leaves it: a step swaps it into rd[2] first, so rd[2] keeps the
+  // step's own grad rho for the read-outs and the adjoint
+  void* rdz_next = nullptr;
   double *partials = nullptr, *diag_out = nullptr;
   // FP64 field of the group's size, FP32 storage only: the field getters'
   // widened copy, and within a step the intermediates of psi's inverse
@@ -513,7 +517,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->diag_out = static_cast<double*>(tmpp);
   std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
-                               &h->divf, &h->nl, &h->tmp};
+                               &h->divf, &h->nl, &h->tmp, &h->rdz_next};
   for (int a = 0; a < 3; ++a) {
     tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vtq[a]); tbufs.push_back(&h->virrq[a]);
@@ -558,7 +562,12 @@ int run_step(smaiso_handle* h) {
   // type (the passes compute in FP64, so the FP32 copy of psi contributes
   // only its storage rounding).  The passes that read one input run back to
   // back, so its later reads find it in the memory-side cache.
-  for (int a = 0; a < 3; ++a) {
+  // Along z it is there already where the density update runs in the z
+  // pass of Nl's transform: that pass leaves d_z rho_new, and setting,
+  // loading or restoring a state computes it with the same code.
+  const bool rdz_kept = density_zpass_supported(h->g.nz);
+  if (rdz_kept) std::swap(h->rd[2], h->rdz_next);
+  for (int a = 0; a < (rdz_kept ? 2 : 3); ++a) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
@@ -714,6 +723,8 @@ int run_step(smaiso_handle* h) {
     io.psi = static_cast<const T*>(h->psi);
     io.divf = static_cast<const T*>(h->divf);
     io.nlz = static_cast<T*>(h->nlq);
+    io.rdz_next = static_cast<T*>(h->rdz_next);
+    io.qs = fb.qs[2];
     {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
       HIPTRY(h, density_zpass<T>(c, io, h->r2r, s));
@@ -755,6 +766,17 @@ int run_step(smaiso_handle* h) {
 
 int run_step_any(smaiso_handle* h) { return h->mixed ? run_step<float>(h) : run_step<double>(h); }
 
+// d_z rho of the current rho (all members, or member m) into rdz_next,
+// where steps keep it from one to the next (the density pass leaves it)
+hipError_t refresh_rdz(smaiso_handle* h, int m = -1) {
+  if (!density_zpass_supported(h->g.nz)) return hipSuccess;
+  const StepConsts c = m < 0 ? h->consts() : h->consts_one(m);
+  const long long off = m < 0 ? 0 : (long long)m * h->n3();
+  const double qs = h->fuse_base().qs[2];
+  if (h->mixed) return rho_dz<float>(c, h->rho + off, static_cast<float*>(h->rdz_next) + off, qs, h->r2r, h->stream);
+  return rho_dz<double>(c, h->rho + off, static_cast<double*>(h->rdz_next) + off, qs, h->r2r, h->stream);
+}
+
 // Pre-loop of member m (S:965-1001) in FP64 for both precisions:
 // psiq = scale DCT(psi), psi = iDCT(psiq), psiq_old = psiq, f_rho = 0.
 int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, hipMemcpyKind kind) {
@@ -774,6 +796,7 @@ int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, 
   else HIPTRY(h, hipMemcpyAsync(static_cast<double*>(h->psi) + off, scratch, db, hipMemcpyDeviceToDevice, s));
   HIPTRY(h, hipMemcpyAsync(h->psiq_old + off, psiq, db, hipMemcpyDeviceToDevice, s));
   HIPTRY(h, hipMemcpyAsync(h->rho + off, rho, db, kind, s));
+  HIPTRY(h, refresh_rdz(h, m));
   fill_zero_d(scratch, n3, s);
   HIPTRY(h, hipGetLastError());
   HIPTRY(h, hipStreamSynchronize(s));
@@ -1482,6 +1505,7 @@ hipError_t restore_state(smaiso_handle* h, const StateStore& st, int k, long lon
   if (e == hipSuccess) e = hipMemcpyAsync(h->f_rho, st.field(k, 3), db, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(h->nlq, st.nlq(k), tb, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(h->psi, st.psi(k), tb, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = refresh_rdz(h);
   h->nloop = nloop;
   return e;
 }
@@ -1811,6 +1835,7 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
   }
   std::fclose(f);
   if (!ok) return fail(h, SMAISO_E_INVALID, fmt("%s is truncated", path));
+  HIPTRY(h, refresh_rdz(h));
   h->nloop = hd.nloop;
   h->member_set[0] = 1;
   h->nlq_valid = true;  // the checkpoint holds the last step's N^l

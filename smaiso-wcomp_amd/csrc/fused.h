@@ -45,11 +45,20 @@ struct DensityIO {
   const T *mu, *amp, *psi, *divf;
   const T* gp[3];       // grad psi
   T* nlz;               // z pass of Nl (REDFT10)
+  // when set: d_z rho_new (the z component of the next step's grad rho, the
+  // gradient one-axis operator of a6) is stored here; qs = pi/(dz Nz)
+  T* rdz_next = nullptr;
+  double qs = 0;
 };
 
 template <typename T>
 hipError_t density_zpass(const StepConsts& c, const DensityIO<T>& io, R2R& r2r, hipStream_t s);
 bool density_zpass_supported(int nz);
+
+// d_z rho of the current rho into rdz with the density pass's tile and code,
+// for a state that was set or loaded rather than left by a step
+template <typename T>
+hipError_t rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, R2R& r2r, hipStream_t s);
 
 // a11's fgrad^ += G erho^, the velocity solve a12 (S:1479-1974) and the
 // spectral div v^ of a13 (S:2042-2063) fused with the z passes of the inverse

@@ -184,6 +184,109 @@ hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hi
 
 namespace {
 
+// Operator phase of a one-axis operator along z on a tile of P sequences
+// (line pairs) after the forward FFT: item k < N/2 of sequence c forms the
+// forward outputs at k and kb (Makhoul post), applies the operator and
+// Makhoul's inverse pre-twiddle, in place at k and kb (the sources of the
+// modes the pre-twiddle reads are the item's own two forward outputs).
+// GRAD: DCT-II in, y[n] = -Vs[n] Y[n+1]/(2N), DST-III out; otherwise the
+// Gaussian factor exp(-1.57^2 2 Vq[n]^2)/(2N) between DCT-II and DCT-III.
+// Vq[i] = i qs.  The caller synchronizes before and after.
+template <bool GRAD, int N, int P, int NT, int SEQ, int SH>
+__device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<double>* __restrict__ mk,
+                                                const MkSeq<double, N, P, NT>& ms0, double qs) {
+  using A = double;
+  constexpr int HN = N / 2, W = P * HN, ITW = (W + NT - 1) / NT;
+  const A cw = A(1) / (2.0 * N);
+  const int tid = threadIdx.x;
+  auto vq = [&](int i) -> A { return (A)i * qs; };
+  MkSeq<A, N, P, NT> ms = ms0;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int c = w / HN, k = w - c * HN;
+      const int kb = (k == 0) ? HN : N - k;
+      const cplx<A> mkk = ms.get(mk, 2, k);
+      A y0a, y1a, y0b, y1b;
+      makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
+      cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
+      if constexpr (GRAD) {
+        // DST-III output: positions pa = N-1-k, pb = N-1-kb; source p+1
+        const int pa = N - 1 - k, pb = N - 1 - kb;
+        if (pa <= N - 2) {
+          const A wgt = -cw * vq(pa + 1);
+          const cplx<A> y = (pa + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+          xk = cplx<A>{wgt * y.x, wgt * y.y};
+        }
+        if (pb <= N - 2) {
+          const A wgt = -cw * vq(pb + 1);
+          const cplx<A> y = (pb + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+          xb = cplx<A>{wgt * y.x, wgt * y.y};
+        }
+      } else {
+        const A qa = vq(k), qb = vq(kb);
+        const A wa = cw * exp(-1.57 * 1.57 * qa * qa * 2), wb = cw * exp(-1.57 * 1.57 * qb * qb * 2);
+        xk = cplx<A>{wa * y0a, wa * y1a};
+        xb = cplx<A>{wb * y0b, wb * y1b};
+      }
+      cplx<A> zk, zb;
+      makhoul_pre<A, N>(mk, k, mkk, xk, xb, zk, zb);
+      buf[c * SEQ + lp<SH>(k)] = zk;
+      buf[c * SEQ + lp<SH>(kb)] = zb;
+    }
+    ms.advance(2, it);
+  }
+}
+
+// d_z of a real z line pair by the gradient one-axis operator (DCT-II along
+// z, shift by -Vs, DST-III): the values of items e = tid + it NT (line e /
+// N, element e mod N; the layout of the density pass) go into the LDS
+// sequence in Makhoul's forward order, and d_z lands in out (line l at
+// out + l N; nvalid lines).  One line pair per workgroup of NT threads.  The
+// caller has synchronized after its last use of buf.
+template <typename T, int N, int NT>
+__device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf, T* out, int nvalid,
+                                             const cplx<double>* __restrict__ tw, const cplx<double>* __restrict__ mk,
+                                             double qs) {
+  using A = double;
+  constexpr int P = 1, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, P, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  constexpr int E = 2 * N, IT = (E + NT - 1) / NT;
+  A* rbuf = reinterpret_cast<A*>(buf);
+  const int tid = threadIdx.x;
+  StageTw<A, N, P, NT, MAXR> stw;
+  stw.load(tw, tid);
+  const MkSeq<A, N, P, NT> ms0(mk, 2, tid);
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + it * NT;
+    if (E % NT == 0 || e < E) {
+      const int l = e / N, m = e - l * N;
+      const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+      rbuf[2 * lp<SH>(pos) + l] = x[it];
+    }
+  }
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
+  z_axis_op_phase<true, N, P, NT, SEQ, SH>(buf, mk, ms0, qs);
+  __syncthreads();
+  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
+  constexpr int ITW = (HN + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int n = tid + it * NT;
+    if (HN % NT == 0 || n < HN) {
+      const cplx<A> ua = buf[lp<SH>(n)];
+      const cplx<A> ub = buf[lp<SH>(N - 1 - n)];
+      // DST-III: y_{2n} = u_n, y_{2n+1} = -u_{N-1-n}
+      if (0 < nvalid) { out[2 * n] = (T)ua.x; out[2 * n + 1] = (T)(-ub.x); }
+      if (1 < nvalid) { out[N + 2 * n] = (T)ua.y; out[N + 2 * n + 1] = (T)(-ub.y); }
+    }
+  }
+}
+
 // One workgroup: one z line pair (P = 1) of 128 threads.  Load phase: every
 // point evaluates a15 and a16 in FP64 registers as k_density_nl does (same
 // expressions, same order), stores rho_new and the source f_rho, and puts Nl
@@ -195,7 +298,7 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
                                                        const cplx<double>* __restrict__ mk) {
   constexpr int P = 1, TL = 2, HN = N / 2;
   constexpr int MAXR = pick_maxr(N, P, NT);
-  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  constexpr int SH = pad_shift(N, MAXR);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using A = double;
   cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
@@ -211,9 +314,11 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
 
   constexpr int E = TL * N;
   constexpr int IT = (E + NT - 1) / NT;
+  A rnew[IT];  // rho_new of the items, for d_z rho_new
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int e = tid + it * NT;
+    rnew[it] = A(0);
     if (E % NT == 0 || e < E) {
       const int l = e / N, m = e - l * N;
       A v = A(0);
@@ -225,6 +330,7 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
         const A r1 = r0 + c.dt * (1.5 * src - 0.5 * io.f_rho[idx]);
         io.rho[idx] = r1;
         io.f_rho[idx] = src;
+        rnew[it] = r1;
         const A am = io.amp[idx];
         v = -(A)io.mu[idx] + ((mc.kp * mc.zeta) / (am + 0.001)) * (r1 - mc.rho0 - mc.kp * am) * (A)io.psi[idx] -
             (A)io.divf[idx] - ux * (A)io.gp[0][idx] - uy * (A)io.gp[1][idx] - uz * (A)io.gp[2][idx];
@@ -257,9 +363,47 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
       }
     }
   }
+  // d_z rho_new: the next step's z component of grad rho
+  if (io.rdz_next) {
+    __syncthreads();
+    dz_line_pair<T, N, NT>(rnew, buf, io.rdz_next + base, nvalid, tw, mk, io.qs);
+  }
+}
+
+// d_z rho of the current rho, by the same tile and code as the density pass
+// (so a state set or loaded gives the values a step would have left)
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void rho_dz_kernel(StepConsts c, const double* __restrict__ rho, T* rdz,
+                                                    double qs, const cplx<double>* __restrict__ tw,
+                                                    const cplx<double>* __restrict__ mk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int TL = 2, E = TL * N, IT = (E + NT - 1) / NT;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
+  const long long base = (long long)blockIdx.y * nl * N + first * N;
+  double x[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = threadIdx.x + it * NT;
+    x[it] = 0.0;
+    if ((E % NT == 0 || e < E) && e / N < nvalid) x[it] = rho[base + e];
+  }
+  dz_line_pair<T, N, NT>(x, reinterpret_cast<cplx<double>*>(smem_raw), rdz + base, nvalid, tw, mk, qs);
 }
 
 constexpr int kDensityNT = 128;
+
+template <typename T, int N>
+hipError_t launch_rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, const void* tw, const void* mk,
+                         hipStream_t s) {
+  const long long blocks = ((long long)c.nx * c.ny + 1) / 2;
+  const size_t lds = (size_t)seq_len(N, pick_maxr(N, 1, kDensityNT)) * sizeof(cplx<double>);
+  hipLaunchKernelGGL((rho_dz_kernel<T, N, kDensityNT>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                     dim3(kDensityNT), lds, s, c, rho, rdz, qs, static_cast<const cplx<double>*>(tw),
+                     static_cast<const cplx<double>*>(mk));
+  return hipGetLastError();
+}
 
 template <typename T, int N>
 hipError_t launch_density(const StepConsts& c, const DensityIO<T>& io, const void* tw, const void* mk,
@@ -539,8 +683,6 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   StageTw<A, N, P, NT, MAXR> stw;
   stw.load(tw, tid);
   const MkSeq<A, N, P, NT> ms0(mk, 2, tid);
-  const A cw = A(1) / (2.0 * N);
-  auto vq = [&](int i) -> A { return (A)i * io.qs; };
 
   // 1. psi -> LDS (DCT-II input order: v[m/2] = x[m] for even m, v[N-1-(m-1)/2] = x[m] for odd m)
   constexpr int E = P * N;
@@ -570,53 +712,9 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   }
   constexpr int W = P * HN;
   constexpr int ITW = (W + NT - 1) / NT;
-  // forward outputs of item k -> operator -> inverse pre-twiddle, in place
-  // (the sources of the modes the pre-twiddle reads are the item's own two
-  // forward outputs: the one-axis operators of r2r_kernels.h)
-  auto op_phase = [&](auto grad_tag) {
-    constexpr bool GRAD = decltype(grad_tag)::value;
-    MkSeq<A, N, P, NT> ms = ms0;
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      const int w = tid + it * NT;
-      if (W % NT == 0 || w < W) {
-        const int c = w / HN, k = w - c * HN;
-        const int kb = (k == 0) ? HN : N - k;
-        const cplx<A> mkk = ms.get(mk, 2, k);
-        A y0a, y1a, y0b, y1b;
-        makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
-        cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
-        if constexpr (GRAD) {
-          // DST-III output: positions pa = N-1-k, pb = N-1-kb; source p+1
-          const int pa = N - 1 - k, pb = N - 1 - kb;
-          if (pa <= N - 2) {
-            const A wgt = -cw * vq(pa + 1);
-            const cplx<A> y = (pa + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-            xk = cplx<A>{wgt * y.x, wgt * y.y};
-          }
-          if (pb <= N - 2) {
-            const A wgt = -cw * vq(pb + 1);
-            const cplx<A> y = (pb + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-            xb = cplx<A>{wgt * y.x, wgt * y.y};
-          }
-        } else {
-          const A qa = vq(k), qb = vq(kb);
-          const A wa = cw * exp(-1.57 * 1.57 * qa * qa * 2), wb = cw * exp(-1.57 * 1.57 * qb * qb * 2);
-          xk = cplx<A>{wa * y0a, wa * y1a};
-          xb = cplx<A>{wb * y0b, wb * y1b};
-        }
-        cplx<A> zk, zb;
-        makhoul_pre<A, N>(mk, k, mkk, xk, xb, zk, zb);
-        buf[c * SEQ + lp<SH>(k)] = zk;
-        buf[c * SEQ + lp<SH>(kb)] = zb;
-      }
-      ms.advance(2, it);
-    }
-  };
-
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  op_phase(std::true_type{});
+  z_axis_op_phase<true, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
 
@@ -654,7 +752,7 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   __syncthreads();
   // 3. the z factor of the Gaussian filter on |amp|
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  op_phase(std::false_type{});
+  z_axis_op_phase<false, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
 #pragma unroll
@@ -717,6 +815,31 @@ hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, hipStrea
 
 template hipError_t amp_zpass<double>(const StepConsts&, const AmpIO<double>&, R2R&, hipStream_t);
 template hipError_t amp_zpass<float>(const StepConsts&, const AmpIO<float>&, R2R&, hipStream_t);
+
+template <typename T>
+hipError_t rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, R2R& r2r, hipStream_t s) {
+  if (!density_zpass_supported(c.nz)) return hipErrorNotSupported;
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
+  if (e != hipSuccess) return e;
+  switch (c.nz) {
+    case 16: return launch_rho_dz<T, 16>(c, rho, rdz, qs, tw, mk, s);
+    case 32: return launch_rho_dz<T, 32>(c, rho, rdz, qs, tw, mk, s);
+    case 48: return launch_rho_dz<T, 48>(c, rho, rdz, qs, tw, mk, s);
+    case 64: return launch_rho_dz<T, 64>(c, rho, rdz, qs, tw, mk, s);
+    case 96: return launch_rho_dz<T, 96>(c, rho, rdz, qs, tw, mk, s);
+    case 128: return launch_rho_dz<T, 128>(c, rho, rdz, qs, tw, mk, s);
+    case 192: return launch_rho_dz<T, 192>(c, rho, rdz, qs, tw, mk, s);
+    case 256: return launch_rho_dz<T, 256>(c, rho, rdz, qs, tw, mk, s);
+    case 384: return launch_rho_dz<T, 384>(c, rho, rdz, qs, tw, mk, s);
+    case 512: return launch_rho_dz<T, 512>(c, rho, rdz, qs, tw, mk, s);
+    case 1024: return launch_rho_dz<T, 1024>(c, rho, rdz, qs, tw, mk, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+template hipError_t rho_dz<double>(const StepConsts&, const double*, double*, double, R2R&, hipStream_t);
+template hipError_t rho_dz<float>(const StepConsts&, const double*, float*, double, R2R&, hipStream_t);
 
 template hipError_t density_zpass<double>(const StepConsts&, const DensityIO<double>&, R2R&, hipStream_t);
 template hipError_t density_zpass<float>(const StepConsts&, const DensityIO<float>&, R2R&, hipStream_t);
