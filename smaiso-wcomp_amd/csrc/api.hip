@@ -88,6 +88,9 @@ struct smaiso_handle {
   // steps read it)
   long long virr_epoch = -1;
   bool keep_virr = false;
+  // divf holds the y and x terms of div f only: the density pass formed the z
+  // term from f_z (still in f[2] until the next step)
+  bool divf_partial = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -695,24 +698,29 @@ int run_step(smaiso_handle* h) {
   }
   if (!vfused) HIPTRY(h, h->xw(kICT, h->divv, h->divv, nullptr, h->a32(SMAISO_STAGE_DIVV)));
   // div f = iCT(scale sum_a Vq_a shift(ST_a f_a)) (S:2064-2092) is a sum of
-  // one-axis operators, one accumulating pass per component
+  // one-axis operators, one accumulating pass per component; the z term is
+  // formed by the density pass where it runs (divf then holds the y and x
+  // terms)
+  const bool dens_z = density_zpass_supported(h->g.nz);
   {
     Fuse d = fb;
     d.op = OP_AX_DIV;
     d.arith32 = h->a32(SMAISO_STAGE_DIVF);
-    for (int a = 2; a >= 0; --a) {
+    const int a0 = dens_z ? 1 : 2;
+    for (int a = a0; a >= 0; --a) {
       d.axis = a;
-      d.accumulate = a != 2;
+      d.accumulate = a != a0;
       HIPTRY(h, h->axpass(h->mixed ? F32 : F64, a, f[a], h->divf, d));
     }
   }
+  h->divf_partial = dens_z;
   for (int a = 0; a < 3; ++a) {
     if (vfused) HIPTRY(h, h->xw_yx(kIST[a], v[a], h->a32(SMAISO_STAGE_VELOCITY)));
     else HIPTRY(h, h->xw(kIST[a], vtq[a], v[a], nullptr, h->a32(SMAISO_STAGE_VELOCITY)));
   }
   // a15 + a16 (S:2121-2162) and the transform of Nl (S:2166-2168): the z
   // pass of the transform computes them on load where the length allows
-  if (density_zpass_supported(h->g.nz)) {
+  if (dens_z) {
     DensityIO<T> io;
     for (int a = 0; a < 3; ++a) { io.v[a] = v[a]; io.rd[a] = rd[a]; io.gp[a] = gp[a]; }
     io.divv = static_cast<const T*>(h->divv);
@@ -724,6 +732,7 @@ int run_step(smaiso_handle* h) {
     io.divf = static_cast<const T*>(h->divf);
     io.nlz = static_cast<T*>(h->nlq);
     io.rdz_next = static_cast<T*>(h->rdz_next);
+    io.fz = f[2];
     io.qs = fb.qs[2];
     {
       smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
@@ -896,7 +905,21 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
     case SMAISO_FIELD_PSIQ_OLD: *out = h->psiq_old; return SMAISO_OK;
     case SMAISO_FIELD_NLQ: *out = as64(h->nlq); return SMAISO_OK;
     case SMAISO_FIELD_F_RHO: *out = h->f_rho; return SMAISO_OK;
-    case SMAISO_FIELD_DIVF: *out = as64(h->divf); return SMAISO_OK;
+    case SMAISO_FIELD_DIVF:
+      if (h->divf_partial) {
+        // the stored y and x terms plus the z term from f_z
+        HIPTRY(h, hipMemcpyAsync(h->tmp, h->divf, (size_t)nt * h->tsz, hipMemcpyDeviceToDevice, s));
+        Fuse d = h->fuse_base();
+        d.op = OP_AX_DIV;
+        d.axis = 2;
+        d.accumulate = 1;
+        d.arith32 = h->a32(SMAISO_STAGE_DIVF);
+        HIPTRY(h, h->axpass(h->mixed ? F32 : F64, 2, h->f[2], h->tmp, d));
+        *out = as64(h->tmp);
+        return SMAISO_OK;
+      }
+      *out = as64(h->divf);
+      return SMAISO_OK;
     case SMAISO_FIELD_GPSIX: case SMAISO_FIELD_GPSIY: case SMAISO_FIELD_GPSIZ:
       *out = as64(h->gp[fid - SMAISO_FIELD_GPSIX]); return SMAISO_OK;
     case SMAISO_FIELD_GRHOX: case SMAISO_FIELD_GRHOY: case SMAISO_FIELD_GRHOZ:

@@ -48,6 +48,9 @@ struct DensityIO {
   // when set: d_z rho_new (the z component of the next step's grad rho, the
   // gradient one-axis operator of a6) is stored here; qs = pi/(dz Nz)
   T* rdz_next = nullptr;
+  // when set: divf holds the y and x terms of div f only, and the pass forms
+  // the z term from f_z (real space) itself
+  const T* fz = nullptr;
   double qs = 0;
 };
 

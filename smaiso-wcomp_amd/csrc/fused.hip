@@ -189,10 +189,13 @@ namespace {
 // forward outputs at k and kb (Makhoul post), applies the operator and
 // Makhoul's inverse pre-twiddle, in place at k and kb (the sources of the
 // modes the pre-twiddle reads are the item's own two forward outputs).
-// GRAD: DCT-II in, y[n] = -Vs[n] Y[n+1]/(2N), DST-III out; otherwise the
-// Gaussian factor exp(-1.57^2 2 Vq[n]^2)/(2N) between DCT-II and DCT-III.
-// Vq[i] = i qs.  The caller synchronizes before and after.
-template <bool GRAD, int N, int P, int NT, int SEQ, int SH>
+// kZGrad: DCT-II in, y[n] = -Vs[n] Y[n+1]/(2N), DST-III out; kZDiv: DST-II
+// in (odd inputs negated, outputs reversed), y[n] = Vq[n] Y[n-1]/(2N),
+// DCT-III out; kZGauss: the Gaussian factor exp(-1.57^2 2 Vq[n]^2)/(2N)
+// between DCT-II and DCT-III.  Vq[i] = i qs.  The caller synchronizes before
+// and after.
+enum ZAxisOp { kZGrad = 0, kZDiv = 1, kZGauss = 2 };
+template <int ZOP, int N, int P, int NT, int SEQ, int SH>
 __device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<double>* __restrict__ mk,
                                                 const MkSeq<double, N, P, NT>& ms0, double qs) {
   using A = double;
@@ -211,7 +214,7 @@ __device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<do
       A y0a, y1a, y0b, y1b;
       makhoul_post<A, N>(mk, k, kb, mkk, buf[c * SEQ + lp<SH>(k)], buf[c * SEQ + lp<SH>(kb)], y0a, y1a, y0b, y1b);
       cplx<A> xk = {A(0), A(0)}, xb = {A(0), A(0)};
-      if constexpr (GRAD) {
+      if constexpr (ZOP == kZGrad) {
         // DST-III output: positions pa = N-1-k, pb = N-1-kb; source p+1
         const int pa = N - 1 - k, pb = N - 1 - kb;
         if (pa <= N - 2) {
@@ -222,6 +225,20 @@ __device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<do
         if (pb <= N - 2) {
           const A wgt = -cw * vq(pb + 1);
           const cplx<A> y = (pb + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+          xb = cplx<A>{wgt * y.x, wgt * y.y};
+        }
+      } else if constexpr (ZOP == kZDiv) {
+        // DST-II input: forward output index N-1-k sits at Makhoul index k;
+        // DCT-III output positions pa = k, pb = kb; source p-1
+        const int fa = N - 1 - k;
+        if (k >= 1) {
+          const A wgt = cw * vq(k);
+          const cplx<A> y = (k - 1 == fa) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
+          xk = cplx<A>{wgt * y.x, wgt * y.y};
+        }
+        {
+          const A wgt = cw * vq(kb);
+          const cplx<A> y = (kb - 1 == fa) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
           xb = cplx<A>{wgt * y.x, wgt * y.y};
         }
       } else {
@@ -270,7 +287,7 @@ __device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf,
   }
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  z_axis_op_phase<true, N, P, NT, SEQ, SH>(buf, mk, ms0, qs);
+  z_axis_op_phase<kZGrad, N, P, NT, SEQ, SH>(buf, mk, ms0, qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
   constexpr int ITW = (HN + NT - 1) / NT;
@@ -291,8 +308,11 @@ __device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf,
 // point evaluates a15 and a16 in FP64 registers as k_density_nl does (same
 // expressions, same order), stores rho_new and the source f_rho, and puts Nl
 // at its Makhoul position of the pair's complex sequence in LDS; then the
-// FFT and Makhoul's post-processing store Nl's z pass (DCT-II).
-template <typename T, int N, int NT>
+// FFT and Makhoul's post-processing store Nl's z pass (DCT-II).  DZF: the
+// z term of div f first, from f_z in the same LDS sequence (the divergence
+// one-axis operator: DST-II, y[n] = Vq[n] Y[n-1]/(2N), DCT-III); each point
+// adds it to the stored y and x terms and overwrites its own slot with Nl.
+template <typename T, int N, int NT, bool DZF>
 __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T> io,
                                                        const cplx<double>* __restrict__ tw,
                                                        const cplx<double>* __restrict__ mk) {
@@ -314,6 +334,27 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
 
   constexpr int E = TL * N;
   constexpr int IT = (E + NT - 1) / NT;
+  if constexpr (DZF) {
+    // f_z in DST-II input order: odd inputs negated at Makhoul positions
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = tid + it * NT;
+      if (E % NT == 0 || e < E) {
+        const int l = e / N, m = e - l * N;
+        A x = l < nvalid ? (A)io.fz[base + e] : A(0);
+        if (m & 1) x = -x;
+        const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        rbuf[2 * lp<SH>(pos) + (l & 1)] = x;
+      }
+    }
+    __syncthreads();
+    fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
+    const MkSeq<A, N, P, NT> ms0(mk, 2, tid);
+    z_axis_op_phase<kZDiv, N, P, NT, seq_len(N, MAXR), SH>(buf, mk, ms0, io.qs);
+    __syncthreads();
+    // DCT-III output at position m sits in the slot of Makhoul position m
+    fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
+  }
   A rnew[IT];  // rho_new of the items, for d_z rho_new
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
@@ -332,8 +373,10 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
         io.f_rho[idx] = src;
         rnew[it] = r1;
         const A am = io.amp[idx];
+        const int pz = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        const A df = DZF ? (A)io.divf[idx] + rbuf[2 * lp<SH>(pz) + (l & 1)] : (A)io.divf[idx];
         v = -(A)io.mu[idx] + ((mc.kp * mc.zeta) / (am + 0.001)) * (r1 - mc.rho0 - mc.kp * am) * (A)io.psi[idx] -
-            (A)io.divf[idx] - ux * (A)io.gp[0][idx] - uy * (A)io.gp[1][idx] - uz * (A)io.gp[2][idx];
+            df - ux * (A)io.gp[0][idx] - uy * (A)io.gp[1][idx] - uz * (A)io.gp[2][idx];
       }
       const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
       rbuf[2 * lp<SH>(pos) + (l & 1)] = v;
@@ -410,9 +453,14 @@ hipError_t launch_density(const StepConsts& c, const DensityIO<T>& io, const voi
                           hipStream_t s) {
   const long long blocks = ((long long)c.nx * c.ny + 1) / 2;
   const size_t lds = (size_t)seq_len(N, pick_maxr(N, 1, kDensityNT)) * sizeof(cplx<double>);
-  hipLaunchKernelGGL((density_z_kernel<T, N, kDensityNT>), dim3((unsigned)blocks, (unsigned)c.nmem),
-                     dim3(kDensityNT), lds, s, c, io, static_cast<const cplx<double>*>(tw),
-                     static_cast<const cplx<double>*>(mk));
+  if (io.fz)
+    hipLaunchKernelGGL((density_z_kernel<T, N, kDensityNT, true>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                       dim3(kDensityNT), lds, s, c, io, static_cast<const cplx<double>*>(tw),
+                       static_cast<const cplx<double>*>(mk));
+  else
+    hipLaunchKernelGGL((density_z_kernel<T, N, kDensityNT, false>), dim3((unsigned)blocks, (unsigned)c.nmem),
+                       dim3(kDensityNT), lds, s, c, io, static_cast<const cplx<double>*>(tw),
+                       static_cast<const cplx<double>*>(mk));
   return hipGetLastError();
 }
 
@@ -714,7 +762,7 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   constexpr int ITW = (W + NT - 1) / NT;
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  z_axis_op_phase<true, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
+  z_axis_op_phase<kZGrad, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
 
@@ -752,7 +800,7 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   __syncthreads();
   // 3. the z factor of the Gaussian filter on |amp|
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  z_axis_op_phase<false, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
+  z_axis_op_phase<kZGauss, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
 #pragma unroll
