@@ -52,13 +52,15 @@ struct PendingTiming {
 };
 
 // FP32-arithmetic stages of an FP32-mixed handle at creation: the stages
-// whose FP32 rounding leaves psi about as close to the FP64 stepper as FP32
-// storage alone does (forces, div v, div f, v's transforms, Nl, the velocity
-// solve: psi within 4.8e-8 of FP64 after 2000 steps at 256^3, against
-// 3.3e-8 with none of them).  The amplitude filter in FP32 gives 2.9e-7 at
-// step 500; grad psi and lap(rho L psi) each reach 3e-5, L psi 1e-6.
+// whose FP32 rounding keeps psi within 3e-7 of the FP64 stepper at 256^3 and
+// 512x512x256 after 100, 500 and 2000 steps: forces, div v, div f, v's
+// transforms, Nl and the velocity solve (psi within 4.8e-8 with these alone,
+// the FP32 storage level) and the amplitude filter's Gaussian factors (psi
+// within 2.8e-7 at step 500, 1.4e-7 at 2000).  grad psi and lap(rho L psi)
+// each reach 3e-5, L psi 1e-6.
 constexpr uint32_t kFp32StagesDefault = SMAISO_STAGE_FORCES | SMAISO_STAGE_DIVV | SMAISO_STAGE_DIVF |
-                                        SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL | SMAISO_STAGE_VELSOLVE;
+                                        SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL | SMAISO_STAGE_VELSOLVE |
+                                        SMAISO_STAGE_AMP;
 
 double derived_amp(const smaiso_params& p) {
   return std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.epsilon * p.gamma)) / 5);
@@ -578,10 +580,9 @@ int run_step(smaiso_handle* h) {
     else HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
   }
   // grad psi along x and y, then one z pass forms d_z psi, |amp| at the
-  // same points and the z factor of a7's separable Gaussian filter (FP64
-  // arithmetic; both stages' passes compute in FP64 unless FP32 is asked for
-  // either, which takes the separate passes)
-  const bool afused = amp_zpass_supported(h->g.nz) && !h->a32(SMAISO_STAGE_GRAD_PSI) && !h->a32(SMAISO_STAGE_AMP);
+  // same points and the z factor of a7's separable Gaussian filter (d_z psi
+  // in FP64; FP32 for grad psi takes the separate passes)
+  const bool afused = amp_zpass_supported(h->g.nz) && !h->a32(SMAISO_STAGE_GRAD_PSI);
   for (int a = 0; a < (afused ? 2 : 3); ++a) {
     Fuse g = fb;
     g.axis = a;
@@ -604,7 +605,7 @@ int run_step(smaiso_handle* h) {
       io.q02 = fb.q02;
       io.qs = fb.qs[2];
       smaiso_handle::Scope sc(h, SMAISO_KC_R2R_AXIS, fb1 * 5);
-      HIPTRY(h, amp_zpass<T>(c, io, h->r2r, s));
+      HIPTRY(h, amp_zpass<T>(c, io, h->r2r, h->a32(SMAISO_STAGE_AMP) != 0, s));
     } else {
       g.op = OP_AX_GAUSS_AMP;
       g.axis = 2;

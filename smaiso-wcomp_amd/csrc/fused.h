@@ -88,7 +88,8 @@ bool velocity_zpass_supported(int nz);
 // pass: the pass computes d_z psi, stores it, forms |amp| = sqrt(q0^2 psi^2
 // + |grad psi|^2) at the same points and applies the z factor of the
 // separable Gaussian filter (DCT-II, exp(-1.57^2 2 Vq^2)/(2N), DCT-III)
-// before storing amp.  FP64 arithmetic for both storage types.
+// before storing amp.  FP64 arithmetic, except the Gaussian factor's
+// transforms in FP32 when gauss32 (FP32 storage, lengths with FP32 tables).
 template <typename T>
 struct AmpIO {
   const T *psi, *gpx, *gpy;
@@ -98,7 +99,7 @@ struct AmpIO {
 };
 
 template <typename T>
-hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, hipStream_t s);
+hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, bool gauss32, hipStream_t s);
 bool amp_zpass_supported(int nz);
 
 }  // namespace smaiso

@@ -195,14 +195,14 @@ namespace {
 // between DCT-II and DCT-III.  Vq[i] = i qs.  The caller synchronizes before
 // and after.
 enum ZAxisOp { kZGrad = 0, kZDiv = 1, kZGauss = 2 };
-template <int ZOP, int N, int P, int NT, int SEQ, int SH>
-__device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<double>* __restrict__ mk,
-                                                const MkSeq<double, N, P, NT>& ms0, double qs) {
-  using A = double;
-  constexpr int HN = N / 2, W = P * HN, ITW = (W + NT - 1) / NT;
-  const A cw = A(1) / (2.0 * N);
+template <int ZOP, int N, int P, int NT, int SEQ, int SH, typename A = double>
+__device__ __forceinline__ void z_axis_op_phase(cplx<A>* buf, const cplx<A>* __restrict__ mk,
+                                                const MkSeq<A, N, P, NT>& ms0, double qs) {
+  // weights in FP64 (the one-axis passes' ax_weight), applied in the tile's type
+  const double cw = 1.0 / (2.0 * N);
   const int tid = threadIdx.x;
-  auto vq = [&](int i) -> A { return (A)i * qs; };
+  constexpr int HN = N / 2, W = P * HN, ITW = (W + NT - 1) / NT;
+  auto vq = [&](int i) -> double { return (double)i * qs; };
   MkSeq<A, N, P, NT> ms = ms0;
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
@@ -218,34 +218,34 @@ __device__ __forceinline__ void z_axis_op_phase(cplx<double>* buf, const cplx<do
         // DST-III output: positions pa = N-1-k, pb = N-1-kb; source p+1
         const int pa = N - 1 - k, pb = N - 1 - kb;
         if (pa <= N - 2) {
-          const A wgt = -cw * vq(pa + 1);
+          const double wgt = -cw * vq(pa + 1);
           const cplx<A> y = (pa + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-          xk = cplx<A>{wgt * y.x, wgt * y.y};
+          xk = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
         }
         if (pb <= N - 2) {
-          const A wgt = -cw * vq(pb + 1);
+          const double wgt = -cw * vq(pb + 1);
           const cplx<A> y = (pb + 1 == k) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-          xb = cplx<A>{wgt * y.x, wgt * y.y};
+          xb = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
         }
       } else if constexpr (ZOP == kZDiv) {
         // DST-II input: forward output index N-1-k sits at Makhoul index k;
         // DCT-III output positions pa = k, pb = kb; source p-1
         const int fa = N - 1 - k;
         if (k >= 1) {
-          const A wgt = cw * vq(k);
+          const double wgt = cw * vq(k);
           const cplx<A> y = (k - 1 == fa) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-          xk = cplx<A>{wgt * y.x, wgt * y.y};
+          xk = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
         }
         {
-          const A wgt = cw * vq(kb);
+          const double wgt = cw * vq(kb);
           const cplx<A> y = (kb - 1 == fa) ? cplx<A>{y0a, y1a} : cplx<A>{y0b, y1b};
-          xb = cplx<A>{wgt * y.x, wgt * y.y};
+          xb = cplx<A>{(A)(wgt * y.x), (A)(wgt * y.y)};
         }
       } else {
-        const A qa = vq(k), qb = vq(kb);
-        const A wa = cw * exp(-1.57 * 1.57 * qa * qa * 2), wb = cw * exp(-1.57 * 1.57 * qb * qb * 2);
-        xk = cplx<A>{wa * y0a, wa * y1a};
-        xb = cplx<A>{wb * y0b, wb * y1b};
+        const double qa = vq(k), qb = vq(kb);
+        const double wa = cw * exp(-1.57 * 1.57 * qa * qa * 2), wb = cw * exp(-1.57 * 1.57 * qb * qb * 2);
+        xk = cplx<A>{(A)(wa * y0a), (A)(wa * y1a)};
+        xb = cplx<A>{(A)(wb * y0b), (A)(wb * y1b)};
       }
       cplx<A> zk, zb;
       makhoul_pre<A, N>(mk, k, mkk, xk, xb, zk, zb);
@@ -715,9 +715,12 @@ namespace {
 //     order (2n -> n, 2n+1 -> N-1-n), which it alone read;
 //  3. FFT, Makhoul post, exp(-1.57^2 2 Vq[n]^2)/(2N) and the pre-twiddle,
 //     inverse FFT, stores amp.
-template <typename T, int N, int P, int NT>
+// AG: the arithmetic of step 3 (FP64, or FP32 with FP32 tables tw32, mk32
+// for FP32 storage where the amplitude filter computes in FP32).
+template <typename T, int N, int P, int NT, typename AG>
 __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, const cplx<double>* __restrict__ tw,
-                                                   const cplx<double>* __restrict__ mk) {
+                                                   const cplx<double>* __restrict__ mk,
+                                                   const cplx<AG>* __restrict__ twg, const cplx<AG>* __restrict__ mkg) {
   using A = double;
   constexpr int TL = 2 * P, HN = N / 2;
   constexpr int MAXR = pick_maxr(N, P, NT);
@@ -768,6 +771,8 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
 
   // 2. item n: d_z psi at 2n, 2n+1 of both lines (DST-III: odd output
   // negated), stored; |amp| at the same points back into its two slots
+  cplx<A> amv[std::is_same<AG, A>::value ? 1 : ITW][2];
+  (void)amv;
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
     const int w = tid + it * NT;
@@ -793,23 +798,45 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
           if (q == 0) { am0.x = a0; am1.x = a1; } else { am0.y = a0; am1.y = a1; }
         }
       }
-      buf[c * SEQ + lp<SH>(n)] = am0;
-      buf[c * SEQ + lp<SH>(N - 1 - n)] = am1;
+      if constexpr (std::is_same<AG, A>::value) {
+        buf[c * SEQ + lp<SH>(n)] = am0;
+        buf[c * SEQ + lp<SH>(N - 1 - n)] = am1;
+      } else {
+        amv[it][0] = am0;
+        amv[it][1] = am1;
+      }
+    }
+  }
+  // 3. the z factor of the Gaussian filter on |amp|, in the AG tile (an FP32
+  // view of the same LDS, written once every FP64 slot has been read)
+  cplx<AG>* const bg = reinterpret_cast<cplx<AG>*>(smem_raw);
+  if constexpr (!std::is_same<AG, A>::value) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int w = tid + it * NT;
+      if (W % NT == 0 || w < W) {
+        const int c = w / HN, n = w - c * HN;
+        bg[c * SEQ + lp<SH>(n)] = cplx<AG>{(AG)amv[it][0].x, (AG)amv[it][0].y};
+        bg[c * SEQ + lp<SH>(N - 1 - n)] = cplx<AG>{(AG)amv[it][1].x, (AG)amv[it][1].y};
+      }
     }
   }
   __syncthreads();
-  // 3. the z factor of the Gaussian filter on |amp|
-  fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
-  z_axis_op_phase<kZGauss, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
+  StageTw<AG, N, P, NT, MAXR> stg;
+  stg.load(twg, tid);
+  const MkSeq<AG, N, P, NT> msg(mkg, 2, tid);
+  fft_stage<AG, N, P, NT, MAXR, -1, 0>(bg, twg, stg);
+  z_axis_op_phase<kZGauss, N, P, NT, SEQ, SH, AG>(bg, mkg, msg, io.qs);
   __syncthreads();
-  fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
+  fft_stage<AG, N, P, NT, MAXR, +1, 0>(bg, twg, stg);
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
     const int w = tid + it * NT;
     if (W % NT == 0 || w < W) {
       const int c = w / HN, n = w - c * HN;
-      const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
-      const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
+      const cplx<AG> ua = bg[c * SEQ + lp<SH>(n)];
+      const cplx<AG> ub = bg[c * SEQ + lp<SH>(N - 1 - n)];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int l = 2 * c + q;
@@ -823,16 +850,49 @@ __global__ __launch_bounds__(NT) void amp_z_kernel(AmpIO<T> io, int nlines, cons
   }
 }
 
-template <typename T, int N>
-hipError_t launch_amp(const StepConsts& c, const AmpIO<T>& io, const void* tw, const void* mk, hipStream_t s) {
+template <typename T, int N, typename AG>
+hipError_t launch_amp(const StepConsts& c, const AmpIO<T>& io, const void* tw, const void* mk, const void* twg,
+                      const void* mkg, hipStream_t s) {
   using C = TileChoice<T, N>;
   constexpr int P = C::PZ, NT = C::NTZ, TL = 2 * P;
   const long long nl = (long long)c.nx * c.ny;
   const long long blocks = (nl + TL - 1) / TL;
   const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
-  hipLaunchKernelGGL((amp_z_kernel<T, N, P, NT>), dim3((unsigned)blocks, (unsigned)c.nmem), dim3(NT), lds, s, io,
-                     (int)nl, static_cast<const cplx<double>*>(tw), static_cast<const cplx<double>*>(mk));
+  hipLaunchKernelGGL((amp_z_kernel<T, N, P, NT, AG>), dim3((unsigned)blocks, (unsigned)c.nmem), dim3(NT), lds, s,
+                     io, (int)nl, static_cast<const cplx<double>*>(tw), static_cast<const cplx<double>*>(mk),
+                     static_cast<const cplx<AG>*>(twg), static_cast<const cplx<AG>*>(mkg));
   return hipGetLastError();
+}
+
+template <typename T, typename AG>
+hipError_t amp_z_n(const StepConsts& c, const AmpIO<T>& io, const void* tw, const void* mk, const void* twg,
+                   const void* mkg, hipStream_t s) {
+  if constexpr (std::is_same<AG, float>::value) {
+    // the lengths with FP32 tables (r2r_arith32_supported)
+    switch (c.nz) {
+      case 64: return launch_amp<T, 64, AG>(c, io, tw, mk, twg, mkg, s);
+      case 96: return launch_amp<T, 96, AG>(c, io, tw, mk, twg, mkg, s);
+      case 128: return launch_amp<T, 128, AG>(c, io, tw, mk, twg, mkg, s);
+      case 256: return launch_amp<T, 256, AG>(c, io, tw, mk, twg, mkg, s);
+      case 512: return launch_amp<T, 512, AG>(c, io, tw, mk, twg, mkg, s);
+      default: return hipErrorNotSupported;
+    }
+  } else {
+    switch (c.nz) {
+      case 16: return launch_amp<T, 16, AG>(c, io, tw, mk, twg, mkg, s);
+      case 32: return launch_amp<T, 32, AG>(c, io, tw, mk, twg, mkg, s);
+      case 48: return launch_amp<T, 48, AG>(c, io, tw, mk, twg, mkg, s);
+      case 64: return launch_amp<T, 64, AG>(c, io, tw, mk, twg, mkg, s);
+      case 96: return launch_amp<T, 96, AG>(c, io, tw, mk, twg, mkg, s);
+      case 128: return launch_amp<T, 128, AG>(c, io, tw, mk, twg, mkg, s);
+      case 192: return launch_amp<T, 192, AG>(c, io, tw, mk, twg, mkg, s);
+      case 256: return launch_amp<T, 256, AG>(c, io, tw, mk, twg, mkg, s);
+      case 384: return launch_amp<T, 384, AG>(c, io, tw, mk, twg, mkg, s);
+      case 512: return launch_amp<T, 512, AG>(c, io, tw, mk, twg, mkg, s);
+      case 1024: return launch_amp<T, 1024, AG>(c, io, tw, mk, twg, mkg, s);
+      default: return hipErrorNotSupported;
+    }
+  }
 }
 
 }  // namespace
@@ -840,29 +900,23 @@ hipError_t launch_amp(const StepConsts& c, const AmpIO<T>& io, const void* tw, c
 bool amp_zpass_supported(int nz) { return R2R::fft_supported(nz); }
 
 template <typename T>
-hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, hipStream_t s) {
+hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, bool gauss32, hipStream_t s) {
   if (!amp_zpass_supported(c.nz)) return hipErrorNotSupported;
   const void *tw = nullptr, *mk = nullptr;
   hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
   if (e != hipSuccess) return e;
-  switch (c.nz) {
-    case 16: return launch_amp<T, 16>(c, io, tw, mk, s);
-    case 32: return launch_amp<T, 32>(c, io, tw, mk, s);
-    case 48: return launch_amp<T, 48>(c, io, tw, mk, s);
-    case 64: return launch_amp<T, 64>(c, io, tw, mk, s);
-    case 96: return launch_amp<T, 96>(c, io, tw, mk, s);
-    case 128: return launch_amp<T, 128>(c, io, tw, mk, s);
-    case 192: return launch_amp<T, 192>(c, io, tw, mk, s);
-    case 256: return launch_amp<T, 256>(c, io, tw, mk, s);
-    case 384: return launch_amp<T, 384>(c, io, tw, mk, s);
-    case 512: return launch_amp<T, 512>(c, io, tw, mk, s);
-    case 1024: return launch_amp<T, 1024>(c, io, tw, mk, s);
-    default: return hipErrorNotSupported;
+  if constexpr (sizeof(T) == 4) {
+    if (gauss32 && r2r_arith32_supported(c.nz)) {
+      const void *tw32 = nullptr, *mk32 = nullptr;
+      if ((e = r2r.table_ptrs(c.nz, &tw32, &mk32, F32)) != hipSuccess) return e;
+      return amp_z_n<T, float>(c, io, tw, mk, tw32, mk32, s);
+    }
   }
+  return amp_z_n<T, double>(c, io, tw, mk, tw, mk, s);
 }
 
-template hipError_t amp_zpass<double>(const StepConsts&, const AmpIO<double>&, R2R&, hipStream_t);
-template hipError_t amp_zpass<float>(const StepConsts&, const AmpIO<float>&, R2R&, hipStream_t);
+template hipError_t amp_zpass<double>(const StepConsts&, const AmpIO<double>&, R2R&, bool, hipStream_t);
+template hipError_t amp_zpass<float>(const StepConsts&, const AmpIO<float>&, R2R&, bool, hipStream_t);
 
 template <typename T>
 hipError_t rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, R2R& r2r, hipStream_t s) {
