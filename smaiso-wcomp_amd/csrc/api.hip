@@ -93,6 +93,9 @@ struct smaiso_handle {
   // divf holds the y and x terms of div f only: the density pass formed the z
   // term from f_z (still in f[2] until the next step)
   bool divf_partial = false;
+  // laprl holds the z lines of lap(rho L psi) before its last inverse z pass
+  // (the forces pass runs that pass in its tile); the read-out completes it
+  bool laprl_zlines = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -206,6 +209,9 @@ struct smaiso_handle {
     const bool relax = !(n > p.divv_switch || p.load_mode == 1);
     return !relax && n > 1;
   }
+  // the forces pass runs the last inverse z pass of lap(rho L psi) in steady
+  // steps (except the adjoint's recomputed ones)
+  bool laprl_in_forces() const { return forces_zpass_laprl(mixed ? F32 : F64, g.nz); }
 
   // --- timing helpers ----------------------------------------------------
   hipEvent_t take_event() {
@@ -338,12 +344,13 @@ struct smaiso_handle {
     return run_pass(dt, kinds[0], 0, io, g);
   }
   // DCT-II (z pass with f's load op), DCT-II along y, then one x pass doing
-  // DCT-II, the diagonal operator mid_op and DCT-III, then DCT-III along y, z.
-  hipError_t xmid(const Fuse& f, int mid_op, void* out) {
-    return xmid_dt(mixed ? F32 : F64, nullptr, f, mid_op, out);
+  // DCT-II, the diagonal operator mid_op and DCT-III, then DCT-III along y, z
+  // (the last z pass only when last_z).
+  hipError_t xmid(const Fuse& f, int mid_op, void* out, bool last_z = true) {
+    return xmid_dt(mixed ? F32 : F64, nullptr, f, mid_op, out, last_z);
   }
   // as xmid in precision dt; `in` is read by the z pass when f.op is OP_PLAIN
-  hipError_t xmid_dt(DType dt, const void* in, const Fuse& f, int mid_op, void* out) {
+  hipError_t xmid_dt(DType dt, const void* in, const Fuse& f, int mid_op, void* out, bool last_z = true) {
     PassIO io;
     io.in = in;
     io.out = out;
@@ -356,7 +363,7 @@ struct smaiso_handle {
     if ((e = run_pass(dt, REDFT10, 0, io, g)) != hipSuccess) return e;
     io.f.op = OP_PLAIN;
     if ((e = run_pass(dt, REDFT01, 1, io, g)) != hipSuccess) return e;
-    return run_pass(dt, REDFT01, 2, io, g);
+    return last_z ? run_pass(dt, REDFT01, 2, io, g) : hipSuccess;
   }
   // one-axis operator pass (OP_AX_*) along `axis`
   hipError_t axpass(DType dt, int axis, const void* in, void* out, const Fuse& f) {
@@ -627,12 +634,18 @@ int run_step(smaiso_handle* h) {
     g.arith32 = h->a32(SMAISO_STAGE_LPSI);
     HIPTRY(h, h->xwf(kICT, g, h->lpsi));
   }
-  // a9 (S:1312-1380)
+  // a9 (S:1312-1380); in steady steps of FP64 handles the forces pass runs
+  // the last inverse z pass of lap(rho L psi) (not in the adjoint's
+  // recomputed steps, whose transpose reads the field)
+  const bool forces_z = forces_zpass_supported(h->mixed ? F32 : F64, h->g.nz);
+  h->laprl_zlines = false;
   if (h->nloop > h->p.divv_switch || h->p.load_mode == 1) {
     Fuse g = fb;
     g.op = OP_LD_PROD; g.src[0] = h->rho; g.src[1] = h->lpsi;
     g.arith32 = h->a32(SMAISO_STAGE_LAPRHOL);
-    HIPTRY(h, h->xmid(g, OP_MID_NEGLAP, h->laprl));
+    const bool lapz = h->laprl_in_forces() && !h->keep_virr;
+    HIPTRY(h, h->xmid(g, OP_MID_NEGLAP, h->laprl, !lapz));
+    h->laprl_zlines = lapz;
   } else {
     Fuse g = fb;
     g.op = OP_LD_RELAXLAP; g.src[0] = h->psiq;
@@ -642,9 +655,10 @@ int run_step(smaiso_handle* h) {
   }
   // a10 (S:1385-1425) and a11 (S:1427-1457): the z pass of the five
   // transforms computes the forces on load where the length allows
-  if (forces_zpass_supported(h->mixed ? F32 : F64, h->g.nz)) {
+  if (forces_z) {
     ForcesIO<T> io;
     io.lpsi = static_cast<const T*>(h->lpsi); io.laprl = static_cast<const T*>(h->laprl);
+    io.laprl_zlines = h->laprl_zlines;
     io.amp = static_cast<const T*>(h->amp); io.psi = static_cast<const T*>(h->psi);
     io.rho = h->rho;
     for (int a = 0; a < 3; ++a) { io.gp[a] = gp[a]; io.rd[a] = rd[a]; io.f[a] = f[a]; io.zout[2 + a] = S[a]; }
@@ -736,7 +750,8 @@ int run_step(smaiso_handle* h) {
     io.fz = f[2];
     io.qs = fb.qs[2];
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 15 + db1 * 4);
+      // reads 15 work fields (f_z among them), writes Nl's z lines and d_z rho
+      smaiso_handle::Scope sc(h, SMAISO_KC_POINTWISE, fb1 * 17 + db1 * 4);
       HIPTRY(h, density_zpass<T>(c, io, h->r2r, s));
     }
     HIPTRY(h, h->xw_yx(kCT, h->nlq, h->a32(SMAISO_STAGE_NL)));
@@ -925,7 +940,19 @@ int field_device(smaiso_handle* h, int fid, const double** out) {
       *out = as64(h->gp[fid - SMAISO_FIELD_GPSIX]); return SMAISO_OK;
     case SMAISO_FIELD_GRHOX: case SMAISO_FIELD_GRHOY: case SMAISO_FIELD_GRHOZ:
       *out = as64(h->rd[fid - SMAISO_FIELD_GRHOX]); return SMAISO_OK;
-    case SMAISO_FIELD_LAPRHOL: *out = as64(h->laprl); return SMAISO_OK;
+    case SMAISO_FIELD_LAPRHOL:
+      if (h->laprl_zlines) {
+        // the last inverse z pass, which the step ran inside the forces pass
+        PassIO io;
+        io.in = h->laprl;
+        io.out = h->tmp;
+        io.f.arith32 = h->a32(SMAISO_STAGE_LAPRHOL);
+        HIPTRY(h, h->run_pass(h->mixed ? F32 : F64, REDFT01, 2, io, h->g));
+        *out = as64(h->tmp);
+        return SMAISO_OK;
+      }
+      *out = as64(h->laprl);
+      return SMAISO_OK;
     case SMAISO_FIELD_PSI_OLD: *out = as64(h->psi_prev); return SMAISO_OK;
     default: return fail(h, SMAISO_E_INVALID, fmt("unknown field id %d", fid));
   }
@@ -1065,7 +1092,12 @@ int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
         if (rc != SMAISO_OK) return rc;
         *parity = par;
       } else {
-        for (smaiso_handle* h : L.hs) { h->nloop += 2; h->epoch += 2; }
+        // the host state the two captured (steady) steps leave
+        for (smaiso_handle* h : L.hs) {
+          h->nloop += 2;
+          h->epoch += 2;
+          h->laprl_zlines = h->laprl_in_forces();
+        }
       }
       HIPTRY(owner, hipGraphLaunch(*gexec, L.master));
       need_fork = true;

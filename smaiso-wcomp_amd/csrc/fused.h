@@ -23,6 +23,11 @@ struct ForcesIO {
   T* mu;                // chemical potential (real space)
   T* f[3];              // forces f_a (real space)
   T* zout[5];           // z pass of fgrad, erho, S_x, S_y, S_z
+  // laprl holds the z lines of lap(rho L psi) before the last inverse z pass
+  // of its transform (REDFT01), which the kernel then runs in its own tile
+  // (FP64 storage: with FP32 storage the kernel is less memory-bound and the
+  // extra phase measured slower than the separate pass)
+  bool laprl_zlines = false;
 };
 
 // hipErrorNotSupported when nz has no fused kernel (a length off the FFT
@@ -30,6 +35,8 @@ struct ForcesIO {
 template <typename T>
 hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hipStream_t s);
 bool forces_zpass_supported(DType dt, int nz);
+// the forces pass runs lap(rho L psi)'s last inverse z pass (laprl_zlines)
+bool forces_zpass_laprl(DType dt, int nz);
 
 // a15 + a16 (S:2121-2162) fused with the z pass of the nonlinear term's
 // transform (S:2166-2168, REDFT10 along z): the density update (rho and

@@ -29,7 +29,13 @@ struct ForcesTile {
 // FFT over the 5P sequences and Makhoul's post-processing store the z pass
 // outputs (S_z is a DST-II along z: odd inputs negated, outputs reversed).
 // The tile and the FFT are FP64 for both storage types, as in the r2r passes.
-template <typename T, int N, int P, int NT>
+// LZ: io.laprl holds the z lines of lap(rho L psi) before the last inverse z
+// pass of its transform (REDFT01); the pass runs first, in the LDS sequences
+// of the first output field (Makhoul's inverse: pre-twiddle, inverse FFT),
+// and leaves x[m] of each line at position m/2 (m even) or N-1-(m-1)/2 (m
+// odd): the slot where the load phase puts point m's value of that field,
+// so each point reads its lap(rho L psi) from the slot it then overwrites.
+template <typename T, int N, int P, int NT, bool LZ>
 __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> io,
                                                       const cplx<double>* __restrict__ tw,
                                                       const cplx<double>* __restrict__ mk) {
@@ -48,6 +54,43 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
   const MemberConsts mc = c.mc[blockIdx.y];
   StageTw<A, N, NF * P, NT, MAXR> stw;
   stw.load(tw, tid);
+  cplx<A>* const lbuf = buf;  // P sequences of lap(rho L psi), then field 0's
+  const A* const lrb = rbuf;
+  if constexpr (LZ) {
+    constexpr int W = P * HN;
+    constexpr int ITW = (W + NT - 1) / NT;
+    MkSeq<A, N, P, NT> ms(mk, 2, tid);
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      const int w = tid + it * NT;
+      if (W % NT == 0 || w < W) {
+        const int cc = w / HN, k = w - cc * HN;
+        const int kb = (k == 0) ? HN : N - k;
+        cplx<A> rk = {A(0), A(0)}, rn = {A(0), A(0)};
+        const int l0 = 2 * cc;
+        if (l0 < nvalid) {
+          const T* q = io.laprl + base + (long long)l0 * N;
+          rk.x = (A)q[k];
+          rn.x = (A)q[kb];
+        }
+        if (l0 + 1 < nvalid) {
+          const T* q = io.laprl + base + (long long)(l0 + 1) * N;
+          rk.y = (A)q[k];
+          rn.y = (A)q[kb];
+        }
+        cplx<A> zk, zb;
+        makhoul_pre<A, N>(mk, k, ms.get(mk, 2, k), rk, rn, zk, zb);
+        lbuf[cc * SEQ + lp<SH>(k)] = zk;
+        lbuf[cc * SEQ + lp<SH>(kb)] = zb;
+      }
+      ms.advance(2, it);
+    }
+    StageTw<A, N, P, NT, MAXR> stl;
+#pragma unroll
+    for (int q = 0; q < StageTw<A, N, P, NT, MAXR>::kStages; ++q) stl.w[q] = stw.w[q];
+    __syncthreads();
+    fft_stage<A, N, P, NT, MAXR, +1, 0>(lbuf, tw, stl);
+  }
 
   constexpr int E = TL * N;
   constexpr int IT = (E + NT - 1) / NT;
@@ -59,7 +102,9 @@ __global__ __launch_bounds__(NT) void forces_z_kernel(StepConsts c, ForcesIO<T> 
       A v[NF] = {A(0), A(0), A(0), A(0), A(0)};
       if (l < nvalid) {
         const long long idx = base + e;
-        const A L = io.lpsi[idx], lr = io.laprl[idx], r = io.rho[idx], am = io.amp[idx], ps = io.psi[idx];
+        const int pl = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
+        const A lr = LZ ? lrb[2 * ((l >> 1) * SEQ + lp<SH>(pl)) + (l & 1)] : (A)io.laprl[idx];
+        const A L = io.lpsi[idx], r = io.rho[idx], am = io.amp[idx], ps = io.psi[idx];
         const A kp = mc.kp, zeta = mc.zeta, rho0 = mc.rho0;
         const A en = A(0.5) * (L * L);
         const A ra = rho0 + kp * am;
@@ -128,9 +173,18 @@ hipError_t launch_forces(const StepConsts& c, const ForcesIO<T>& io, const void*
   } else {
     constexpr int TL = 2 * F::P;
     const long long blocks = ((long long)c.nx * c.ny + TL - 1) / TL;
-    hipLaunchKernelGGL((forces_z_kernel<T, N, F::P, F::NT>), dim3((unsigned)blocks, (unsigned)c.nmem),
-                       dim3(F::NT), F::lds, s, c, io, static_cast<const cplx<double>*>(tw),
-                       static_cast<const cplx<double>*>(mk));
+    const dim3 grid((unsigned)blocks, (unsigned)c.nmem);
+    const auto* t = static_cast<const cplx<double>*>(tw);
+    const auto* m = static_cast<const cplx<double>*>(mk);
+    if constexpr (sizeof(T) == 8) {
+      if (io.laprl_zlines) {
+        hipLaunchKernelGGL((forces_z_kernel<T, N, F::P, F::NT, true>), grid, dim3(F::NT), F::lds, s, c, io, t, m);
+        return hipGetLastError();
+      }
+    } else {
+      if (io.laprl_zlines) return hipErrorNotSupported;
+    }
+    hipLaunchKernelGGL((forces_z_kernel<T, N, F::P, F::NT, false>), grid, dim3(F::NT), F::lds, s, c, io, t, m);
     return hipGetLastError();
   }
 }
@@ -158,6 +212,8 @@ bool fits_for(int n) {
 bool forces_zpass_supported(DType dt, int nz) {
   return dt == F64 ? fits_for<double>(nz) : fits_for<float>(nz);
 }
+
+bool forces_zpass_laprl(DType dt, int nz) { return dt == F64 && fits_for<double>(nz); }
 
 template <typename T>
 hipError_t forces_zpass(const StepConsts& c, const ForcesIO<T>& io, R2R& r2r, hipStream_t s) {
