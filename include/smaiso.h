@@ -184,7 +184,8 @@ int64_t smaiso_step_count(const smaiso_handle* h);
 #define SMAISO_STAGE_VELOCITY (1u << 7) /* v's inverse transforms (S:2094-2117) */
 #define SMAISO_STAGE_NL (1u << 8)       /* y and x passes of Nl's transform (S:2166-2168) */
 #define SMAISO_STAGE_VELSOLVE (1u << 9) /* the velocity solve's arithmetic (S:1479-1974) */
-#define SMAISO_STAGES_ALL 0x3ffu
+#define SMAISO_STAGE_GRAD_RHO (1u << 10) /* grad rho's y and x passes from the FP64 rho (S:1104-1250) */
+#define SMAISO_STAGES_ALL 0x7ffu
 
 /* The FP32-arithmetic stages of a SMAISO_FP32_MIXED handle (SMAISO_STAGE_*
  * bits; the default is smaiso_fp32_stages_default()).  Ignored by FP64

@@ -60,7 +60,7 @@ struct PendingTiming {
 // each reach 3e-5, L psi 1e-6.
 constexpr uint32_t kFp32StagesDefault = SMAISO_STAGE_FORCES | SMAISO_STAGE_DIVV | SMAISO_STAGE_DIVF |
                                         SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL | SMAISO_STAGE_VELSOLVE |
-                                        SMAISO_STAGE_AMP;
+                                        SMAISO_STAGE_AMP | SMAISO_STAGE_GRAD_RHO;
 
 double derived_amp(const smaiso_params& p) {
   return std::sqrt((3 * p.beta + std::sqrt(9 * p.beta * p.beta + 40 * p.epsilon * p.gamma)) / 5);
@@ -583,8 +583,15 @@ int run_step(smaiso_handle* h) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
-    if (h->mixed) HIPTRY(h, h->axpass_cast(F64, F32, a, h->rho, rd[a], g));
-    else HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
+    if (h->mixed) {
+      // FP32 arithmetic (the 256-point y and x passes) on rho - rho0: the
+      // gradient ignores the constant, the FP32 tile rounds the difference
+      g.arith32 = h->a32(SMAISO_STAGE_GRAD_RHO);
+      g.ld_shift = h->mc_host[0].rho0;
+      HIPTRY(h, h->axpass_cast(F64, F32, a, h->rho, rd[a], g));
+    } else {
+      HIPTRY(h, h->axpass(F64, a, h->rho, rd[a], g));
+    }
   }
   // grad psi along x and y, then one z pass forms d_z psi, |amp| at the
   // same points and the z factor of a7's separable Gaussian filter (d_z psi

@@ -74,8 +74,14 @@ struct Fuse {
   int accumulate = 0;             // OP_AX_DIV: add the result to out
   // FP32 storage only: compute the pass in FP32 (FP32 tile and tables)
   // where the length has such instances (r2r_arith32_supported); otherwise,
-  // and for FP64 storage, the pass computes in FP64
+  // and for FP64 storage, the pass computes in FP64.  Also the gradient pass
+  // from FP64 to FP32 storage along y or x at length 256 (R2R::pass_cast).
   int arith32 = 0;
+  // OP_AX_GRAD from FP64 input in FP32 arithmetic: subtracted from the input
+  // before it is rounded to the FP32 tile.  The operator never reads mode 0,
+  // so a constant leaves the result unchanged in exact arithmetic while a
+  // constant near the field's mean shrinks the rounding of the tile.
+  double ld_shift = 0;
 };
 
 // Lengths with FP32-arithmetic instances of the transform passes.

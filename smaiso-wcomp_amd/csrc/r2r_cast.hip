@@ -92,7 +92,15 @@ hipError_t R2R::pass_cast(DType in_dt, DType out_dt, int kind, int axis, const P
   }
   const int n = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
   Tables* tb = nullptr;
-  hipError_t e = tables(F64, n, &tb);
+  hipError_t e;
+  if (op == OP_AX_GRAD && io.f.arith32 && n == 256 && axis != 2) {
+    // FP32 arithmetic over the FP64 input (256-point y/x passes only; other
+    // lengths compute in FP64)
+    if ((e = tables(F32, n, &tb)) != hipSuccess) return e;
+    e = launch_yx256(F64, F32, kind, axis, io, g, tb->tw, tb->mk, s, true);
+    if (e != hipErrorNotSupported) return e;
+  }
+  e = tables(F64, n, &tb);
   if (e != hipSuccess) return e;
   if (in_dt == F32) return dispatch_cast<float, double>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
   return dispatch_cast<double, float>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);

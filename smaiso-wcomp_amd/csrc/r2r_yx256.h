@@ -324,8 +324,15 @@ __global__ __launch_bounds__(16 * P, (ax_min_waves<T, A>())) void axop_kernel(Ar
     {
       cplx<T> ld[16];
       load(ld, t, j_t);
+      if constexpr (OP == OP_AX_GRAD && sizeof(T) > sizeof(A)) {
+        // FP64 input rounded to the FP32 tile after the constant shift
+        const T sh = (T)f.ld_shift;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = C{(A)ld[q].x, (A)ld[q].y};
+        for (int q = 0; q < 16; ++q) v[q] = C{(A)(ld[q].x - sh), (A)(ld[q].y - sh)};
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = C{(A)ld[q].x, (A)ld[q].y};
+      }
     }
     // ---- forward: DST-II negates the odd rows (q >= 8)
     if constexpr (FS) {

@@ -62,6 +62,11 @@ hipError_t launch_op(const PassIO& io, const Geom& g, const void* tw, const void
 template <typename T, typename TO, int AXIS, typename A>
 hipError_t launch_axis(int kind, const PassIO& io, const Geom& g, const void* tw, const void* mk, hipStream_t s) {
   const int op = io.f.op;
+  if constexpr (sizeof(T) > sizeof(A)) {
+    // FP64 input in FP32 arithmetic: the gradient only
+    if (op != OP_AX_GRAD || io.f.axis != AXIS || io.f.qs[AXIS] == 0.0) return hipErrorNotSupported;
+    return launch_op<T, TO, AXIS, OP_AX_GRAD, A>(io, g, tw, mk, s);
+  }
   if (op == OP_PLAIN) {
     if (kind == REDFT10 || kind == RODFT10) return launch<T, TO, AXIS, yx256::kFwd, A>(io, g, kind, tw, mk, s);
     return launch<T, TO, AXIS, yx256::kInv, A>(io, g, kind, tw, mk, s);
@@ -101,6 +106,7 @@ hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const Pas
                         const void* tw, const void* mk, hipStream_t s, bool arith32) {
   if (axis != 0 && axis != 1) return hipErrorNotSupported;
   if (arith32) {
+    if (in_dt == F64 && out_dt == F32) return launch_typed<double, float, float>(kind, axis, io, g, tw, mk, s);
     if (in_dt != F32 || out_dt != F32) return hipErrorNotSupported;
     return launch_typed<float, float, float>(kind, axis, io, g, tw, mk, s);
   }

@@ -35,7 +35,8 @@ FIELDS = {
 TILE_VARIANTS = 4
 # stages of the FP32-mixed step that may compute in FP32 (SMAISO_STAGE_*)
 STAGES = {"grad_psi": 1 << 0, "amp": 1 << 1, "lpsi": 1 << 2, "laprhol": 1 << 3, "forces": 1 << 4,
-          "divv": 1 << 5, "divf": 1 << 6, "velocity": 1 << 7, "nl": 1 << 8, "velsolve": 1 << 9}
+          "divv": 1 << 5, "divf": 1 << 6, "velocity": 1 << 7, "nl": 1 << 8, "velsolve": 1 << 9,
+          "grad_rho": 1 << 10}
 
 
 def fp32_stages_default() -> int:

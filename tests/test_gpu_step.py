@@ -365,7 +365,7 @@ def test_mixed_fp32_stages_one_step_every_field(mask):
 def test_fp32_stages_api():
     assert S.fp32_stages_default() == (S.STAGES["amp"] | S.STAGES["forces"] | S.STAGES["divv"]
                                        | S.STAGES["divf"] | S.STAGES["velocity"] | S.STAGES["nl"]
-                                       | S.STAGES["velsolve"])
+                                       | S.STAGES["velsolve"] | S.STAGES["grad_rho"])
     sp, op = config1(32)
     psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=3))
     a, b = S.Stepper(sp), S.Stepper(sp)
