@@ -307,6 +307,15 @@ int smaiso_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, con
                    int32_t nsteps, int32_t checkpoint_every, double* loss, double* dpsi0,
                    double* drho0, double* dzeta, double* dnu);
 
+/* enable != 0 (the default): while smaiso_adjoint recomputes a segment it
+ * keeps each step's intermediates (20 work fields and two FP64 fields per
+ * step, checkpoint_every steps), so every step is recomputed once; it does
+ * so only when they fit in free device memory with 1 GiB to spare.
+ * enable == 0, or no room: each step is recomputed again from its stored
+ * state before its transpose (less memory, one more forward step per step).
+ * No reference counterpart. */
+int smaiso_set_adjoint_store(smaiso_handle* h, int32_t enable);
+
 /* dL/dh(x, y) of the interface initial condition (smaiso_make_interface_ic)
  * from dL/dpsi and dL/drho of its output: sum over z of (amplitude cos(q0 z)
  * dpsi + (rho_sm - rho_iso) drho) * d profile/dh, d profile/dh =

@@ -90,6 +90,10 @@ struct smaiso_handle {
   // steps read it)
   long long virr_epoch = -1;
   bool keep_virr = false;
+  // the adjoint stores the intermediates of each recomputed segment's steps
+  // when they fit in free device memory (otherwise, or when off, it
+  // recomputes each step before transposing it)
+  bool adj_store = true;
   // divf holds the y and x terms of div f only: the density pass formed the z
   // term from f_z (still in f[2] until the next step)
   bool divf_partial = false;
@@ -1547,6 +1551,71 @@ struct StateStore {
   void* psi(int k) const { return static_cast<char*>(mem[k]) + (32 + tsz) * n3; }
 };
 
+// The forward intermediates that the transposed step reads (AdjArgs), for
+// `count` steps: 20 work fields of the storage type, then rho at the step's
+// start and after its density update (FP64).
+struct StepStore {
+  static constexpr int kT = 20;  // psi, amp, lpsi, laprl, divv, gp[3], rd[3], v[3], vtq[3], virrq[3]
+  std::vector<void*> mem;
+  size_t n3 = 0, tsz = 8;
+  int count = 0;
+  ~StepStore() { release(); }
+  void release() {
+    for (void* m : mem) (void)hipFree(m);
+    mem.clear();
+    count = 0;
+  }
+  static size_t bytes_per_step(long long points, size_t es) { return (size_t)points * (kT * es + 16); }
+  hipError_t alloc(int n, long long points, size_t es) {
+    n3 = (size_t)points;
+    tsz = es;
+    for (int k = 0; k < n; ++k) {
+      void* m = nullptr;
+      hipError_t e = hipMalloc(&m, bytes_per_step(points, es));
+      if (e != hipSuccess) return e;
+      mem.push_back(m);
+      count = k + 1;
+    }
+    return hipSuccess;
+  }
+  void* tf(int k, int f) const { return static_cast<char*>(mem[k]) + (size_t)f * n3 * tsz; }
+  double* rho(int k, int which) const {  // which: 0 at the step's start, 1 after it
+    return reinterpret_cast<double*>(static_cast<char*>(mem[k]) + kT * n3 * tsz) + (size_t)which * n3;
+  }
+};
+
+// Copies the intermediates of the step the handle has just run, and its new
+// rho, into slot k (the caller stores rho at the step's start).  The step
+// ran as any steady step does, so v^ and virr^ are solved again from the
+// S^, fgrad^ and erho^ it left (the same solve), and lap(rho L psi) gets its
+// last inverse z pass where the forces pass ran it.
+hipError_t save_step(smaiso_handle* h, const StepStore& st, int k) {
+  const size_t tb = st.n3 * st.tsz, db = st.n3 * 8;
+  hipStream_t s = h->stream;
+  hipError_t e = h->mixed ? solve_virr<float>(h) : solve_virr<double>(h);
+  if (e != hipSuccess) return e;
+  h->virr_epoch = h->epoch;
+  if (h->laprl_zlines) {
+    PassIO io;
+    io.in = h->laprl;
+    io.out = st.tf(k, 3);
+    io.f.arith32 = h->a32(SMAISO_STAGE_LAPRHOL);
+    if ((e = h->run_pass(h->mixed ? F32 : F64, REDFT01, 2, io, h->g)) != hipSuccess) return e;
+  }
+  const void* src[StepStore::kT] = {h->psi_prev, h->amp, h->lpsi, h->laprl, h->divv};
+  for (int a = 0; a < 3; ++a) {
+    src[5 + a] = h->gp[a];
+    src[8 + a] = h->rd[a];
+    src[11 + a] = h->v[a];
+    src[14 + a] = h->vtq[a];
+    src[17 + a] = h->virrq[a];
+  }
+  for (int f = 0; f < StepStore::kT && e == hipSuccess; ++f)
+    if (f != 3 || !h->laprl_zlines) e = hipMemcpyAsync(st.tf(k, f), src[f], tb, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st.rho(k, 1), h->rho, db, hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
 hipError_t save_state(smaiso_handle* h, const StateStore& st, int k) {
   const size_t db = st.n3 * 8, tb = st.n3 * st.tsz;
   hipStream_t s = h->stream;
@@ -1600,24 +1669,43 @@ int adj_alloc(smaiso_handle* h) {
   return SMAISO_OK;
 }
 
+// The forward intermediates come from the handle's work fields after it has
+// recomputed the step (st null) or from slot k of a step store.
 template <typename T>
-AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
+AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first, const StepStore* st, int k) {
   AdjArgs<T> A{};
   auto tp = [](void* p) { return static_cast<const T*>(p); };
-  A.psi = tp(h->psi_prev);  // the step swapped psi and psi_prev
-  A.amp = tp(h->amp);
-  A.lpsi = tp(h->lpsi);
-  A.laprl = tp(h->laprl);
-  A.divv = tp(h->divv);
-  for (int a = 0; a < 3; ++a) {
-    A.gp[a] = tp(h->gp[a]);
-    A.rd[a] = tp(h->rd[a]);
-    A.v[a] = tp(h->v[a]);
-    A.vtq[a] = tp(h->vtq[a]);
-    A.virrq[a] = tp(h->virrq[a]);
+  if (st) {
+    A.psi = tp(st->tf(k, 0));
+    A.amp = tp(st->tf(k, 1));
+    A.lpsi = tp(st->tf(k, 2));
+    A.laprl = tp(st->tf(k, 3));
+    A.divv = tp(st->tf(k, 4));
+    for (int a = 0; a < 3; ++a) {
+      A.gp[a] = tp(st->tf(k, 5 + a));
+      A.rd[a] = tp(st->tf(k, 8 + a));
+      A.v[a] = tp(st->tf(k, 11 + a));
+      A.vtq[a] = tp(st->tf(k, 14 + a));
+      A.virrq[a] = tp(st->tf(k, 17 + a));
+    }
+    A.rho = st->rho(k, 0);
+    A.rho_new = st->rho(k, 1);
+  } else {
+    A.psi = tp(h->psi_prev);  // the step swapped psi and psi_prev
+    A.amp = tp(h->amp);
+    A.lpsi = tp(h->lpsi);
+    A.laprl = tp(h->laprl);
+    A.divv = tp(h->divv);
+    for (int a = 0; a < 3; ++a) {
+      A.gp[a] = tp(h->gp[a]);
+      A.rd[a] = tp(h->rd[a]);
+      A.v[a] = tp(h->v[a]);
+      A.vtq[a] = tp(h->vtq[a]);
+      A.virrq[a] = tp(h->virrq[a]);
+    }
+    A.rho = rho_n;
+    A.rho_new = h->rho;
   }
-  A.rho = rho_n;
-  A.rho_new = h->rho;
   double* const* f = h->adj.data();
   A.bP = f[AF_P]; A.bPo = f[AF_PO]; A.bR = f[AF_R]; A.bFr = f[AF_FR]; A.bNo = f[AF_NO];
   A.bnl = f[AF_NL]; A.bpsi = f[AF_PSI]; A.bamp = f[AF_AMP]; A.blap = f[AF_LAP];
@@ -1634,14 +1722,15 @@ AdjArgs<T> adj_args(smaiso_handle* h, const double* rho_n, bool first) {
 }
 
 // Transpose of the step that the handle has just recomputed from the stored
-// input state (rho at its start is rho_n): the adjoint of the step's outputs,
-// held in the adjoint state fields, becomes that of its inputs.
+// input state (rho at its start is rho_n), or of the step in slot k of st:
+// the adjoint of the step's outputs, held in the adjoint state fields,
+// becomes that of its inputs.
 template <typename T>
-int adjoint_step(smaiso_handle* h, const double* rho_n, bool first) {
+int adjoint_step(smaiso_handle* h, const double* rho_n, bool first, const StepStore* st, int k) {
   using K = AdjKernels<T>;
   const StepConsts c = h->consts();
   hipStream_t s = h->stream;
-  const AdjArgs<T> A = adj_args<T>(h, rho_n, first);
+  const AdjArgs<T> A = adj_args<T>(h, rho_n, first, st, k);
   const int nb = (int)adj_blocks(c);
   Fuse fb = h->fuse_base();
   fb.op = OP_PLAIN;
@@ -1698,8 +1787,9 @@ int adjoint_step(smaiso_handle* h, const double* rho_n, bool first) {
   return SMAISO_OK;
 }
 
-int adjoint_step_any(smaiso_handle* h, const double* rho_n, bool first) {
-  return h->mixed ? adjoint_step<float>(h, rho_n, first) : adjoint_step<double>(h, rho_n, first);
+int adjoint_step_any(smaiso_handle* h, const double* rho_n, bool first, const StepStore* st = nullptr,
+                     int k = 0) {
+  return h->mixed ? adjoint_step<float>(h, rho_n, first, st, k) : adjoint_step<double>(h, rho_n, first, st, k);
 }
 
 int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const double* target,
@@ -1729,7 +1819,6 @@ int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const 
   const int ncp = nsteps > 0 ? (nsteps + every - 1) / every : 0;
   StateStore cps, seg;
   HIPTRY(h, cps.alloc(ncp, n3, h->tsz));
-  HIPTRY(h, seg.alloc(std::min(every, std::max(nsteps, 1)), n3, h->tsz));
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0) HIPTRY(h, save_state(h, cps, n / every));
     rc = run_step_any(h);
@@ -1746,8 +1835,39 @@ int run_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, const 
   for (int f : {AF_PO, AF_R, AF_FR, AF_NO}) fill_zero_d(F[f], n3, s);
 
   // reverse sweep: recompute each segment from its checkpoint, then
-  // transpose its steps last to first
-  for (int sg = ncp - 1; sg >= 0; --sg) {
+  // transpose its steps last to first.  With a step store the recomputation
+  // keeps every step's intermediates (one forward step per transposed
+  // step); without one each step is recomputed again from its stored state
+  // before its transpose.
+  const int seg_len = std::min(every, std::max(nsteps, 1));
+  StepStore inter;
+  if (h->adj_store && nsteps > 0) {
+    size_t free_b = 0, total_b = 0;
+    HIPTRY(h, hipMemGetInfo(&free_b, &total_b));
+    const size_t need = StepStore::bytes_per_step(n3, h->tsz) * (size_t)seg_len;
+    const size_t reserve = (size_t)1 << 30;
+    if (need + reserve < free_b && inter.alloc(seg_len, n3, h->tsz) != hipSuccess) {
+      (void)hipGetLastError();  // fall back to recomputation
+      inter.release();
+    }
+  }
+  const bool stored = inter.count == seg_len;
+  if (!stored) HIPTRY(h, seg.alloc(seg_len, n3, h->tsz));
+  for (int sg = ncp - 1; stored && sg >= 0; --sg) {
+    const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
+    HIPTRY(h, restore_state(h, cps, sg, n0));
+    for (int n = n0; n < n1; ++n) {
+      HIPTRY(h, hipMemcpyAsync(inter.rho(n - n0, 0), h->rho, db, hipMemcpyDeviceToDevice, s));
+      rc = run_step_any(h);  // the forward sweep's step, so the states match it
+      if (rc != SMAISO_OK) return rc;
+      HIPTRY(h, save_step(h, inter, n - n0));
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      rc = adjoint_step_any(h, inter.rho(n - n0, 0), n == 0, &inter, n - n0);
+      if (rc != SMAISO_OK) return rc;
+    }
+  }
+  for (int sg = ncp - 1; !stored && sg >= 0; --sg) {
     const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
     HIPTRY(h, restore_state(h, cps, sg, n0));
     for (int n = n0; n < n1; ++n) {
@@ -1797,6 +1917,12 @@ int smaiso_adjoint(smaiso_handle* h, const double* psi0, const double* rho0, con
   if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
   if (!psi0 || !rho0 || !target) return fail(h, SMAISO_E_INVALID, "psi0/rho0/target is NULL");
   return run_adjoint(h, psi0, rho0, target, nsteps, checkpoint_every, loss, dpsi0, drho0, dzeta, dnu);
+}
+
+int smaiso_set_adjoint_store(smaiso_handle* h, int32_t enable) {
+  if (!h) return fail(nullptr, SMAISO_E_INVALID, "handle is NULL");
+  h->adj_store = enable != 0;
+  return SMAISO_OK;
 }
 
 

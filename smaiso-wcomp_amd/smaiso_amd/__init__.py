@@ -115,6 +115,7 @@ def lib():
         L.smaiso_timing_enable.argtypes = [vp, C.c_int32]
         L.smaiso_timing_get.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64), dp]
         L.smaiso_set_graph.argtypes = [vp, C.c_int32]
+        L.smaiso_set_adjoint_store.argtypes = [vp, C.c_int32]
         L.smaiso_get_slice.argtypes = [vp, C.c_int32, C.c_int32, dp]
         L.smaiso_surface_height.argtypes = [vp, C.c_double, dp]
         L.smaiso_save_checkpoint.argtypes = [vp, C.c_char_p]
@@ -145,7 +146,8 @@ def lib():
                      "smaiso_timing_get", "smaiso_set_graph", "smaiso_ensemble_create",
                      "smaiso_ensemble_destroy", "smaiso_ensemble_groups", "smaiso_ensemble_set_state",
                      "smaiso_ensemble_set_graph", "smaiso_ensemble_step", "smaiso_ensemble_get_field",
-                     "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_interface_ic_vjp",
+                     "smaiso_ensemble_diagnostics", "smaiso_adjoint", "smaiso_set_adjoint_store",
+                     "smaiso_interface_ic_vjp",
                      "smaiso_autotune", "smaiso_ensemble_autotune", "smaiso_r2r3d_tiles",
                      "smaiso_get_slice", "smaiso_surface_height", "smaiso_save_checkpoint",
                      "smaiso_load_checkpoint", "smaiso_read_psi_text"):
@@ -353,6 +355,12 @@ class Stepper:
                                     int(checkpoint_every), C.byref(loss), _dp(dpsi), _dp(drho),
                                     C.byref(dz), C.byref(dn)), self.h)
         return AdjointResult(loss.value, dpsi, drho, dz.value, dn.value)
+
+    def set_adjoint_store(self, enable: bool):
+        """Keep each recomputed step's intermediates in the adjoint (default)
+        or recompute every step again before its transpose
+        (smaiso_set_adjoint_store)."""
+        _check(lib().smaiso_set_adjoint_store(self.h, 1 if enable else 0), self.h)
 
     def timing(self, enable: bool):
         _check(lib().smaiso_timing_enable(self.h, 1 if enable else 0), self.h)

@@ -136,6 +136,29 @@ def test_adjoint_errors_and_state():
     st2.step(1)
 
 
+@pytest.mark.parametrize("precision,tol", [(S.FP64, 1e-11), (S.FP32_MIXED, 1e-6)])
+def test_adjoint_step_store_matches_recompute(precision, tol):
+    """Intermediates kept while a segment is recomputed (the default) against
+    recomputing every step again before its transpose: the same gradients,
+    up to the rounding by which the two recomputations differ (the first
+    runs the forward sweep's steps and solves virr^ afterwards, the second
+    runs the steps that keep virr^).  nz = 32 puts lap(rho L psi)'s last z
+    pass in the forces pass of the FP64 step."""
+    sp = S.Params(nx=24, ny=16, nz=32, dt=0.01, rho0=0.85, rho_m=1.0, divv_switch=0, nu=0.7, zeta=1.3,
+                  lam=0.9, precision=precision)
+    _, _, psi0, rho0, target = _problem(sp)
+    a = S.Stepper(sp).adjoint(psi0, rho0, target, 9, 4)
+    st = S.Stepper(sp)
+    st.set_adjoint_store(False)
+    b = st.adjoint(psi0, rho0, target, 9, 4)
+    assert a.loss == b.loss  # the forward sweep is the same
+    for name, got, want in (("psi0", a.psi0, b.psi0), ("rho0", a.rho0, b.rho0)):
+        mx, l2 = rel_err(got, want)
+        assert mx < tol and l2 < tol, (name, mx, l2)
+    assert abs(a.zeta - b.zeta) <= tol * abs(b.zeta), (a.zeta, b.zeta)
+    assert abs(a.nu - b.nu) <= tol * abs(b.nu), (a.nu, b.nu)
+
+
 def test_adjoint_fp32_mixed_close_to_fp64():
     """The FP32-mixed trajectory's gradients against the FP64 ones (the
     adjoint fields are FP64 in both)."""
