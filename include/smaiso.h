@@ -175,16 +175,21 @@ int64_t smaiso_step_count(const smaiso_handle* h);
  * line relative to the line's norm, so it is kept to the stages where that
  * rounding does not reach psi. */
 #define SMAISO_STAGE_GRAD_PSI (1u << 0) /* grad psi, one pass per axis (S:1104-1236) */
-#define SMAISO_STAGE_AMP (1u << 1)      /* Gaussian amplitude filter (S:1254-1289) */
+#define SMAISO_STAGE_AMP (1u << 1)      /* Gaussian amplitude filter's factors (S:1254-1289);
+                                         * d_z psi, formed in the same z pass, stays FP64 */
 #define SMAISO_STAGE_LPSI (1u << 2)     /* L psi (S:1293-1308) */
 #define SMAISO_STAGE_LAPRHOL (1u << 3)  /* lap(rho L psi) (S:1312-1342) */
 #define SMAISO_STAGE_FORCES (1u << 4)   /* y and x passes of fgrad, erho, S_a (S:1427-1457) */
-#define SMAISO_STAGE_DIVV (1u << 5)     /* div v and its transform (S:2042-2088) */
-#define SMAISO_STAGE_DIVF (1u << 6)     /* div f, one pass per axis (S:2042-2092) */
+#define SMAISO_STAGE_DIVV (1u << 5)     /* div v's transform (S:2042-2088); its z pass, shared
+                                         * with v's in the fused velocity pass, is FP32 only
+                                         * with SMAISO_STAGE_VELOCITY too */
+#define SMAISO_STAGE_DIVF (1u << 6)     /* div f's y and x passes (S:2042-2092); the z term is
+                                         * formed in FP64 by the density pass */
 #define SMAISO_STAGE_VELOCITY (1u << 7) /* v's inverse transforms (S:2094-2117) */
 #define SMAISO_STAGE_NL (1u << 8)       /* y and x passes of Nl's transform (S:2166-2168) */
 #define SMAISO_STAGE_VELSOLVE (1u << 9) /* the velocity solve's arithmetic (S:1479-1974) */
-#define SMAISO_STAGE_GRAD_RHO (1u << 10) /* grad rho's y and x passes from the FP64 rho (S:1104-1250) */
+#define SMAISO_STAGE_GRAD_RHO (1u << 10) /* grad rho's y and x passes from the FP64 rho, at
+                                          * length 256 (S:1104-1250) */
 #define SMAISO_STAGES_ALL 0x7ffu
 
 /* The FP32-arithmetic stages of a SMAISO_FP32_MIXED handle (SMAISO_STAGE_*

@@ -55,9 +55,10 @@ struct PendingTiming {
 // whose FP32 rounding keeps psi within 3e-7 of the FP64 stepper at 256^3 and
 // 512x512x256 after 100, 500 and 2000 steps: forces, div v, div f, v's
 // transforms, Nl and the velocity solve (psi within 4.8e-8 with these alone,
-// the FP32 storage level) and the amplitude filter's Gaussian factors (psi
-// within 2.8e-7 at step 500, 1.4e-7 at 2000).  grad psi and lap(rho L psi)
-// each reach 3e-5, L psi 1e-6.
+// the FP32 storage level), the amplitude filter's Gaussian factors (psi
+// within 2.8e-7 at step 500, 1.4e-7 at 2000) and grad rho's y and x passes
+// (from rho - rho0: psi within 2.8e-7 at step 500, 1.5e-7 at 2000).  grad psi
+// and lap(rho L psi) each reach 3e-5, L psi 1e-6.
 constexpr uint32_t kFp32StagesDefault = SMAISO_STAGE_FORCES | SMAISO_STAGE_DIVV | SMAISO_STAGE_DIVF |
                                         SMAISO_STAGE_VELOCITY | SMAISO_STAGE_NL | SMAISO_STAGE_VELSOLVE |
                                         SMAISO_STAGE_AMP | SMAISO_STAGE_GRAD_RHO;
