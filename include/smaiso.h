@@ -42,7 +42,8 @@ extern "C" {
 #define SMAISO_FP64 0 /* every field, transform and kernel in FP64 */
 /* FP32 storage of the work fields (psi, grad psi, grad rho, amp, L psi, mu,
  * forces, velocities, Nl, ...); FP64 state (psiq, psiq_old), rho and f_rho;
- * grad rho formed in FP64 from the FP64 rho;
+ * grad rho formed from the FP64 rho (its y and x passes in FP32 arithmetic
+ * under SMAISO_STAGE_GRAD_RHO, its z component in FP64 by the density pass);
  * FP64 arithmetic in the forces and density kernels, the psi update, the
  * implicit denominators and the compensated reductions; the transform that
  * forms psi from psiq keeps FP64 intermediates.  Transform passes and the
