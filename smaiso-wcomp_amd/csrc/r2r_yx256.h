@@ -20,9 +20,9 @@
 // and loads the next tile into registers before transforming the current
 // one, so its loads overlap its own arithmetic and stores.  The threads hold
 // 32 complex values (register caps in min_waves).
-// The one-axis operator passes (forward, operator, inverse per tile) stay on
-// fft_pass_kernel: built on this layout they measured slower (100 against 85
-// us at 256^3), with or without the next tile's prefetch.
+// The one-axis operator passes (axop_kernel below) run forward transform,
+// operator and inverse transform per tile on the same layout, with the
+// operator applied in registers between the two.
 #pragma once
 
 #include "r2r_kernels.h"
