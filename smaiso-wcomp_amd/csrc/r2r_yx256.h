@@ -35,6 +35,17 @@ constexpr int kN = 256;
 // first-stage writes of a thread are 16 consecutive items) and one at the end
 constexpr int kSeq = kN + kN / 16 + 1;
 __device__ __forceinline__ int slot(int x) { return x + (x >> 4); }
+// slot(16 j + q) and slot(j + 16 q) for 0 <= j, q < 16, written so that the
+// q-dependence is a constant: the 16 accesses of a thread then share one
+// address register with immediate offsets (as slot(), the compiler kept one
+// register per access)
+__device__ __forceinline__ int row_slot(int j, int q) { return 17 * j + q; }
+__device__ __forceinline__ int col_slot(int j, int q) { return j + 17 * q; }
+// slot of the partner 256 - k of k = j + 16 q (k = 0 pairs with itself):
+// 271 - j - 17 q for j > 0, 272 - 17 q for j = 0 < q
+__device__ __forceinline__ int partner_slot(int j, int q) {
+  return (j == 0 && q == 0) ? 0 : (j == 0 ? 272 : 271 - j) - 17 * q;
+}
 
 // pass structure
 constexpr int kFwd = 0;  // DCT-II / DST-II (kind)
@@ -183,7 +194,7 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
     if constexpr (MODE == kInv) {
       // pre-twiddle: z_k = conj(mk[k]) (Y0_k + Y1_{N-k} + i (Y1_k - Y0_{N-k})), Y_N = 0
 #pragma unroll
-      for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
+      for (int q = 0; q < 16; ++q) s[col_slot(j, q)] = v[q];
       __syncthreads();
       C m = mk0_t;
 #pragma unroll
@@ -191,8 +202,8 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
         const int k = j_t + 16 * q;
         // FP64: both from LDS, so the loaded values need no registers past
         // the barrier
-        const C rk = sizeof(A) == 8 ? s[slot(k)] : v[q];
-        const C rn = k == 0 ? C{A(0), A(0)} : s[slot((N - k) & (N - 1))];
+        const C rk = sizeof(A) == 8 ? s[col_slot(j_t, q)] : v[q];
+        const C rn = k == 0 ? C{A(0), A(0)} : s[partner_slot(j_t, q)];
         v[q] = cmul(C{m.x, -m.y}, C{rk.x + rn.y, rk.y - rn.x});
         m = cmul(m, mks_t);
       }
@@ -201,10 +212,10 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
       dft<A, 16, +1>(v);
       __syncthreads();  // every read of this tile's slots is done
 #pragma unroll
-      for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+      for (int q = 0; q < 16; ++q) s[row_slot(j, q)] = v[q];
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+      for (int q = 0; q < 16; ++q) v[q] = s[col_slot(j, q)];
       stage_twiddle(v, wi_t);
       dft<A, 16, +1>(v);
 #pragma unroll
@@ -221,22 +232,22 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
       }
       dft<A, 16, -1>(v);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+      for (int q = 0; q < 16; ++q) s[row_slot(j, q)] = v[q];
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+      for (int q = 0; q < 16; ++q) v[q] = s[col_slot(j, q)];
       stage_twiddle(v, wf_t);
       dft<A, 16, -1>(v);
       // v[q] = Z[j + 16 q]; the slots written are the ones this thread read
 #pragma unroll
-      for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
+      for (int q = 0; q < 16; ++q) s[col_slot(j, q)] = v[q];
       __syncthreads();
       // Y_k = (Re(mk[k] (Z_k + conj Z_{N-k})), Im(mk[k] (Z_k - conj Z_{N-k})))
       C m = mk0_t;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int k = j_t + 16 * q;
-        const C za = v[q], pa = s[slot((N - k) & (N - 1))];
+        const C za = v[q], pa = s[partner_slot(j_t, q)];
         const A y0 = m.x * (za.x + pa.x) - m.y * (za.y - pa.y);
         const A y1 = m.x * (za.y + pa.y) + m.y * (za.x - pa.x);
         st2(o + (sine ? N - 1 - k : k) * stride, y0, y1, false);
@@ -341,23 +352,23 @@ __global__ __launch_bounds__(16 * P, (ax_min_waves<T, A>())) void axop_kernel(Ar
     }
     dft<A, 16, -1>(v);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+    for (int q = 0; q < 16; ++q) s[row_slot(j, q)] = v[q];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+    for (int q = 0; q < 16; ++q) v[q] = s[col_slot(j, q)];
     stage_twiddle(v, wf_t);
     dft<A, 16, -1>(v);
     // v[q] = Z[j + 16 q], into the slots this thread read
 #pragma unroll
-    for (int q = 0; q < 16; ++q) s[slot(j + 16 * q)] = v[q];
+    for (int q = 0; q < 16; ++q) s[col_slot(j, q)] = v[q];
     __syncthreads();
     // ---- forward outputs at k and N - k, operator, inverse pre-twiddle
     C m = mk0_t;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = j_t + 16 * q;
-      const int kn = (N - k) & (N - 1);  // partner; k = 0 and k = 128 pair with themselves
-      const C za = v[q], pa = s[slot(kn)];
+      // partner N - k (k = 0 and k = 128 pair with themselves)
+      const C za = v[q], pa = s[partner_slot(j_t, q)];
       // Makhoul outputs (both lines of the pair): Y at k with mk[k], and at
       // N - k with mk[N - k] = (-mk[k].y, -mk[k].x)
       const C mb = k == 0 ? m : C{-m.y, -m.x};
@@ -400,10 +411,10 @@ __global__ __launch_bounds__(16 * P, (ax_min_waves<T, A>())) void axop_kernel(Ar
     dft<A, 16, +1>(v);
     __syncthreads();  // every partner read of this tile is done
 #pragma unroll
-    for (int q = 0; q < 16; ++q) s[slot(16 * j + q)] = v[q];
+    for (int q = 0; q < 16; ++q) s[row_slot(j, q)] = v[q];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = s[slot(j + 16 * q)];
+    for (int q = 0; q < 16; ++q) v[q] = s[col_slot(j, q)];
     stage_twiddle(v, wi_t);
     dft<A, 16, +1>(v);
     // u_n goes to row 2n (q < 8) or 2N - 1 - 2n (q >= 8), negated there for
