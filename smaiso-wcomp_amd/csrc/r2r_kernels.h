@@ -212,6 +212,15 @@ constexpr int pad_shift(int n, int maxr) {
 constexpr int seq_len(int n, int maxr) { return n + (n >> pad_shift(n, maxr)) + 1; }
 template <int SH>
 __device__ __forceinline__ int lp(int x) { return x + (x >> SH); }
+// lp(x0 + q STEP) for x0 >= 0, as lp(x0) plus a constant where STEP is a
+// multiple of 2^SH, so that a thread's accesses for q = 0, 1, ... share one
+// address register with immediate offsets (written as lp of the sum, the
+// compiler keeps one register per access)
+template <int SH, int STEP>
+__device__ __forceinline__ int lp_step(int x0, int q) {
+  if constexpr (STEP % (1 << SH) == 0) return lp<SH>(x0) + q * (STEP + (STEP >> SH));
+  else return lp<SH>(x0 + q * STEP);
+}
 
 // T: storage type of the input and of the load ops' work-type sources;
 // TO: storage type of the output (T unless a pass changes the storage type)
@@ -323,7 +332,7 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
         const int p = w / J;
         const int j = w - p * J;
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + lp<SH>(j + q * J)];
+        for (int q = 0; q < R; ++q) v[it][q] = buf[p * SEQ + lp_step<SH, J>(j, q)];
         if constexpr (NS > 1) {
           // twiddles w^q, q = 1..R-1, of w = tw[k STRIDE]: one table load and
           // powers by products (a few ulp from the table's entries)
@@ -351,8 +360,12 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, const cplx<T>* __restric
         const int j = w - p * J;
         const int k = j % NS;
         const int idx = (j / NS) * NS * R + k;
+        // the first stage (NS = 1) writes R = 2^SH consecutive items from a
+        // multiple of R: one padding slot before them, none among them
+        constexpr bool kRun = NS == 1 && R == (1 << SH);
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[p * SEQ + lp<SH>(idx + q * NS)] = v[it][q];
+        for (int q = 0; q < R; ++q)
+          buf[p * SEQ + (kRun ? lp<SH>(idx) + q : lp_step<SH, NS>(idx, q))] = v[it][q];
       }
     }
     __syncthreads();
