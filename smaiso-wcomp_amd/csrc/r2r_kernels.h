@@ -1174,11 +1174,12 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
 }
 
 
-// Tile and block size per (length, axis); the LDS tile is FP64 for both
-// storage types, so T does not change the choice.  Measured at 256^3 on
-// MI355X: one wavefront per line pair along z (half the LDS of two pairs, so
-// more wavefronts per CU hide the loads), i.e. 256 elements per wavefront;
-// along y and x 8 pairs x 256 threads.
+// Tile and block size per (length, axis).  Measured at 256^3 on MI355X: one
+// wavefront per line pair along z (half the LDS of two pairs, so more
+// wavefronts per CU hide the loads), i.e. 256 elements per wavefront; along y
+// and x 8 pairs x 256 threads, whose rows are 16 columns: 128-byte segments
+// for FP64 storage.  FP32 storage takes 16 pairs x 512 threads from length
+// 64 on (the same elements per thread, 128-byte rows of 32 columns).
 template <typename T, int N>
 struct TileChoice {
   static constexpr int cap(int p) { return p < 1 ? 1 : p; }
@@ -1187,12 +1188,13 @@ struct TileChoice {
     while (q * 2 <= p) q *= 2;
     return q;
   }
+  static constexpr bool kWide = sizeof(T) == 4 && N >= 64;
   // z: about 256 elements per wavefront (one line pair at N = 256, more for
   // shorter lines so every thread keeps a butterfly)
   static constexpr int PZ = cap(budget(256));
-  static constexpr int PS = cap(budget(2048) < 8 ? budget(2048) : 8);
+  static constexpr int PS = kWide ? cap(budget(4096) < 16 ? budget(4096) : 16) : cap(budget(2048) < 8 ? budget(2048) : 8);
   static constexpr int NTZ = 64;
-  static constexpr int NTS = 256;
+  static constexpr int NTS = kWide ? 512 : 256;
 };
 
 // A plain y or x pass on the 256-point kernels of r2r_yx256.h when they
