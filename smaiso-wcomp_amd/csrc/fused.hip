@@ -312,21 +312,31 @@ __device__ __forceinline__ void z_axis_op_phase(cplx<A>* buf, const cplx<A>* __r
   }
 }
 
-// d_z of a real z line pair by the gradient one-axis operator (DCT-II along
+// Line pairs per workgroup of the density pass and of rho_dz: about 256
+// points per pair-set, a power of two (one pair from length 192 on), so that
+// short lines still give each of the 128 threads several points.
+template <int N>
+constexpr int density_pairs() {
+  int p = 256 / N, q = 1;
+  while (q * 2 <= p) q *= 2;
+  return q;
+}
+
+// d_z of P real z line pairs by the gradient one-axis operator (DCT-II along
 // z, shift by -Vs, DST-III): the values of items e = tid + it NT (line e /
 // N, element e mod N; the layout of the density pass) go into the LDS
-// sequence in Makhoul's forward order, and d_z lands in out (line l at
-// out + l N; nvalid lines).  One line pair per workgroup of NT threads.  The
-// caller has synchronized after its last use of buf.
-template <typename T, int N, int NT>
-__device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf, T* out, int nvalid,
-                                             const cplx<double>* __restrict__ tw, const cplx<double>* __restrict__ mk,
-                                             double qs) {
+// sequences in Makhoul's forward order (line pair c -> sequence c), and d_z
+// lands in out (line l at out + l N; nvalid lines).  The caller has
+// synchronized after its last use of buf.
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void dz_lines(const double* x, cplx<double>* buf, T* out, int nvalid,
+                                         const cplx<double>* __restrict__ tw, const cplx<double>* __restrict__ mk,
+                                         double qs) {
   using A = double;
-  constexpr int P = 1, HN = N / 2;
+  constexpr int HN = N / 2;
   constexpr int MAXR = pick_maxr(N, P, NT);
   constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
-  constexpr int E = 2 * N, IT = (E + NT - 1) / NT;
+  constexpr int E = 2 * P * N, IT = (E + NT - 1) / NT;
   A* rbuf = reinterpret_cast<A*>(buf);
   const int tid = threadIdx.x;
   StageTw<A, N, P, NT, MAXR> stw;
@@ -338,7 +348,7 @@ __device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf,
     if (E % NT == 0 || e < E) {
       const int l = e / N, m = e - l * N;
       const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-      rbuf[2 * lp<SH>(pos) + l] = x[it];
+      rbuf[2 * ((l >> 1) * SEQ + lp<SH>(pos)) + (l & 1)] = x[it];
     }
   }
   __syncthreads();
@@ -346,24 +356,27 @@ __device__ __forceinline__ void dz_line_pair(const double* x, cplx<double>* buf,
   z_axis_op_phase<kZGrad, N, P, NT, SEQ, SH>(buf, mk, ms0, qs);
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
-  constexpr int ITW = (HN + NT - 1) / NT;
+  constexpr int W = P * HN, ITW = (W + NT - 1) / NT;
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
-    const int n = tid + it * NT;
-    if (HN % NT == 0 || n < HN) {
-      const cplx<A> ua = buf[lp<SH>(n)];
-      const cplx<A> ub = buf[lp<SH>(N - 1 - n)];
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int cc = w / HN, n = w - cc * HN;
+      const cplx<A> ua = buf[cc * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[cc * SEQ + lp<SH>(N - 1 - n)];
       // DST-III: y_{2n} = u_n, y_{2n+1} = -u_{N-1-n}
-      if (0 < nvalid) { out[2 * n] = (T)ua.x; out[2 * n + 1] = (T)(-ub.x); }
-      if (1 < nvalid) { out[N + 2 * n] = (T)ua.y; out[N + 2 * n + 1] = (T)(-ub.y); }
+      const int l0 = 2 * cc;
+      if (l0 < nvalid) { out[l0 * N + 2 * n] = (T)ua.x; out[l0 * N + 2 * n + 1] = (T)(-ub.x); }
+      if (l0 + 1 < nvalid) { out[(l0 + 1) * N + 2 * n] = (T)ua.y; out[(l0 + 1) * N + 2 * n + 1] = (T)(-ub.y); }
     }
   }
 }
 
-// One workgroup: one z line pair (P = 1) of 128 threads.  Load phase: every
-// point evaluates a15 and a16 in FP64 registers as k_density_nl does (same
-// expressions, same order), stores rho_new and the source f_rho, and puts Nl
-// at its Makhoul position of the pair's complex sequence in LDS; then the
+// One workgroup: P = density_pairs<N>() z line pairs of 128 threads.  Load
+// phase: every point evaluates a15 and a16 in FP64 registers as k_density_nl
+// does (same expressions, same order), stores rho_new and the source f_rho,
+// and puts Nl at its Makhoul position of its pair's complex sequence in
+// LDS; then the
 // FFT and Makhoul's post-processing store Nl's z pass (DCT-II).  DZF: the
 // z term of div f first, from f_z in the same LDS sequence (the divergence
 // one-axis operator: DST-II, y[n] = Vq[n] Y[n-1]/(2N), DCT-III); each point
@@ -372,9 +385,9 @@ template <typename T, int N, int NT, bool DZF>
 __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T> io,
                                                        const cplx<double>* __restrict__ tw,
                                                        const cplx<double>* __restrict__ mk) {
-  constexpr int P = 1, TL = 2, HN = N / 2;
+  constexpr int P = density_pairs<N>(), TL = 2 * P, HN = N / 2;
   constexpr int MAXR = pick_maxr(N, P, NT);
-  constexpr int SH = pad_shift(N, MAXR);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using A = double;
   cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
@@ -400,13 +413,13 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
         A x = l < nvalid ? (A)io.fz[base + e] : A(0);
         if (m & 1) x = -x;
         const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-        rbuf[2 * lp<SH>(pos) + (l & 1)] = x;
+        rbuf[2 * ((l >> 1) * SEQ + lp<SH>(pos)) + (l & 1)] = x;
       }
     }
     __syncthreads();
     fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
     const MkSeq<A, N, P, NT> ms0(mk, 2, tid);
-    z_axis_op_phase<kZDiv, N, P, NT, seq_len(N, MAXR), SH>(buf, mk, ms0, io.qs);
+    z_axis_op_phase<kZDiv, N, P, NT, SEQ, SH>(buf, mk, ms0, io.qs);
     __syncthreads();
     // DCT-III output at position m sits in the slot of Makhoul position m
     fft_stage<A, N, P, NT, MAXR, +1, 0>(buf, tw, stw);
@@ -430,33 +443,36 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
         rnew[it] = r1;
         const A am = io.amp[idx];
         const int pz = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-        const A df = DZF ? (A)io.divf[idx] + rbuf[2 * lp<SH>(pz) + (l & 1)] : (A)io.divf[idx];
+        const A df = DZF ? (A)io.divf[idx] + rbuf[2 * ((l >> 1) * SEQ + lp<SH>(pz)) + (l & 1)] : (A)io.divf[idx];
         v = -(A)io.mu[idx] + ((mc.kp * mc.zeta) / (am + 0.001)) * (r1 - mc.rho0 - mc.kp * am) * (A)io.psi[idx] -
             df - ux * (A)io.gp[0][idx] - uy * (A)io.gp[1][idx] - uz * (A)io.gp[2][idx];
       }
       const int pos = (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1);
-      rbuf[2 * lp<SH>(pos) + (l & 1)] = v;
+      rbuf[2 * ((l >> 1) * SEQ + lp<SH>(pos)) + (l & 1)] = v;
     }
   }
   __syncthreads();
   fft_stage<A, N, P, NT, MAXR, -1, 0>(buf, tw, stw);
 
-  constexpr int W = HN;
+  constexpr int W = P * HN;
   constexpr int ITW = (W + NT - 1) / NT;
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
-    const int k = tid + it * NT;
-    if (W % NT == 0 || k < W) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int cc = w / HN, k = w - cc * HN;
       const int kb = (k == 0) ? HN : N - k;
       A y0a, y1a, y0b, y1b;
-      makhoul_post<A, N>(mk, k, kb, mk[k], buf[lp<SH>(k)], buf[lp<SH>(kb)], y0a, y1a, y0b, y1b);
-      if (0 < nvalid) {
-        T* q = io.nlz + base;
+      makhoul_post<A, N>(mk, k, kb, mk[k], buf[cc * SEQ + lp<SH>(k)], buf[cc * SEQ + lp<SH>(kb)], y0a, y1a, y0b,
+                         y1b);
+      const int l0 = 2 * cc;
+      if (l0 < nvalid) {
+        T* q = io.nlz + base + (long long)l0 * N;
         q[k] = (T)y0a;
         q[kb] = (T)y0b;
       }
-      if (1 < nvalid) {
-        T* q = io.nlz + base + N;
+      if (l0 + 1 < nvalid) {
+        T* q = io.nlz + base + (long long)(l0 + 1) * N;
         q[k] = (T)y1a;
         q[kb] = (T)y1b;
       }
@@ -465,7 +481,7 @@ __global__ __launch_bounds__(NT) void density_z_kernel(StepConsts c, DensityIO<T
   // d_z rho_new: the next step's z component of grad rho
   if (io.rdz_next) {
     __syncthreads();
-    dz_line_pair<T, N, NT>(rnew, buf, io.rdz_next + base, nvalid, tw, mk, io.qs);
+    dz_lines<T, N, P, NT>(rnew, buf, io.rdz_next + base, nvalid, tw, mk, io.qs);
   }
 }
 
@@ -476,7 +492,7 @@ __global__ __launch_bounds__(NT) void rho_dz_kernel(StepConsts c, const double* 
                                                     double qs, const cplx<double>* __restrict__ tw,
                                                     const cplx<double>* __restrict__ mk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int TL = 2, E = TL * N, IT = (E + NT - 1) / NT;
+  constexpr int P = density_pairs<N>(), TL = 2 * P, E = TL * N, IT = (E + NT - 1) / NT;
   const long long nl = (long long)c.nx * c.ny;
   const long long first = (long long)blockIdx.x * TL;
   const int nvalid = (int)((nl - first) < TL ? (nl - first) : TL);
@@ -488,7 +504,7 @@ __global__ __launch_bounds__(NT) void rho_dz_kernel(StepConsts c, const double* 
     x[it] = 0.0;
     if ((E % NT == 0 || e < E) && e / N < nvalid) x[it] = rho[base + e];
   }
-  dz_line_pair<T, N, NT>(x, reinterpret_cast<cplx<double>*>(smem_raw), rdz + base, nvalid, tw, mk, qs);
+  dz_lines<T, N, P, NT>(x, reinterpret_cast<cplx<double>*>(smem_raw), rdz + base, nvalid, tw, mk, qs);
 }
 
 constexpr int kDensityNT = 128;
@@ -496,8 +512,9 @@ constexpr int kDensityNT = 128;
 template <typename T, int N>
 hipError_t launch_rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, const void* tw, const void* mk,
                          hipStream_t s) {
-  const long long blocks = ((long long)c.nx * c.ny + 1) / 2;
-  const size_t lds = (size_t)seq_len(N, pick_maxr(N, 1, kDensityNT)) * sizeof(cplx<double>);
+  constexpr int P = density_pairs<N>();
+  const long long blocks = ((long long)c.nx * c.ny + 2 * P - 1) / (2 * P);
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, kDensityNT)) * sizeof(cplx<double>);
   hipLaunchKernelGGL((rho_dz_kernel<T, N, kDensityNT>), dim3((unsigned)blocks, (unsigned)c.nmem),
                      dim3(kDensityNT), lds, s, c, rho, rdz, qs, static_cast<const cplx<double>*>(tw),
                      static_cast<const cplx<double>*>(mk));
@@ -507,8 +524,9 @@ hipError_t launch_rho_dz(const StepConsts& c, const double* rho, T* rdz, double 
 template <typename T, int N>
 hipError_t launch_density(const StepConsts& c, const DensityIO<T>& io, const void* tw, const void* mk,
                           hipStream_t s) {
-  const long long blocks = ((long long)c.nx * c.ny + 1) / 2;
-  const size_t lds = (size_t)seq_len(N, pick_maxr(N, 1, kDensityNT)) * sizeof(cplx<double>);
+  constexpr int P = density_pairs<N>();
+  const long long blocks = ((long long)c.nx * c.ny + 2 * P - 1) / (2 * P);
+  const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, kDensityNT)) * sizeof(cplx<double>);
   if (io.fz)
     hipLaunchKernelGGL((density_z_kernel<T, N, kDensityNT, true>), dim3((unsigned)blocks, (unsigned)c.nmem),
                        dim3(kDensityNT), lds, s, c, io, static_cast<const cplx<double>*>(tw),
