@@ -1178,8 +1178,9 @@ hipError_t launch_fft_pn(const PassIO& io, void* tw, void* mk, int kind, int axi
 // wavefront per line pair along z (half the LDS of two pairs, so more
 // wavefronts per CU hide the loads), i.e. 256 elements per wavefront; along y
 // and x 8 pairs x 256 threads, whose rows are 16 columns: 128-byte segments
-// for FP64 storage.  FP32 storage takes 16 pairs x 512 threads from length
-// 64 on (the same elements per thread, 128-byte rows of 32 columns).
+// for FP64 storage.  FP32 storage takes 16 pairs from length 64 on (128-byte
+// rows of 32 columns) with P N / 8 threads, so that radix-8 stages give
+// every thread a butterfly (16 elements per thread, as before).
 template <typename T, int N>
 struct TileChoice {
   static constexpr int cap(int p) { return p < 1 ? 1 : p; }
@@ -1194,7 +1195,7 @@ struct TileChoice {
   static constexpr int PZ = cap(budget(256));
   static constexpr int PS = kWide ? cap(budget(4096) < 16 ? budget(4096) : 16) : cap(budget(2048) < 8 ? budget(2048) : 8);
   static constexpr int NTZ = 64;
-  static constexpr int NTS = kWide ? 512 : 256;
+  static constexpr int NTS = kWide ? (PS * N / 8 < 512 ? PS * N / 8 : 512) : 256;
 };
 
 // A plain y or x pass on the 256-point kernels of r2r_yx256.h when they
