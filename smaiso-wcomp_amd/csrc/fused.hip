@@ -12,9 +12,19 @@ namespace {
 
 constexpr int kForceOut = 5;  // fgrad, erho, S_x, S_y, S_z
 
+// Line-pair multiple of the z tiles for short lines: the largest power of two
+// up to 256/N (1 from length 192 on), so that the threads of a tile keep
+// several points each.
+template <int N>
+constexpr int short_line_scale() {
+  int p = 256 / N, q = 1;
+  while (q * 2 <= p) q *= 2;
+  return q;
+}
+
 template <typename T, int N>
 struct ForcesTile {
-  static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
+  static constexpr int P = N <= 256 ? 2 * short_line_scale<N>() : 1;  // line pairs per tile
   // 5 waves: the radix-8 stages of the 5P sequences (N <= 256: 10 x N/8
   // butterflies) give every thread one butterfly at N = 256 and 512
   static constexpr int NT = 320;
@@ -312,14 +322,12 @@ __device__ __forceinline__ void z_axis_op_phase(cplx<A>* buf, const cplx<A>* __r
   }
 }
 
-// Line pairs per workgroup of the density pass and of rho_dz: about 256
-// points per pair-set, a power of two (one pair from length 192 on), so that
-// short lines still give each of the 128 threads several points.
+// Line pairs per workgroup of the density pass and of rho_dz (one from
+// length 192 on), so that short lines still give each of the 128 threads
+// several points.
 template <int N>
 constexpr int density_pairs() {
-  int p = 256 / N, q = 1;
-  while (q * 2 <= p) q *= 2;
-  return q;
+  return short_line_scale<N>();
 }
 
 // d_z of P real z line pairs by the gradient one-axis operator (DCT-II along
@@ -570,7 +578,7 @@ constexpr int kVelOut = 4;  // v_x, v_y, v_z, div v
 
 template <int N, typename AT = double>
 struct VelocityTile {
-  static constexpr int P = N <= 256 ? 2 : 1;  // line pairs per tile
+  static constexpr int P = N <= 256 ? 2 * short_line_scale<N>() : 1;  // line pairs per tile
   static constexpr int NT = 256;
   static constexpr size_t lds = (size_t)kVelOut * P * seq_len(N, pick_maxr(N, kVelOut * P, NT)) * sizeof(cplx<AT>);
   static constexpr bool fits = lds <= 65536;
