@@ -101,6 +101,11 @@ struct smaiso_handle {
   // laprl holds the z lines of lap(rho L psi) before its last inverse z pass
   // (the forces pass runs that pass in its tile); the read-out completes it
   bool laprl_zlines = false;
+  // gpx_next holds d_x psi of the current psi, formed by the x pass of psi's
+  // transform at the end of the last step (or after a state was set, loaded
+  // or restored); the next step takes it as grad_x psi
+  void* gpx_next = nullptr;
+  bool gpx_valid = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -217,6 +222,9 @@ struct smaiso_handle {
   // the forces pass runs the last inverse z pass of lap(rho L psi) in steady
   // steps (except the adjoint's recomputed ones)
   bool laprl_in_forces() const { return forces_zpass_laprl(mixed ? F32 : F64, g.nz); }
+  // psi's transform ends with the x pass that also forms d_x psi (length 256
+  // along x; not when grad psi computes in FP32)
+  bool gpx_in_psi_pass() const { return yx256_invgrad_supported(g) && !a32(SMAISO_STAGE_GRAD_PSI); }
 
   // --- timing helpers ----------------------------------------------------
   hipEvent_t take_event() {
@@ -336,6 +344,73 @@ struct smaiso_handle {
     if ((e = run_pass(F64, kinds[1], 1, io, g)) != hipSuccess) return e;
     io.out = out;
     return run_pass_cast(F64, F32, kinds[0], 0, io);
+  }
+  // The x pass of psi's inverse transform from its FP64 x-spectral lines
+  // `wide` (members of gg from member m0), storing psi and d_x psi (into
+  // gpx_next) in the storage type.
+  hipError_t psi_dx_pass(const double* wide, const Geom& gg, long long m0) {
+    const void *tw = nullptr, *mk = nullptr;
+    hipError_t e = r2r.table_ptrs(gg.nx, &tw, &mk);
+    if (e != hipSuccess) return e;
+    const long long off = m0 * g.n3();
+    char* const psi_o = static_cast<char*>(psi) + off * tsz;
+    char* const gpx_o = static_cast<char*>(gpx_next) + off * tsz;
+    Scope sc(this, SMAISO_KC_R2R_AXIS, (double)gg.n3() * gg.nb * (8 + 2 * tsz));
+    return launch_yx256_invgrad(mixed ? F32 : F64, wide, psi_o, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
+  }
+  // psi = ICT(psiq_new) with the update computed on load by the z pass (f:
+  // OP_LD_PSI), whose output and the y pass's stay FP64 (in psi for FP64
+  // storage, out64 for FP32 storage); the y pass on its default tile; the x
+  // pass also forms d_x psi for the next step.  psi_dx_refresh repeats the z
+  // and y passes from a stored psiq with the same tiles, so it reproduces
+  // d_x psi bit for bit.
+  hipError_t psi_transform_with_dx(const Fuse& f) {
+    PassIO io;
+    io.f = f;
+    double* const wide = mixed ? out64 : static_cast<double*>(psi);
+    io.out = wide;
+    hipError_t e = mixed ? run_pass_cast(F32, F64, REDFT01, 2, io) : run_pass(F64, REDFT01, 2, io, g);
+    if (e != hipSuccess) return e;
+    io.in = wide;
+    io.f.op = OP_PLAIN;
+    io.f.untuned = 1;
+    if ((e = run_pass(F64, REDFT01, 1, io, g)) != hipSuccess) return e;
+    return psi_dx_pass(wide, g, 0);
+  }
+  // d_x psi of psi = ICT(psiq) (member m, or all) into gpx_next after a state
+  // was set, loaded or restored: z and y passes of psiq as the step runs them
+  // (the z pass's plan is that of the update-on-load pass), then the fused x
+  // pass, whose psi output (equal to the stored psi where it came from a
+  // step) goes to scratch.
+  hipError_t psi_dx_refresh(int m) {
+    gpx_valid = gpx_in_psi_pass();
+    if (!gpx_valid) return hipSuccess;
+    const Geom G = m < 0 ? g : g_one();
+    const long long m0 = m < 0 ? 0 : m, off = m0 * g.n3();
+    double* const wide = (mixed ? out64 : static_cast<double*>(tmp)) + off;
+    PassIO io;
+    io.in = psiq + off;
+    io.out = wide;
+    io.f.untuned = 1;
+    hipError_t e = run_pass(F64, REDFT01, 2, io, G);
+    if (e != hipSuccess) return e;
+    io.in = wide;
+    if ((e = run_pass(F64, REDFT01, 1, io, G)) != hipSuccess) return e;
+    return psi_dx_scratch_pass(wide, G, m0);
+  }
+  // as psi_dx_pass with psi stored back over `wide` (FP64) rather than into
+  // the field; d_x psi into gpx_next
+  hipError_t psi_dx_scratch_pass(double* wide, const Geom& gg, long long m0) {
+    const void *tw = nullptr, *mk = nullptr;
+    hipError_t e = r2r.table_ptrs(gg.nx, &tw, &mk);
+    if (e != hipSuccess) return e;
+    char* const gpx_o = static_cast<char*>(gpx_next) + m0 * g.n3() * tsz;
+    if (mixed) {
+      // FP32 outputs: psi into the FP32 scratch field, d_x psi into gpx_next
+      char* const tmp_o = static_cast<char*>(tmp) + m0 * g.n3() * tsz;
+      return launch_yx256_invgrad(F32, wide, tmp_o, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
+    }
+    return launch_yx256_invgrad(F64, wide, wide, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
   }
   // the y and x passes of a transform whose z pass has run, in place
   hipError_t xw_yx(const int kinds[3], void* buf, int arith32 = 0) {
@@ -534,7 +609,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->diag_out = static_cast<double*>(tmpp);
   std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
-                               &h->divf, &h->nl, &h->tmp, &h->rdz_next};
+                               &h->divf, &h->nl, &h->tmp, &h->rdz_next, &h->gpx_next};
   for (int a = 0; a < 3; ++a) {
     tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vtq[a]); tbufs.push_back(&h->virrq[a]);
@@ -602,7 +677,10 @@ int run_step(smaiso_handle* h) {
   // same points and the z factor of a7's separable Gaussian filter (d_z psi
   // in FP64; FP32 for grad psi takes the separate passes)
   const bool afused = amp_zpass_supported(h->g.nz) && !h->a32(SMAISO_STAGE_GRAD_PSI);
-  for (int a = 0; a < (afused ? 2 : 3); ++a) {
+  // along x it is there already where psi's transform left it
+  const bool gx = h->gpx_valid && !h->a32(SMAISO_STAGE_GRAD_PSI);
+  if (gx) std::swap(h->gp[0], h->gpx_next);
+  for (int a = gx ? 1 : 0; a < (afused ? 2 : 3); ++a) {
     Fuse g = fb;
     g.axis = a;
     g.op = OP_AX_GRAD;
@@ -791,8 +869,12 @@ int run_step(smaiso_handle* h) {
     // FP32 storage: psi's transform keeps FP64 intermediates.  Their FP32
     // rounding, fed back through the nonlinear terms every step, was the
     // largest FP32 storage error on near-stationary states (about 20x the
-    // rounding of the stored fields themselves on a flat interface)
-    if (h->mixed) HIPTRY(h, h->xwf_wide(kICT, g, h->psi));
+    // rounding of the stored fields themselves on a flat interface).
+    // Where it can, the x pass also forms d_x psi for the next step (the z
+    // and y passes on their default tiles, which psi_dx_refresh repeats)
+    h->gpx_valid = h->gpx_in_psi_pass();
+    if (h->gpx_valid) HIPTRY(h, h->psi_transform_with_dx(g));
+    else if (h->mixed) HIPTRY(h, h->xwf_wide(kICT, g, h->psi));
     else HIPTRY(h, h->xwf(kICT, g, h->psi));
   }
   std::swap(h->psiq, h->psiq_old);
@@ -834,6 +916,7 @@ int do_set_state(smaiso_handle* h, int m, const double* psi, const double* rho, 
   HIPTRY(h, hipMemcpyAsync(h->psiq_old + off, psiq, db, hipMemcpyDeviceToDevice, s));
   HIPTRY(h, hipMemcpyAsync(h->rho + off, rho, db, kind, s));
   HIPTRY(h, refresh_rdz(h, m));
+  HIPTRY(h, h->psi_dx_refresh(m));
   fill_zero_d(scratch, n3, s);
   HIPTRY(h, hipGetLastError());
   HIPTRY(h, hipStreamSynchronize(s));
@@ -1109,6 +1192,7 @@ int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
           h->nloop += 2;
           h->epoch += 2;
           h->laprl_zlines = h->laprl_in_forces();
+          h->gpx_valid = h->gpx_in_psi_pass();
         }
       }
       HIPTRY(owner, hipGraphLaunch(*gexec, L.master));
@@ -1639,6 +1723,7 @@ hipError_t restore_state(smaiso_handle* h, const StateStore& st, int k, long lon
   if (e == hipSuccess) e = hipMemcpyAsync(h->nlq, st.nlq(k), tb, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(h->psi, st.psi(k), tb, hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = refresh_rdz(h);
+  if (e == hipSuccess) e = h->psi_dx_refresh(-1);
   h->nloop = nloop;
   return e;
 }
@@ -2026,6 +2111,7 @@ int smaiso_load_checkpoint(smaiso_handle* h, const char* path) {
   std::fclose(f);
   if (!ok) return fail(h, SMAISO_E_INVALID, fmt("%s is truncated", path));
   HIPTRY(h, refresh_rdz(h));
+  HIPTRY(h, h->psi_dx_refresh(-1));
   h->nloop = hd.nloop;
   h->member_set[0] = 1;
   h->nlq_valid = true;  // the checkpoint holds the last step's N^l

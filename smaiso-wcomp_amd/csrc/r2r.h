@@ -82,6 +82,9 @@ struct Fuse {
   // so a constant leaves the result unchanged in exact arithmetic while a
   // constant near the field's mean shrinks the rounding of the tile.
   double ld_shift = 0;
+  // plain passes: the default tile even where the autotuner chose another
+  // (passes whose results a later computation must reproduce bit for bit)
+  int untuned = 0;
 };
 
 // Lengths with FP32-arithmetic instances of the transform passes.
@@ -174,5 +177,14 @@ void r2r_table_values(int n, double* tw, double* mk, double* ct);
 // cache, so it allocates nothing.
 hipError_t r2r_plain_pass_with_tables(DType dt, int kind, int axis, const PassIO& io, const Geom& g,
                                       const void* tw, const void* mk, const void* ct, hipStream_t s);
+
+// The x pass of an inverse transform (DCT-III along x) of FP64 x-spectral
+// lines `in` that stores the result to out and its x derivative (the
+// gradient one-axis operator: DST-III of -Vs[n] in[n+1], qs = pi/(dx Nx)) to
+// out2, both in out_dt, FP64 arithmetic; length 256 along x
+// (yx256_invgrad_supported).  Defined in r2r_yx256.hip.
+bool yx256_invgrad_supported(const Geom& g);
+hipError_t launch_yx256_invgrad(DType out_dt, const double* in, void* out, void* out2, double qs, const Geom& g,
+                                const void* tw, const void* mk, hipStream_t s);
 
 }  // namespace smaiso

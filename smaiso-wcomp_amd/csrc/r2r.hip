@@ -173,7 +173,7 @@ hipError_t R2R::pass(DType dt, int kind, int axis, const PassIO& io, const Geom&
   hipError_t e = tables(a32 ? F32 : F64, n, &tb);
   if (e != hipSuccess) return e;
   if (a32) return dispatch<float, float>(n, io, tb->tw, tb->mk, tb->costab, kind, axis, g, s);
-  if (io.f.op == OP_PLAIN && !tuned_.empty()) {
+  if (io.f.op == OP_PLAIN && !io.f.untuned && !tuned_.empty()) {
     const auto it = tuned_.find(tune_key(dt, axis, g));
     if (it != tuned_.end() && it->second > 0) {
       if (dt == F64) return launch_plain_variant<double>(n, it->second, io, tb->tw, tb->mk, kind, axis, g, s);

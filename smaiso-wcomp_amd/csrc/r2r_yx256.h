@@ -447,6 +447,95 @@ __global__ __launch_bounds__(16 * P, (ax_min_waves<T, A>())) void axop_kernel(Ar
   }
 }
 
+// The x pass of psi's inverse transform (DCT-III along x of the x-spectral
+// lines Y) that also forms d_x psi from the same lines: d_x psi is the
+// gradient one-axis operator applied to psi, whose forward DCT-II along x is
+// 2N Y, so it is DST-III(-Vs[n] Y[n+1]) with no other factor.  The
+// workgroup's first 16 P threads form psi and the other 16 P threads d_x psi
+// (whole wavefronts each), with an LDS tile each: the first group loads the
+// tile into its LDS sequences, both take their Makhoul pre-twiddle from them
+// (d_x psi's as in axop_kernel: input position k takes -(N-k) qs Y[N-k],
+// position N-k takes -k qs Y[k], none at k = 0), and the two inverse
+// transforms run side by side, psi stored to out, d_x psi to out2.
+template <typename T, typename TO, int P, typename A = double>
+__global__ __launch_bounds__(32 * P) void invgrad_kernel(Args<T, TO> a, TO* __restrict__ out2, double qs) {
+  using C = cplx<A>;
+  constexpr int N = kN;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* const buf = reinterpret_cast<C*>(smem_raw);
+  const int tid = threadIdx.x;
+  const bool dx = tid >= 16 * P;  // wavefront-uniform
+  const int lt = dx ? tid - 16 * P : tid;
+  const int c = lt % P, j = lt / P;
+  C* const sa = buf + c * kSeq;        // the loaded tile, then psi's exchange
+  C* const sx = dx ? buf + (P + c) * kSeq : sa;  // this group's exchange
+  const long long mo = (long long)blockIdx.y * ((long long)a.nx * a.ny * a.nz);
+  const int stride = a.ny * a.nz;
+  const C* const tw = reinterpret_cast<const C*>(a.tw);
+  const C* const mkt = reinterpret_cast<const C*>(a.mk);
+  const C wf = tw[j];
+  const C wi = C{wf.x, -wf.y};
+  const C mk0 = mkt[j], mks = mkt[16];
+  auto tile = [&](int t) -> long long {
+    const int o = t / a.tpo, kc = t - o * a.tpo;
+    return mo + (long long)o * a.nz + kc * 2 * P;
+  };
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    C wi_t = wi, mk0_t = mk0, mks_t = mks;
+    int j_t = j;
+    asm volatile("" : "+v"(j_t), "+v"(wi_t.x), "+v"(wi_t.y));
+    asm volatile("" : "+v"(mk0_t.x), "+v"(mk0_t.y), "+v"(mks_t.x), "+v"(mks_t.y));
+    C v[16];
+    if (!dx) {
+      const T* p = a.in + tile(t) + 2 * c;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const cplx<T> u = *reinterpret_cast<const cplx<T>*>(p + (j + 16 * q) * stride);
+        sa[col_slot(j, q)] = C{(A)u.x, (A)u.y};
+      }
+    }
+    __syncthreads();
+    // z_k = conj(mk[k]) (R0_k + R1_{N-k} + i (R1_k - R0_{N-k})) of R = Y (psi)
+    // or R = the gradient's inputs (d_x psi)
+    C m = mk0_t;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = j_t + 16 * q;
+      const C rk = sa[col_slot(j_t, q)];
+      const C rn = k == 0 ? C{A(0), A(0)} : sa[partner_slot(j_t, q)];
+      C r0 = rk, r1 = rn;
+      if (dx) {
+        const double dk = (double)k * qs, dn = (double)(N - k) * qs;
+        r0 = k == 0 ? C{A(0), A(0)} : C{(A)(-dn * (double)rn.x), (A)(-dn * (double)rn.y)};
+        r1 = k == 0 ? C{A(0), A(0)} : C{(A)(-dk * (double)rk.x), (A)(-dk * (double)rk.y)};
+      }
+      v[q] = cmul(C{m.x, -m.y}, C{r0.x + r1.y, r0.y - r1.x});
+      m = cmul(m, mks_t);
+    }
+    // first butterfly, exchange, second butterfly; y_{2n} = u_n, y_{2n+1} =
+    // u_{N-1-n} (negated for d_x psi's DST-III)
+    dft<A, 16, +1>(v);
+    __syncthreads();  // every pre-twiddle read of the loaded tile is done
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sx[row_slot(j, q)] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = sx[col_slot(j, q)];
+    stage_twiddle(v, wi_t);
+    dft<A, 16, +1>(v);
+    TO* const o = (dx ? out2 : a.out) + tile(t) + 2 * c;
+    const A sg = dx ? A(-1) : A(1);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int n = j_t + 16 * q;
+      const int row = q < 8 ? 2 * n : 2 * N - 1 - 2 * n;
+      const C u = q >= 8 ? C{sg * v[q].x, sg * v[q].y} : v[q];
+      *reinterpret_cast<cplx<TO>*>(o + row * stride) = cplx<TO>{(TO)u.x, (TO)u.y};
+    }
+    __syncthreads();  // the next tile's loads reuse slots other threads read last
+  }
+}
+
 // Sequences per tile: 16 columns of FP64 or 32 of FP32 (128-byte row
 // segments).
 template <typename T>

@@ -100,7 +100,45 @@ hipError_t launch_typed(int kind, int axis, const PassIO& io, const Geom& g, con
   return axis == 1 ? launch_axis<T, TO, 1, A>(kind, io, g, tw, mk, s) : launch_axis<T, TO, 0, A>(kind, io, g, tw, mk, s);
 }
 
+template <typename TO>
+hipError_t launch_invgrad_t(const double* in, TO* out, TO* out2, double qs, const Geom& g, const void* tw,
+                            const void* mk, hipStream_t s) {
+  constexpr int P = yx256::tile_p<double>();
+  yx256::Args<double, TO> a;
+  a.in = in;
+  a.out = out;
+  a.tw = tw;
+  a.mk = mk;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.nz = g.nz;
+  a.kind = REDFT01;
+  a.tpo = g.nz / (2 * P);
+  a.ntiles = g.ny * a.tpo;
+  // two LDS tiles (the loaded lines and psi's exchange, d_x psi's exchange)
+  const size_t lds = (size_t)2 * P * yx256::kSeq * sizeof(cplx<double>);
+  const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+  const int grid = std::min(a.ntiles, 256 * per_cu * 2);
+  hipLaunchKernelGGL((yx256::invgrad_kernel<double, TO, P, double>), dim3((unsigned)grid, (unsigned)g.nb),
+                     dim3(32 * P), lds, s, a, out2, qs);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+bool yx256_invgrad_supported(const Geom& g) {
+  constexpr int P = yx256::tile_p<double>();
+  return g.nx == yx256::kN && g.nz % (2 * P) == 0 && (g.n3() % 2) == 0 &&
+         (long long)(yx256::kN - 1) * g.ny * g.nz + g.nz <= 2147483647LL;
+}
+
+hipError_t launch_yx256_invgrad(DType out_dt, const double* in, void* out, void* out2, double qs, const Geom& g,
+                                const void* tw, const void* mk, hipStream_t s) {
+  if (!yx256_invgrad_supported(g) || qs == 0.0) return hipErrorNotSupported;
+  if (out_dt == F64)
+    return launch_invgrad_t<double>(in, static_cast<double*>(out), static_cast<double*>(out2), qs, g, tw, mk, s);
+  return launch_invgrad_t<float>(in, static_cast<float*>(out), static_cast<float*>(out2), qs, g, tw, mk, s);
+}
 
 hipError_t launch_yx256(DType in_dt, DType out_dt, int kind, int axis, const PassIO& io, const Geom& g,
                         const void* tw, const void* mk, hipStream_t s, bool arith32) {
