@@ -101,19 +101,19 @@ __device__ __forceinline__ void stage_twiddle(cplx<A>* v, cplx<A> w1) {
 // CU, two waves per SIMD; an FP64 tile of 16 sequences over FP32 storage (70
 // KiB, 256 threads) two workgroups, two waves per SIMD.  FP64 arithmetic: the
 // forward pass fits two waves with the prefetch (FP32 storage: y pass 44 ->
-// 36 us against one wave); the inverse pass (one more LDS exchange) keeps
-// the full register file and the prefetch for FP64 storage (50 us against 53
-// at two waves without prefetch) and drops the prefetch for FP32 storage
-// (two waves: 43 -> 37-42 us; with the prefetch at two waves it spilled).
-// FP32 arithmetic (16 sequences, 256 threads) measured fastest with the full
+// 36 us against one wave); the inverse pass (one more LDS exchange) runs at
+// two waves without the prefetch (FP64 storage: 244 VGPRs, same-box FP64 step
+// 279.2 -> 280.5 steps/s against one wave with the prefetch, whose LDS
+// addresses once took a register each; FP32 storage: 43 -> 37-42 us).  FP32
+// arithmetic (16 sequences, 256 threads) measured fastest with the full
 // register file and the tables held across tiles.
 template <typename T, int MODE, typename A>
 constexpr int min_waves() {
-  return sizeof(A) == 8 && !(MODE == kInv && sizeof(T) == 8) ? 2 : 1;
+  return sizeof(A) == 8 ? 2 : 1;
 }
 template <typename T, int MODE, typename A>
 constexpr bool pass_prefetch() {
-  return !(MODE == kInv && sizeof(A) == 8 && sizeof(T) == 4);
+  return !(MODE == kInv && sizeof(A) == 8);
 }
 
 // AXIS: 1 = y pass, 0 = x pass.  MODE: kFwd or kInv.  P sequences (2P
@@ -166,8 +166,8 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
   };
 
   // the next tile is held in the storage type until it is transformed; the
-  // FP64-arithmetic inverse pass over FP32 storage loads each tile at its
-  // start instead (two waves per SIMD without spilling the prefetch)
+  // FP64-arithmetic inverse pass loads each tile at its start instead (two
+  // waves per SIMD without spilling the prefetch)
   constexpr bool kPrefetch = pass_prefetch<T, MODE, A>();
   C v[16];
   cplx<T> nxt[16];
