@@ -106,6 +106,11 @@ struct smaiso_handle {
   // or restored); the next step takes it as grad_x psi
   void* gpx_next = nullptr;
   bool gpx_valid = false;
+  // lpsi_next holds the z lines of L psi of the current psi, formed by the z
+  // pass of psi's transform at the end of the last step (or after a state
+  // was set, loaded or restored); the next step runs their y and x passes
+  void* lpsi_next = nullptr;
+  bool lz_valid = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -225,6 +230,11 @@ struct smaiso_handle {
   // psi's transform ends with the x pass that also forms d_x psi (length 256
   // along x; not when grad psi computes in FP32)
   bool gpx_in_psi_pass() const { return yx256_invgrad_supported(g) && !a32(SMAISO_STAGE_GRAD_PSI); }
+  // ... and its z pass also forms the z lines of the next step's L psi (FP64
+  // arithmetic, so not when L psi computes in FP32)
+  bool lz_in_psi_pass() const {
+    return gpx_in_psi_pass() && !a32(SMAISO_STAGE_LPSI) && psi_sh_zpass_supported(g.nz);
+  }
 
   // --- timing helpers ----------------------------------------------------
   hipEvent_t take_event() {
@@ -358,18 +368,37 @@ struct smaiso_handle {
     Scope sc(this, SMAISO_KC_R2R_AXIS, (double)gg.n3() * gg.nb * (8 + 2 * tsz));
     return launch_yx256_invgrad(mixed ? F32 : F64, wide, psi_o, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
   }
+  // The z pass of psi = ICT(psiq) that also stores the z lines of L psi
+  // (into zl): with the update computed on load (f, as OP_LD_PSI), or from
+  // psiq as it is (pq, update = false); psi's z lines (FP64) into zpsi.
+  hipError_t psi_lz_pass(const Fuse& f, bool update, const double* pq, double* zpsi, void* zl,
+                         const StepConsts& cc) {
+    const double n = (double)cc.nx * cc.ny * cc.nz * cc.nmem;
+    Scope sc(this, SMAISO_KC_R2R_Z, n * ((update ? 16 + 2 * (double)tsz + 8 : 8) + 8 + (double)tsz));
+    if (mixed) {
+      PsiShIO<float> io;
+      io.f = f; io.psiq = pq; io.zpsi = zpsi; io.zlpsi = static_cast<float*>(zl);
+      return psi_sh_zpass<float>(cc, io, update, r2r, stream);
+    }
+    PsiShIO<double> io;
+    io.f = f; io.psiq = pq; io.zpsi = zpsi; io.zlpsi = static_cast<double*>(zl);
+    return psi_sh_zpass<double>(cc, io, update, r2r, stream);
+  }
   // psi = ICT(psiq_new) with the update computed on load by the z pass (f:
   // OP_LD_PSI), whose output and the y pass's stay FP64 (in psi for FP64
   // storage, out64 for FP32 storage); the y pass on its default tile; the x
-  // pass also forms d_x psi for the next step.  psi_dx_refresh repeats the z
-  // and y passes from a stored psiq with the same tiles, so it reproduces
-  // d_x psi bit for bit.
+  // pass also forms d_x psi for the next step.  With lz_valid the z pass is
+  // psi_lz_pass, which also leaves the next step's L psi z lines in
+  // lpsi_next.  psi_dx_refresh repeats the z and y passes from a stored psiq
+  // with the same kernels and tiles, so it reproduces both bit for bit.
   hipError_t psi_transform_with_dx(const Fuse& f) {
     PassIO io;
     io.f = f;
     double* const wide = mixed ? out64 : static_cast<double*>(psi);
     io.out = wide;
-    hipError_t e = mixed ? run_pass_cast(F32, F64, REDFT01, 2, io) : run_pass(F64, REDFT01, 2, io, g);
+    hipError_t e = lz_valid ? psi_lz_pass(f, true, nullptr, wide, lpsi_next, consts())
+                   : mixed  ? run_pass_cast(F32, F64, REDFT01, 2, io)
+                            : run_pass(F64, REDFT01, 2, io, g);
     if (e != hipSuccess) return e;
     io.in = wide;
     io.f.op = OP_PLAIN;
@@ -377,13 +406,15 @@ struct smaiso_handle {
     if ((e = run_pass(F64, REDFT01, 1, io, g)) != hipSuccess) return e;
     return psi_dx_pass(wide, g, 0);
   }
-  // d_x psi of psi = ICT(psiq) (member m, or all) into gpx_next after a state
-  // was set, loaded or restored: z and y passes of psiq as the step runs them
-  // (the z pass's plan is that of the update-on-load pass), then the fused x
-  // pass, whose psi output (equal to the stored psi where it came from a
+  // d_x psi of psi = ICT(psiq) (member m, or all) into gpx_next, and L psi's
+  // z lines into lpsi_next, after a state was set, loaded or restored: z and
+  // y passes of psiq as the step runs them (the z pass's plan is that of the
+  // update-on-load pass, or psi_lz_pass without the update), then the fused
+  // x pass, whose psi output (equal to the stored psi where it came from a
   // step) goes to scratch.
   hipError_t psi_dx_refresh(int m) {
     gpx_valid = gpx_in_psi_pass();
+    lz_valid = lz_in_psi_pass();
     if (!gpx_valid) return hipSuccess;
     const Geom G = m < 0 ? g : g_one();
     const long long m0 = m < 0 ? 0 : m, off = m0 * g.n3();
@@ -392,7 +423,10 @@ struct smaiso_handle {
     io.in = psiq + off;
     io.out = wide;
     io.f.untuned = 1;
-    hipError_t e = run_pass(F64, REDFT01, 2, io, G);
+    hipError_t e = lz_valid ? psi_lz_pass(fuse_base(), false, psiq + off, wide,
+                                          static_cast<char*>(lpsi_next) + off * tsz,
+                                          m < 0 ? consts() : consts_one(m))
+                            : run_pass(F64, REDFT01, 2, io, G);
     if (e != hipSuccess) return e;
     io.in = wide;
     if ((e = run_pass(F64, REDFT01, 1, io, G)) != hipSuccess) return e;
@@ -609,7 +643,8 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
   h->diag_out = static_cast<double*>(tmpp);
   std::vector<void**> tbufs = {&h->psi, &h->psi_prev, &h->nlq, &h->nlq_old, &h->amp,
                                &h->lpsi, &h->laprl, &h->mu, &h->fgrad, &h->erho, &h->divv,
-                               &h->divf, &h->nl, &h->tmp, &h->rdz_next, &h->gpx_next};
+                               &h->divf, &h->nl, &h->tmp, &h->rdz_next, &h->gpx_next,
+                               &h->lpsi_next};
   for (int a = 0; a < 3; ++a) {
     tbufs.push_back(&h->gp[a]); tbufs.push_back(&h->rd[a]); tbufs.push_back(&h->S[a]);
     tbufs.push_back(&h->f[a]); tbufs.push_back(&h->vtq[a]); tbufs.push_back(&h->virrq[a]);
@@ -622,7 +657,7 @@ int create_group(const smaiso_params* ps, int n, smaiso_handle** out) {
     h->out64 = static_cast<double*>(tmpp);
   }
   for (int a = 0; a < 3; ++a) {
-    if ((e = h->r2r.prepare(nn[a])) != hipSuccess) return hipfail(e, "r2r tables");
+    if ((e = h->r2r.prepare(nn[a], h->mixed)) != hipSuccess) return hipfail(e, "r2r tables");
   }
   *out = h.release();
   return SMAISO_OK;
@@ -717,8 +752,11 @@ int run_step(smaiso_handle* h) {
       HIPTRY(h, h->axpass(dt, a, h->amp, h->amp, g));
     }
   }
-  // a8 (S:1293-1308)
-  {
+  // a8 (S:1293-1308); its z pass ran with psi's transform where it could
+  if (h->lz_valid && !h->a32(SMAISO_STAGE_LPSI)) {
+    std::swap(h->lpsi, h->lpsi_next);
+    HIPTRY(h, h->xw_yx(kICT, h->lpsi));
+  } else {
     Fuse g = fb;
     g.op = OP_LD_SH; g.src[0] = h->psiq;
     g.arith32 = h->a32(SMAISO_STAGE_LPSI);
@@ -871,8 +909,10 @@ int run_step(smaiso_handle* h) {
     // largest FP32 storage error on near-stationary states (about 20x the
     // rounding of the stored fields themselves on a flat interface).
     // Where it can, the x pass also forms d_x psi for the next step (the z
-    // and y passes on their default tiles, which psi_dx_refresh repeats)
+    // and y passes on their default tiles, which psi_dx_refresh repeats) and
+    // the z pass the z lines of the next step's L psi
     h->gpx_valid = h->gpx_in_psi_pass();
+    h->lz_valid = h->lz_in_psi_pass();
     if (h->gpx_valid) HIPTRY(h, h->psi_transform_with_dx(g));
     else if (h->mixed) HIPTRY(h, h->xwf_wide(kICT, g, h->psi));
     else HIPTRY(h, h->xwf(kICT, g, h->psi));
@@ -1193,6 +1233,7 @@ int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
           h->epoch += 2;
           h->laprl_zlines = h->laprl_in_forces();
           h->gpx_valid = h->gpx_in_psi_pass();
+          h->lz_valid = h->lz_in_psi_pass();
         }
       }
       HIPTRY(owner, hipGraphLaunch(*gexec, L.master));

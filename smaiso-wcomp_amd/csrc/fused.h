@@ -109,4 +109,23 @@ template <typename T>
 hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, bool gauss32, hipStream_t s);
 bool amp_zpass_supported(int nz);
 
+// The z pass of psi's inverse transform (a17, S:2177-2198: the update of
+// OP_LD_PSI computed on load, new psiq stored) together with the z pass of
+// the next step's L psi (a8, S:1293-1308: iDCT of alpha (q0^2 - |q|^2) psiq
+// from the same new psiq) in one tile: psi's z lines go to zpsi (FP64),
+// L psi's to zlpsi, whose y and x passes the next step runs.  update =
+// false transforms psiq as it is (the same two outputs, for a state that was
+// set, loaded or restored).  FP64 arithmetic.
+template <typename T>
+struct PsiShIO {
+  Fuse f;                        // OP_LD_PSI's fields, alpha, q02, qs
+  const double* psiq = nullptr;  // update = false: the coefficients
+  double* zpsi = nullptr;
+  T* zlpsi = nullptr;
+};
+
+template <typename T>
+hipError_t psi_sh_zpass(const StepConsts& c, const PsiShIO<T>& io, bool update, R2R& r2r, hipStream_t s);
+bool psi_sh_zpass_supported(int nz);
+
 }  // namespace smaiso

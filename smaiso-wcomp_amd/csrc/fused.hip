@@ -1002,6 +1002,166 @@ hipError_t amp_zpass(const StepConsts& c, const AmpIO<T>& io, R2R& r2r, bool gau
 template hipError_t amp_zpass<double>(const StepConsts&, const AmpIO<double>&, R2R&, bool, hipStream_t);
 template hipError_t amp_zpass<float>(const StepConsts&, const AmpIO<float>&, R2R&, bool, hipStream_t);
 
+namespace {
+
+// One workgroup: P z line pairs of 128 threads, 2P complex sequences in LDS
+// (psi of pair c at sequence c, L psi of pair c at sequence P + c).  Item
+// (c, k), k < N/2, takes modes k and kb = N - k (N/2 for k = 0) of both lines
+// of pair c: UPD forms the new psiq there (the update of OP_LD_PSI) and
+// stores it, else it reads psiq; L psi's mode is alpha (q0^2 - |q|^2) times
+// it (the load of OP_LD_SH).  Makhoul's inverse pre-twiddle puts both into
+// their sequences, one inverse FFT runs over all 2P, and y_2n = u_n,
+// y_2n+1 = u_{N-1-n} are stored: psi's z lines in FP64, L psi's in T.
+template <typename T, int N, int P, int NT, bool UPD>
+__global__ __launch_bounds__(NT) void psi_sh_z_kernel(PsiShIO<T> io, int nlines, int ny,
+                                                     const cplx<double>* __restrict__ tw,
+                                                     const cplx<double>* __restrict__ mk) {
+  using A = double;
+  constexpr int NS = 2 * P, TL = 2 * P, HN = N / 2;
+  constexpr int MAXR = pick_maxr(N, NS, NT);
+  constexpr int SEQ = seq_len(N, MAXR), SH = pad_shift(N, MAXR);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<A>* buf = reinterpret_cast<cplx<A>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const long long first = (long long)blockIdx.x * TL;
+  const int nvalid = (int)((nlines - first) < TL ? (nlines - first) : TL);
+  const long long base = (long long)blockIdx.y * nlines * N + first * N;
+  const Fuse& f = io.f;
+  StageTw<A, N, NS, NT, MAXR> stw;
+  stw.load(tw, tid);
+
+  constexpr int W = P * HN;
+  constexpr int ITW = (W + NT - 1) / NT;
+  cplx<A> pk[ITW], pb[ITW], sk[ITW], sb[ITW];  // psi and L psi at modes k, kb (line 0, line 1)
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    pk[it] = pb[it] = sk[it] = sb[it] = cplx<A>{A(0), A(0)};
+    if (W % NT == 0 || w < W) {
+      const int c = w / HN, k = w - c * HN;
+      const int kb = (k == 0) ? HN : N - k;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int l = 2 * c + q;
+        if (l < nvalid) {
+          int i, j;
+          line_ij(first + l, ny, i, j);
+          const long long o = base + (long long)l * N;
+          A ra, rb;
+          if constexpr (UPD) {
+            ra = psi_update_value<T>(f, o + k, i, j, k);
+            rb = psi_update_value<T>(f, o + kb, i, j, kb);
+            f.psiq_new[o + k] = ra;
+            f.psiq_new[o + kb] = rb;
+          } else {
+            ra = io.psiq[o + k];
+            rb = io.psiq[o + kb];
+          }
+          const A la = f.alpha * (f.q02 - mq2c(f, i, j, k)) * ra;
+          const A lb = f.alpha * (f.q02 - mq2c(f, i, j, kb)) * rb;
+          if (q == 0) {
+            pk[it].x = ra; pb[it].x = rb; sk[it].x = la; sb[it].x = lb;
+          } else {
+            pk[it].y = ra; pb[it].y = rb; sk[it].y = la; sb[it].y = lb;
+          }
+        }
+      }
+    }
+  }
+  MkSeq<A, N, P, NT> ms(mk, 2, tid);
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    const int w = tid + it * NT;
+    if (W % NT == 0 || w < W) {
+      const int c = w / HN, k = w - c * HN;
+      const int kb = (k == 0) ? HN : N - k;
+      const cplx<A> mkk = ms.get(mk, 2, k);
+      cplx<A> zk, zb;
+      makhoul_pre<A, N>(mk, k, mkk, pk[it], pb[it], zk, zb);
+      buf[c * SEQ + lp<SH>(k)] = zk;
+      buf[c * SEQ + lp<SH>(kb)] = zb;
+      makhoul_pre<A, N>(mk, k, mkk, sk[it], sb[it], zk, zb);
+      buf[(P + c) * SEQ + lp<SH>(k)] = zk;
+      buf[(P + c) * SEQ + lp<SH>(kb)] = zb;
+    }
+    ms.advance(2, it);
+  }
+  __syncthreads();
+  fft_stage<A, N, NS, NT, MAXR, +1, 0>(buf, tw, stw);
+
+  constexpr int WO = NS * HN;
+  constexpr int ITO = (WO + NT - 1) / NT;
+#pragma unroll
+  for (int it = 0; it < ITO; ++it) {
+    const int w = tid + it * NT;
+    if (WO % NT == 0 || w < WO) {
+      const int c = w / HN, n = w - c * HN;
+      const cplx<A> ua = buf[c * SEQ + lp<SH>(n)];
+      const cplx<A> ub = buf[c * SEQ + lp<SH>(N - 1 - n)];
+      const bool lz = c >= P;
+      const int pc = lz ? c - P : c;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int l = 2 * pc + q;
+        if (l < nvalid) {
+          const long long o = base + (long long)l * N + 2 * n;
+          const A v0 = q == 0 ? ua.x : ua.y, v1 = q == 0 ? ub.x : ub.y;
+          if (lz) *reinterpret_cast<cplx<T>*>(io.zlpsi + o) = cplx<T>{(T)v0, (T)v1};
+          else *reinterpret_cast<cplx<double>*>(io.zpsi + o) = cplx<double>{v0, v1};
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int N, bool UPD>
+hipError_t launch_psi_sh(const StepConsts& c, const PsiShIO<T>& io, const void* tw, const void* mk,
+                         hipStream_t s) {
+  // the general z tile's line pairs with twice its sequences on 128 threads
+  constexpr int P = TileChoice<double, N>::PZ, NT = 128, TL = 2 * P;
+  const long long nl = (long long)c.nx * c.ny;
+  const long long blocks = (nl + TL - 1) / TL;
+  const size_t lds = (size_t)2 * P * seq_len(N, pick_maxr(N, 2 * P, NT)) * sizeof(cplx<double>);
+  hipLaunchKernelGGL((psi_sh_z_kernel<T, N, P, NT, UPD>), dim3((unsigned)blocks, (unsigned)c.nmem), dim3(NT), lds,
+                     s, io, (int)nl, c.ny, static_cast<const cplx<double>*>(tw),
+                     static_cast<const cplx<double>*>(mk));
+  return hipGetLastError();
+}
+
+template <typename T, bool UPD>
+hipError_t psi_sh_n(const StepConsts& c, const PsiShIO<T>& io, const void* tw, const void* mk, hipStream_t s) {
+  switch (c.nz) {
+    case 16: return launch_psi_sh<T, 16, UPD>(c, io, tw, mk, s);
+    case 32: return launch_psi_sh<T, 32, UPD>(c, io, tw, mk, s);
+    case 48: return launch_psi_sh<T, 48, UPD>(c, io, tw, mk, s);
+    case 64: return launch_psi_sh<T, 64, UPD>(c, io, tw, mk, s);
+    case 96: return launch_psi_sh<T, 96, UPD>(c, io, tw, mk, s);
+    case 128: return launch_psi_sh<T, 128, UPD>(c, io, tw, mk, s);
+    case 192: return launch_psi_sh<T, 192, UPD>(c, io, tw, mk, s);
+    case 256: return launch_psi_sh<T, 256, UPD>(c, io, tw, mk, s);
+    case 384: return launch_psi_sh<T, 384, UPD>(c, io, tw, mk, s);
+    case 512: return launch_psi_sh<T, 512, UPD>(c, io, tw, mk, s);
+    case 1024: return launch_psi_sh<T, 1024, UPD>(c, io, tw, mk, s);
+    default: return hipErrorNotSupported;
+  }
+}
+
+}  // namespace
+
+bool psi_sh_zpass_supported(int nz) { return R2R::fft_supported(nz); }
+
+template <typename T>
+hipError_t psi_sh_zpass(const StepConsts& c, const PsiShIO<T>& io, bool update, R2R& r2r, hipStream_t s) {
+  if (!psi_sh_zpass_supported(c.nz)) return hipErrorNotSupported;
+  const void *tw = nullptr, *mk = nullptr;
+  hipError_t e = r2r.table_ptrs(c.nz, &tw, &mk);
+  if (e != hipSuccess) return e;
+  return update ? psi_sh_n<T, true>(c, io, tw, mk, s) : psi_sh_n<T, false>(c, io, tw, mk, s);
+}
+
+template hipError_t psi_sh_zpass<double>(const StepConsts&, const PsiShIO<double>&, bool, R2R&, hipStream_t);
+template hipError_t psi_sh_zpass<float>(const StepConsts&, const PsiShIO<float>&, bool, R2R&, hipStream_t);
+
 template <typename T>
 hipError_t rho_dz(const StepConsts& c, const double* rho, T* rdz, double qs, R2R& r2r, hipStream_t s) {
   if (!density_zpass_supported(c.nz)) return hipErrorNotSupported;

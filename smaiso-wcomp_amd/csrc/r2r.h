@@ -117,12 +117,14 @@ class R2R {
                        const Geom& g, hipStream_t s);
   // True when length n runs on the LDS FFT path (else the direct O(n^2) path).
   static bool fft_supported(int n);
-  // Build the (FP64) twiddle tables of length n now (they are otherwise
-  // built on first use, with synchronous copies that a stream capture cannot
-  // hold).
-  hipError_t prepare(int n) {
+  // Build the twiddle tables of length n now (they are otherwise built on
+  // first use, with synchronous copies that a stream capture cannot hold):
+  // FP64, and with fp32 also the FP32 ones of the FP32-arithmetic passes.
+  hipError_t prepare(int n, bool fp32 = false) {
     Tables* t = nullptr;
-    return tables(F64, n, &t);
+    hipError_t e = tables(F64, n, &t);
+    if (e == hipSuccess && fp32 && r2r_arith32_supported(n)) e = tables(F32, n, &t);
+    return e;
   }
   // Device twiddle tables of length n (FP64 complex; the passes compute in
   // FP64 for both storage types): tw[m] = exp(-2 pi i m/n), mk[m] =

@@ -82,6 +82,28 @@ def test_checkpoint_continuation_is_bit_identical(tmp_path, precision):
 
 
 
+@pytest.mark.parametrize("precision", [S.FP64, S.FP32_MIXED])
+def test_checkpoint_continuation_with_carried_lines(tmp_path, precision):
+    """With 256 points along x each step leaves d_x psi and the z lines of
+    the next step's L psi for the next step, and a checkpoint load rebuilds
+    them from psiq: continuing from the checkpoint (graph replay on that
+    side) stays bit-identical."""
+    p = _params((256, 16, 48), precision=precision)
+    a = S.Stepper(p)
+    a.set_state(*_ic(p))
+    a.step(5)
+    ck = tmp_path / "state.ck"
+    a.save_checkpoint(ck)
+    b = S.Stepper(p)
+    b.set_graph(True)
+    b.load_checkpoint(ck)
+    a.step(6)
+    b.step(6)
+    for name in ("psi", "rho", "psiq", "psiq_old", "nlq", "lpsi", "gpx", "amp", "mu", "vx"):
+        assert np.array_equal(a.field(name), b.field(name)), name
+    assert a.diagnostics() == b.diagnostics()
+
+
 def test_derived_fields_need_a_step_after_set_or_load(tmp_path):
     """Fields a step produces (amp, L psi, v, psi_old, ...) and the diagnostics
     are refused until a step has run since set_state or a checkpoint load; the

@@ -52,6 +52,16 @@ def test_first_step_every_field_odd_grid():
     compare_all(st, o, TOL64)
 
 
+def test_carried_lines_every_field_nx256():
+    """Three steps on 256 x 16 x 48, where each step's psi transform also
+    leaves d_x psi and the z lines of the next step's L psi (the 256-point x
+    kernel; z length 48 on the FFT path): every field against the oracle."""
+    sp, op = params_pair(nx=256, ny=16, nz=48, dt=0.01, rho0=0.9, rho_m=1.0, divv_switch=0)
+    psi, rho = O.make_interface_ic(op, O.InterfaceIC(undulation=0.5, seed=6))
+    st, o = run_pair(sp, op, psi, rho, 3)
+    compare_all(st, o, TOL64)
+
+
 def test_config1_20_steps_every_field():
     sp, op = config1(64)
     psi, rho = O.make_interface_ic(op, O.InterfaceIC(seed=1))
