@@ -936,9 +936,7 @@ template <typename T, int N, typename AG>
 hipError_t launch_amp(const StepConsts& c, const AmpIO<T>& io, const void* tw, const void* mk, const void* twg,
                       const void* mkg, hipStream_t s) {
   using C = TileChoice<T, N>;
-  // two line pairs per 128-thread workgroup from N = 256 on (same elements
-  // per thread as the default z tile, half the workgroups)
-  constexpr int P = N >= 256 ? 2 : C::PZ, NT = N >= 256 ? 128 : C::NTZ, TL = 2 * P;
+  constexpr int P = C::PZ, NT = C::NTZ, TL = 2 * P;
   const long long nl = (long long)c.nx * c.ny;
   const long long blocks = (nl + TL - 1) / TL;
   const size_t lds = (size_t)P * seq_len(N, pick_maxr(N, P, NT)) * sizeof(cplx<double>);
