@@ -111,6 +111,10 @@ struct smaiso_handle {
   // was set, loaded or restored); the next step runs their y and x passes
   void* lpsi_next = nullptr;
   bool lz_valid = false;
+  // fgrad holds fgrad^ + G erho^ (formed by the x pass of the two forward
+  // transforms) and erho is not transformed along x: the velocity solve
+  // reads fgrad alone
+  bool fg_combined = false;
   bool mixed = false;
   size_t tsz = 8;  // bytes per work element
   uint32_t fp32_stages = 0;  // SMAISO_STAGE_* computed in FP32 (FP32-mixed only)
@@ -232,6 +236,9 @@ struct smaiso_handle {
   bool gpx_in_psi_pass() const { return yx256_invgrad_supported(g) && !a32(SMAISO_STAGE_GRAD_PSI); }
   // ... and its z pass also forms the z lines of the next step's L psi (FP64
   // arithmetic, so not when L psi computes in FP32)
+  // FP64 storage, 256 points along x: steady steps form fgrad^ + G erho^ in
+  // one x pass
+  bool fg_in_x_pass() const { return !mixed && yx256_fgx_supported(g) && velocity_zpass_supported(g.nz); }
   bool lz_in_psi_pass() const {
     return gpx_in_psi_pass() && !a32(SMAISO_STAGE_LPSI) && psi_sh_zpass_supported(g.nz);
   }
@@ -445,6 +452,24 @@ struct smaiso_handle {
       return launch_yx256_invgrad(F32, wide, tmp_o, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
     }
     return launch_yx256_invgrad(F64, wide, wide, gpx_o, fuse_base().qs[0], gg, tw, mk, stream);
+  }
+  // one y pass in place (the work type, FP64 arithmetic)
+  hipError_t y_pass(int kind, void* buf) {
+    PassIO io;
+    io.in = buf;
+    io.out = buf;
+    return run_pass(mixed ? F32 : F64, kind, 1, io, g);
+  }
+  // the x pass of the forward transforms of fgrad and erho (their z and y
+  // passes done) that stores fgrad^ + G erho^ over fgrad (fg_in_x_pass)
+  hipError_t fg_x_pass() {
+    const void *tw = nullptr, *mk = nullptr;
+    hipError_t e = r2r.table_ptrs(g.nx, &tw, &mk);
+    if (e != hipSuccess) return e;
+    const StepConsts c = consts();
+    Scope sc(this, SMAISO_KC_R2R_X, (double)g.n3() * g.nb * 24);
+    return launch_yx256_fgx(static_cast<const double*>(fgrad), static_cast<const double*>(erho),
+                            static_cast<double*>(fgrad), c.qs, g, tw, mk, stream);
   }
   // the y and x passes of a transform whose z pass has run, in place
   hipError_t xw_yx(const int kinds[3], void* buf, int arith32 = 0) {
@@ -767,6 +792,7 @@ int run_step(smaiso_handle* h) {
   // recomputed steps, whose transpose reads the field)
   const bool forces_z = forces_zpass_supported(h->mixed ? F32 : F64, h->g.nz);
   h->laprl_zlines = false;
+  h->fg_combined = false;
   if (h->nloop > h->p.divv_switch || h->p.load_mode == 1) {
     Fuse g = fb;
     g.op = OP_LD_PROD; g.src[0] = h->rho; g.src[1] = h->lpsi;
@@ -798,8 +824,16 @@ int run_step(smaiso_handle* h) {
       HIPTRY(h, forces_zpass<T>(c, io, h->r2r, s));
     }
     const int fa = h->a32(SMAISO_STAGE_FORCES);
-    HIPTRY(h, h->xw_yx(kCT, h->fgrad, fa));
-    HIPTRY(h, h->xw_yx(kCT, h->erho, fa));
+    if (!h->keep_virr && h->fg_in_x_pass()) {
+      // the y passes of both, then one x pass storing fgrad^ + G erho^
+      HIPTRY(h, h->y_pass(kCT[1], h->fgrad));
+      HIPTRY(h, h->y_pass(kCT[1], h->erho));
+      HIPTRY(h, h->fg_x_pass());
+      h->fg_combined = true;
+    } else {
+      HIPTRY(h, h->xw_yx(kCT, h->fgrad, fa));
+      HIPTRY(h, h->xw_yx(kCT, h->erho, fa));
+    }
     for (int a = 0; a < 3; ++a) HIPTRY(h, h->xw_yx(kST[a], S[a], fa));
   } else {
     {
@@ -824,17 +858,18 @@ int run_step(smaiso_handle* h) {
     VelocityIO<T> io;
     for (int a = 0; a < 3; ++a) { io.S[a] = S[a]; io.zout[a] = v[a]; }
     io.fgq = static_cast<const T*>(h->fgrad);
-    io.erq = static_cast<const T*>(h->erho);
+    io.erq = h->fg_combined ? nullptr : static_cast<const T*>(h->erho);
     io.zout[3] = static_cast<T*>(h->divv);
     {
-      smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * 9);
+      smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * (h->fg_combined ? 8 : 9));
       HIPTRY(h, velocity_zpass<T>(c, io, h->r2r, h->a32(SMAISO_STAGE_VELSOLVE) != 0,
                                   h->a32(SMAISO_STAGE_DIVV) && h->a32(SMAISO_STAGE_VELOCITY), s));
     }
     HIPTRY(h, h->xw_yx(kICT, h->divv, h->a32(SMAISO_STAGE_DIVV)));
   } else {
     smaiso_handle::Scope sc(h, SMAISO_KC_SPECTRAL, fb1 * (h->keep_virr ? 12 : 9));
-    K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad), static_cast<T*>(h->erho),
+    K::velocity(c, S[0], S[1], S[2], static_cast<T*>(h->fgrad),
+                h->fg_combined ? nullptr : static_cast<T*>(h->erho),
                 vtq[0], vtq[1], vtq[2], h->keep_virr ? virrq[0] : nullptr, virrq[1], virrq[2],
                 static_cast<T*>(h->divv), h->a32(SMAISO_STAGE_VELSOLVE) != 0, s);
     if (h->keep_virr) h->virr_epoch = h->epoch;
@@ -984,7 +1019,8 @@ hipError_t solve_virr(smaiso_handle* h) {
   T* const* vtq = reinterpret_cast<T* const*>(h->vtq);
   T* const* virrq = reinterpret_cast<T* const*>(h->virrq);
   StepKernels<T>::velocity(h->consts(), S[0], S[1], S[2], static_cast<T*>(h->fgrad),
-                           static_cast<T*>(h->erho), vtq[0], vtq[1], vtq[2], virrq[0], virrq[1],
+                           h->fg_combined ? nullptr : static_cast<T*>(h->erho), vtq[0], vtq[1], vtq[2],
+                           virrq[0], virrq[1],
                            virrq[2], static_cast<T*>(h->tmp), h->a32(SMAISO_STAGE_VELSOLVE) != 0,
                            h->stream);
   return hipGetLastError();
@@ -1234,6 +1270,8 @@ int advance(smaiso_handle* owner, Lanes& L, bool use_graph, hipGraph_t* graph,
           h->laprl_zlines = h->laprl_in_forces();
           h->gpx_valid = h->gpx_in_psi_pass();
           h->lz_valid = h->lz_in_psi_pass();
+          h->fg_combined = forces_zpass_supported(h->mixed ? F32 : F64, h->g.nz) && !h->keep_virr &&
+                           h->fg_in_x_pass();
         }
       }
       HIPTRY(owner, hipGraphLaunch(*gexec, L.master));

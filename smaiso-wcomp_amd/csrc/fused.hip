@@ -633,7 +633,8 @@ __global__ __launch_bounds__(NT) void velocity_z_kernel(StepConsts c, VelocityIO
         const AV Sx = i > 0 ? (AV)io.S[0][idx - (long long)c.ny * N] : AV(0);
         const AV Sy = j > 0 ? (AV)io.S[1][idx - N] : AV(0);
         const AV Sz = k > 0 ? (AV)io.S[2][idx - 1] : AV(0);
-        const VelMode<AV> r = velocity_mode<AV>(c, mc, i, j, k, Sx, Sy, Sz, (AV)io.fgq[idx], (AV)io.erq[idx]);
+        const AV er = io.erq ? (AV)io.erq[idx] : AV(0);
+        const VelMode<AV> r = velocity_mode<AV>(c, mc, i, j, k, Sx, Sy, Sz, (AV)io.fgq[idx], er);
         v[0] = (A)r.tx;
         v[1] = (A)r.ty;
         v[2] = (A)r.tz;

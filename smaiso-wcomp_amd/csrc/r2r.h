@@ -186,6 +186,12 @@ hipError_t r2r_plain_pass_with_tables(DType dt, int kind, int axis, const PassIO
 // out2, both in out_dt, FP64 arithmetic; length 256 along x
 // (yx256_invgrad_supported).  Defined in r2r_yx256.hip.
 bool yx256_invgrad_supported(const Geom& g);
+// The x pass of the forward DCTs of fgrad and e rho at length 256 (FP64
+// fields, after their z and y passes) that stores fgrad^ + G e rho^ into out
+// (may alias fg): G = exp(-1.57^2 2 |q|^2) with |q|^2 from the spacings qs.
+bool yx256_fgx_supported(const Geom& g);
+hipError_t launch_yx256_fgx(const double* fg, const double* er, double* out, const double qs[3], const Geom& g,
+                            const void* tw, const void* mk, hipStream_t s);
 hipError_t launch_yx256_invgrad(DType out_dt, const double* in, void* out, void* out2, double qs, const Geom& g,
                                 const void* tw, const void* mk, hipStream_t s);
 

@@ -258,6 +258,94 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
   }
 }
 
+// The x pass of the forward transforms of fgrad and e rho (DCT-II along x)
+// that also forms fgrad^ + G e rho^ (S:1479-1488, G = exp(-1.57^2 2 |q|^2)
+// at mode (i, j, k), with the velocity solve's own expression), the one
+// combination of the two the solve reads.  A sequence packs the fgrad line
+// (real part) and the e rho line (imaginary part) at one (y, z) position, so
+// Makhoul's post-processing yields both transforms at mode i and the pass
+// stores one real value per point: P z-adjacent lines per tile of 16 P
+// threads (128-byte row segments per field at P = 16).  FP64 storage and
+// arithmetic; the next tile is loaded while the current one is transformed.
+template <int P>
+__global__ __launch_bounds__(16 * P, 2) void fgx_kernel(const double* fg, const double* __restrict__ er, double* out,
+                                                      const void* twv, const void* mkv,
+                                                      int ny, int nz, int ntiles, int tpo, double qs0, double qs1,
+                                                      double qs2) {
+  using A = double;
+  using C = cplx<A>;
+  constexpr int N = kN;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* const buf = reinterpret_cast<C*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int c = tid % P, j = tid / P;
+  C* const s = buf + c * kSeq;
+  const long long mo = (long long)blockIdx.y * ((long long)N * ny * nz);
+  const int stride = ny * nz;
+  const C* const tw = reinterpret_cast<const C*>(twv);
+  const C* const mkt = reinterpret_cast<const C*>(mkv);
+  const C wf = tw[j];
+  const C mk0 = mkt[j], mks = mkt[16];
+  // line (y index o, z index kz) of tile t and of this thread
+  auto line = [&](int t, int& o, int& kz) {
+    o = t / tpo;
+    kz = (t - o * tpo) * P + c;
+  };
+  auto load = [&](C* v, int t) {
+    int o, kz;
+    line(t, o, kz);
+    const long long b = mo + (long long)o * nz + kz;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int row = q < 8 ? 2 * j + 32 * q : 2 * N - 1 - 2 * j - 32 * q;
+      v[q] = C{fg[b + (long long)row * stride], er[b + (long long)row * stride]};
+    }
+  };
+  C v[16], nxt[16];
+  int t = blockIdx.x;
+  if (t < ntiles) load(nxt, t);
+  for (; t < ntiles; t += gridDim.x) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = nxt[q];
+    const int tn = t + gridDim.x;
+    if (tn < ntiles) load(nxt, tn);
+    C wf_t = wf, mk0_t = mk0, mks_t = mks;
+    int j_t = j;
+    asm volatile("" : "+v"(j_t), "+v"(wf_t.x), "+v"(wf_t.y));
+    asm volatile("" : "+v"(mk0_t.x), "+v"(mk0_t.y), "+v"(mks_t.x), "+v"(mks_t.y));
+    dft<A, 16, -1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[row_slot(j, q)] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = s[col_slot(j, q)];
+    stage_twiddle(v, wf_t);
+    dft<A, 16, -1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[col_slot(j, q)] = v[q];
+    __syncthreads();
+    int o, kz;
+    line(t, o, kz);
+    const A qy = o * qs1, qz = kz * qs2;
+    double* const po = out + mo + (long long)o * nz + kz;
+    // Y_k: fgrad^ (y0) and e rho^ (y1) at mode k along x
+    C m = mk0_t;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = j_t + 16 * q;
+      const C za = v[q], pa = s[partner_slot(j_t, q)];
+      const A y0 = m.x * (za.x + pa.x) - m.y * (za.y - pa.y);
+      const A y1 = m.x * (za.y + pa.y) + m.y * (za.x - pa.x);
+      const A qx = k * qs0;
+      const A q2 = qx * qx + qy * qy + qz * qz;
+      const A G = exp(A(-1.57 * 1.57 * 2) * q2);
+      po[(long long)k * stride] = y0 + y1 * G;
+      m = cmul(m, mks_t);
+    }
+    __syncthreads();  // the next tile's first writes reuse slots other threads read last
+  }
+}
+
 // One-axis operator passes (OP_AX_GRAD, OP_AX_DIV, OP_AX_GAUSS, OP_AX_GRAD_T,
 // OP_AX_DIV_T) of length 256 along y and x on the same layout: forward
 // transform, operator, inverse transform per tile with three LDS exchanges.

@@ -126,6 +126,24 @@ hipError_t launch_invgrad_t(const double* in, TO* out, TO* out2, double qs, cons
 
 }  // namespace
 
+bool yx256_fgx_supported(const Geom& g) {
+  return g.nx == yx256::kN && g.nz % 16 == 0 && (long long)(yx256::kN - 1) * g.ny * g.nz + g.nz <= 2147483647LL;
+}
+
+hipError_t launch_yx256_fgx(const double* fg, const double* er, double* out, const double qs[3], const Geom& g,
+                            const void* tw, const void* mk, hipStream_t s) {
+  constexpr int P = 16;
+  if (!yx256_fgx_supported(g)) return hipErrorNotSupported;
+  const int tpo = g.nz / P;
+  const int ntiles = g.ny * tpo;
+  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<double>);
+  const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+  const int grid = std::min(ntiles, 256 * per_cu * 2);
+  hipLaunchKernelGGL((yx256::fgx_kernel<P>), dim3((unsigned)grid, (unsigned)g.nb), dim3(16 * P), lds, s, fg, er,
+                     out, tw, mk, g.ny, g.nz, ntiles, tpo, qs[0], qs[1], qs[2]);
+  return hipGetLastError();
+}
+
 bool yx256_invgrad_supported(const Geom& g) {
   constexpr int P = yx256::tile_p<double>();
   return g.nx == yx256::kN && g.nz % (2 * P) == 0 && (g.n3() % 2) == 0 &&

@@ -155,7 +155,8 @@ __global__ __launch_bounds__(BS) void k_velocity(
   const A Sx = hx ? (A)sxq_[mo + idx - sI] : A(0);
   const A Sy = hy ? (A)syq_[mo + idx - sJ] : A(0);
   const A Sz = hz ? (A)szq_[mo + idx - 1] : A(0);
-  const VelMode<A> r = velocity_mode<A>(c, mm, p.i, p.j, p.k, Sx, Sy, Sz, (A)fgq_[mo + idx], (A)erq_[mo + idx]);
+  const A er = erq_ ? (A)erq_[mo + idx] : A(0);
+  const VelMode<A> r = velocity_mode<A>(c, mm, p.i, p.j, p.k, Sx, Sy, Sz, (A)fgq_[mo + idx], er);
   const long long o = mo + idx;
   const long long pox = hx ? o - sI : o + (long long)(c.nx - 1) * sI;
   const long long poy = hy ? o - sJ : o + (long long)(c.ny - 1) * sJ;
