@@ -265,8 +265,11 @@ __global__ __launch_bounds__(16 * P, (min_waves<T, MODE, A>())) void pass_kernel
 // (real part) and the e rho line (imaginary part) at one (y, z) position, so
 // Makhoul's post-processing yields both transforms at mode i and the pass
 // stores one real value per point: P z-adjacent lines per tile of 16 P
-// threads (128-byte row segments per field at P = 16).  FP64 storage and
-// arithmetic; the next tile is loaded while the current one is transformed.
+// threads (128-byte row segments per field at P = 16).  G is taken as the
+// product of the x factor exp(-c qx^2) (a table of the 256 modes in LDS,
+// made once per workgroup) and exp(-c (qy^2 + qz^2)) (once per line), within
+// a few ulp of the exponential of the sum.  FP64 storage and arithmetic;
+// the next tile is loaded while the current one is transformed.
 template <int P>
 __global__ __launch_bounds__(16 * P, 2) void fgx_kernel(const double* fg, const double* __restrict__ er, double* out,
                                                       const void* twv, const void* mkv,
@@ -286,6 +289,13 @@ __global__ __launch_bounds__(16 * P, 2) void fgx_kernel(const double* fg, const 
   const C* const mkt = reinterpret_cast<const C*>(mkv);
   const C wf = tw[j];
   const C mk0 = mkt[j], mks = mkt[16];
+  constexpr A kG = A(-1.57 * 1.57 * 2);
+  A* const gx = reinterpret_cast<A*>(buf + P * kSeq);  // exp(-c qx^2) of the N modes
+  for (int i = tid; i < N; i += 16 * P) {
+    const A qx = i * qs0;
+    gx[i] = exp(kG * (qx * qx));
+  }
+  __syncthreads();
   // line (y index o, z index kz) of tile t and of this thread
   auto line = [&](int t, int& o, int& kz) {
     o = t / tpo;
@@ -327,6 +337,7 @@ __global__ __launch_bounds__(16 * P, 2) void fgx_kernel(const double* fg, const 
     int o, kz;
     line(t, o, kz);
     const A qy = o * qs1, qz = kz * qs2;
+    const A gyz = exp(kG * (qy * qy + qz * qz));
     double* const po = out + mo + (long long)o * nz + kz;
     // Y_k: fgrad^ (y0) and e rho^ (y1) at mode k along x
     C m = mk0_t;
@@ -336,10 +347,7 @@ __global__ __launch_bounds__(16 * P, 2) void fgx_kernel(const double* fg, const 
       const C za = v[q], pa = s[partner_slot(j_t, q)];
       const A y0 = m.x * (za.x + pa.x) - m.y * (za.y - pa.y);
       const A y1 = m.x * (za.y + pa.y) + m.y * (za.x - pa.x);
-      const A qx = k * qs0;
-      const A q2 = qx * qx + qy * qy + qz * qz;
-      const A G = exp(A(-1.57 * 1.57 * 2) * q2);
-      po[(long long)k * stride] = y0 + y1 * G;
+      po[(long long)k * stride] = y0 + y1 * (gx[k] * gyz);
       m = cmul(m, mks_t);
     }
     __syncthreads();  // the next tile's first writes reuse slots other threads read last

@@ -136,7 +136,7 @@ hipError_t launch_yx256_fgx(const double* fg, const double* er, double* out, con
   if (!yx256_fgx_supported(g)) return hipErrorNotSupported;
   const int tpo = g.nz / P;
   const int ntiles = g.ny * tpo;
-  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<double>);
+  const size_t lds = (size_t)P * yx256::kSeq * sizeof(cplx<double>) + yx256::kN * sizeof(double);
   const int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
   const int grid = std::min(ntiles, 256 * per_cu * 2);
   hipLaunchKernelGGL((yx256::fgx_kernel<P>), dim3((unsigned)grid, (unsigned)g.nb), dim3(16 * P), lds, s, fg, er,
